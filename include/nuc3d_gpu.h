@@ -1,0 +1,138 @@
+/*
+ * nuc3d_gpu.h -- C ABI of the MI355X (gfx950) implementation of NUC3d's
+ * per-stage right-hand side and TVD-RK3 time advance.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every call returns an int
+ * status (NUC_OK on success) and never exits the process; nuc_last_error()
+ * gives the message of the last failure on a context.
+ *
+ * Layout contract (src/field.cpp:96-100): each scalar field is an FP64 array
+ * with x fastest, idx = i + nx*(j + ny*k).  A conserved state is five such
+ * arrays back to back: rho, rho*u, rho*v, rho*w, rho*E.  Reconstructed
+ * interface arrays hold n1+1 values per line, out[m] being the interface
+ * between cells m-1 and m (src/weno5js.cpp:108).  Ghost strips are
+ * (bf, n2, n3) fields: left ghost g <-> cell g-bf, right ghost g <-> cell n1+g
+ * (src/bufferData.cpp:56-192).
+ */
+#ifndef NUC3D_GPU_H
+#define NUC3D_GPU_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nuc_status {
+    NUC_OK = 0,
+    NUC_EINVAL = 1,       /* bad argument or unsupported configuration */
+    NUC_ENOMEM = 2,       /* device allocation failed */
+    NUC_EHIP = 3,         /* a HIP runtime call failed */
+    NUC_ENONFINITE = 4,   /* state became NaN/Inf */
+    NUC_ENODEV = 5        /* no usable gfx950 device */
+};
+
+/* inviscid schemes, names as accepted in inp/Method.in (src/fieldOperator.cpp:128-220) */
+enum nuc_scheme { NUC_WENO5JS = 0, NUC_WENO5Z = 1, NUC_WENO7JS = 2, NUC_CRWENO5 = 3, NUC_HCCS = 4 };
+/* viscous first-derivative schemes (src/fieldOperator.cpp:298-345) */
+enum nuc_vis_scheme { NUC_CD6 = 0, NUC_CD8 = 1 };
+/* flux-vector splitting (src/physicsModel.cpp:16-20) */
+enum nuc_riemann { NUC_LF = 0, NUC_AUSM = 1 };
+/* PDE model (src/block.cpp:146-157) */
+enum nuc_model { NUC_EULER3D = 0, NUC_NAVIERSTOKES3D = 1 };
+/* viscosity law (src/physicsModel.cpp:45-49) */
+enum nuc_viscosity { NUC_SUTHERLAND = 0, NUC_CONSTANT = 1 };
+/* forcing (src/physicsModel.cpp:50-56) */
+enum nuc_forcing { NUC_FORCING_NONE = 0, NUC_FORCING_CONST = 1 };
+/* arithmetic of the stencils; the conserved state is FP64 in both */
+enum nuc_precision { NUC_FP64 = 0, NUC_MIXED = 1 };
+
+typedef struct nuc_config {
+    int nx, ny, nz;           /* global grid (inp/block.in line 2) */
+    int npx, npy, npz;        /* rank grid the reference would run on (inp/block.in line 1);
+                                 it places the per-rank closures of the schemes */
+    double lx, ly, lz;        /* box lengths (inp/block.in line 3) */
+    int scheme[3];            /* enum nuc_scheme per direction */
+    int vis_scheme[3];        /* enum nuc_vis_scheme per direction */
+    int riemann, model, viscosity, forcing, precision;
+    double gamma, mach, reynolds, prandtl, t_ref, t_inf;
+    double forcing_amp, forcing_amp_hel, forcing_k;
+    double cfl;
+} nuc_config;
+
+typedef struct nuc_step_info {
+    double dt;      /* dt_global of the last step (src/PDEData3d.cpp:76-80) */
+    double time;    /* accumulated time (src/block.cpp:208-209) */
+    double res;     /* residual of the last step (src/PDEData3d.cpp:93-128) */
+    long step;      /* steps taken on this context */
+} nuc_step_info;
+
+typedef struct nuc_ctx nuc_ctx;
+
+/* ---- context --------------------------------------------------------------
+ * Replaces the per-rank state the reference's block owns (src/block.cpp:143-187):
+ * PDEData3d (Q, Q_Euler, Q_work, RHS), EulerData3D/NaiverStokesData3d work
+ * fields and the halo buffers.  All device memory is owned by the context. */
+int nuc_create(const nuc_config *cfg, nuc_ctx **out);
+int nuc_destroy(nuc_ctx *ctx);
+const char *nuc_last_error(const nuc_ctx *ctx);
+/* bytes of device memory a context of this configuration allocates */
+long long nuc_footprint(const nuc_config *cfg);
+
+/* ---- state ---------------------------------------------------------------
+ * q: 5*nx*ny*nz host doubles (the restart payload of src/block.cpp:640-671). */
+int nuc_upload_state(nuc_ctx *ctx, const double *q);
+int nuc_download_state(nuc_ctx *ctx, double *q);
+
+/* ---- hot path ------------------------------------------------------------
+ * nuc_rhs: one right-hand-side evaluation of the current state, i.e.
+ * NaiverStokesData3d::solve / EulerData3D::solve (src/NaiverStokes3d.cpp:37-73,
+ * src/euler3d.cpp:260-277) with the on-device periodic wrap in place of
+ * MPIComunicator.  rhs_host (5N, may be NULL) receives RHS with dQ/dt = -RHS;
+ * dt_local (may be NULL) the min over ranks of getDt (src/euler3d.cpp:459-466). */
+int nuc_rhs(nuc_ctx *ctx, double *rhs_host, double *dt_local);
+/* nsteps TVD-RK3 steps: block::solve (src/block.cpp:191-215), i.e. three
+ * (RHS + PDEData3d::solve) stages each.  info may be NULL. */
+int nuc_step(nuc_ctx *ctx, int nsteps, nuc_step_info *info);
+/* enqueue nsteps steps without waiting (timing); nuc_sync waits */
+int nuc_step_async(nuc_ctx *ctx, int nsteps);
+int nuc_sync(nuc_ctx *ctx, nuc_step_info *info);
+/* Global statistics of postproc::solveQ (src/postproc.cpp:736-929), FP64:
+ * out[0..15] means of ux,uy,uz,vx,vy,vz,wx,wy,wz, omega_x,y,z, enstrophy, Q,
+ * helicity, kinetic energy; out[16..24] u_rms, M_t, sqrt(<(du_i/dx_i)^2>/3),
+ * mu_mean, Re_lambda, lambda, rho_mean, tau_t, skewness. */
+int nuc_stats(nuc_ctx *ctx, double *out25);
+
+/* ---- operator hooks (reference plug-in API, one Field per call) -----------
+ * These mirror interoplation::interpolationInner/BoundaryL/BoundaryR
+ * (src/fieldOperator.h:13-37), differential::differential* (:58-81) and
+ * integration::integrationAll (:39-56) on host arrays, running the same
+ * device code as the fused sweeps.  zones is a bit mask: 1 = inner
+ * interfaces/cells (interpolationInner), 2 = left ghost-aware edge
+ * (interpolationBoundaryL), 4 = right edge (interpolationBoundaryR); 7 = all.
+ * gL may be NULL unless bit 2 is set, gR unless bit 4 is set. */
+int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const double *in, int n1, int n2, int n3,
+                       const double *gL, const double *gR, int bf, double *out, int zones);
+int nuc_op_diff(int vis_scheme, int tilesize, const double *in, double h, int n1, int n2, int n3,
+                const double *gL, const double *gR, int bf, double *out, int zones);
+int nuc_op_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n);
+
+/* ---- configuration -------------------------------------------------------
+ * Fill cfg from a directory holding the reference's inp files
+ * (inp/block.in, inp/Method.in, inp/PhysModel.in, inp/IOController.in) with the
+ * reference's parsing rules, including the positional first six pairs of
+ * PhysModel.in (src/physicsModel.cpp:72-77).  start/end steps and cfl come
+ * from IOController.in (src/IOcontroller.cpp:12-110). */
+int nuc_config_from_inp(const char *dir, nuc_config *cfg, int *start_step, int *end_step);
+
+/* ---- host-side initial conditions (5N doubles into q) ---------------------
+ * nuc_init_tgv: Taylor-Green vortex of initial_taylorgreen (src/block.cpp:359-395),
+ *   cell centres placed per rank as getXYZ does (src/block.cpp:103-140).
+ * nuc_init_hit: random-phase solenoidal velocity with
+ *   E(k) ~ (k/k0)^4 exp(-2 (k/k0)^2) as initial_hit (src/block.cpp:397-551), but
+ *   with phases drawn from mt19937_64(seed) instead of std::random_device and
+ *   rescaled to the given u_rms; rho = 1, p = 1/(gamma Mach^2). */
+int nuc_init_tgv(const nuc_config *cfg, double *q);
+int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsigned long long seed, double *q);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

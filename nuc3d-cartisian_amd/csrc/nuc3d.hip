@@ -1,0 +1,1093 @@
+// nuc3d.hip -- gfx950 kernels and the C ABI (include/nuc3d_gpu.h) for NUC3d's
+// per-stage right-hand side and TVD-RK3 advance.
+//
+// Per RK stage (FP64 path):
+//   k_prim   : con2prim of the stage input; at stage 0 the per-rank maximum
+//              eigenvalues (src/physicsModel.cpp:546-654, LF z-into-eta quirk kept)
+//   k_dt     : dt_local = min over ranks of getDt (src/euler3d.cpp:459-466), dt_global = cfl*dt_local
+//   k_grad   : (Navier-Stokes) gradients of u,v,w,T with cd6/cd8, Sutherland
+//              viscosity, stress and heat flux (src/NaiverStokes3d.cpp:87-303);
+//              writes the 9 distinct viscous flux components
+//   k_sweep  : one kernel per direction.  Each workgroup holds LPB whole lines
+//              in LDS; per variable it splits the flux, reconstructs f^+ and
+//              f^- (explicit WENO or compact tridiagonal solved per wave),
+//              differences the interface flux and the viscous flux and
+//              accumulates the RHS.  The periodic halo of MPIComunicator is an
+//              index wrap inside the line buffer.  The z sweep ends with the
+//              TVD-RK3 update (src/TVD-RK3.cpp:37-157) and the residual.
+#include "nuc3d_gpu.h"
+#include "nuc_device.h"
+#include "nuc_lines.h"
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+using namespace nucdev;
+
+namespace {
+
+constexpr int MAXRANK = 64;
+
+// ---------------------------------------------------------------------------
+// kernel arguments
+// ---------------------------------------------------------------------------
+struct SweepArgs {
+    int dir, n, nloc, ts, tsv, vis, riemann;
+    int nx, ny, nz;
+    long nlines;
+    double inv_h, gamma;
+    const double *q[5];
+    const double *fv[4];
+    double *rhs[5];
+    int stage, epilogue;
+    const double *qn[5];
+    double *qout[5];
+    const double *dtp;
+    double *res_part;
+};
+
+struct PrimArgs {
+    const double *q[5];
+    double *prim[4];
+    long N;
+    int nx, ny, nlx, nly, nlz, npx, npy, P;
+    int riemann, do_lam, write_prim;
+    double gamma, mach;
+    unsigned long long *lam;
+};
+
+struct GradArgs {
+    const double *prim[4];
+    double *out[9];
+    int nx, ny, nz;
+    int nloc[3], tsv[3], vis[3];
+    double inv_h[3];
+    int law;
+    double re, mach, pr, gamma, tref, tinf;
+};
+
+__device__ __forceinline__ double bits2d(unsigned long long b) { return __longlong_as_double((long long)b); }
+__device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigned long long)__double_as_longlong(d); }
+
+// ---------------------------------------------------------------------------
+// con2prim (src/physicsModel.cpp:331-344) and eigenvalue maxima
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_prim(PrimArgs A)
+{
+    __shared__ unsigned long long sl[3 * MAXRANK];
+    const int t = threadIdx.x;
+    if (A.do_lam) {
+        for (int i = t; i < 3 * A.P; i += 256) sl[i] = 0ull;
+        __syncthreads();
+    }
+    int cur = -1;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    const double gm1 = A.gamma - 1.0;
+    for (long id = (long)blockIdx.x * 256 + t; id < A.N; id += (long)gridDim.x * 256) {
+        double rho = A.q[0][id];
+        double ir = 1.0 / rho;
+        double u = A.q[1][id] * ir, v = A.q[2][id] * ir, w = A.q[3][id] * ir;
+        double E = A.q[4][id];
+        double p = fabs((E - 0.5 * rho * (u * u + v * v + w * w)) * gm1);
+        if (A.write_prim) {
+            A.prim[0][id] = u;
+            A.prim[1][id] = v;
+            A.prim[2][id] = w;
+            A.prim[3][id] = A.gamma * A.mach * A.mach * p * ir;
+        }
+        if (A.do_lam) {
+            double c = sqrt(A.gamma * p * ir);
+            long i = id % A.nx, r2 = id / A.nx;
+            long j = r2 % A.ny, k = r2 / A.ny;
+            int rank = (int)(i / A.nlx) + A.npx * ((int)(j / A.nly) + A.npy * (int)(k / A.nlz));
+            if (rank != cur) {
+                if (cur >= 0) {
+                    atomicMax(&sl[3 * cur + 0], d2bits(m0));
+                    atomicMax(&sl[3 * cur + 1], d2bits(m1));
+                    atomicMax(&sl[3 * cur + 2], d2bits(m2));
+                }
+                cur = rank; m0 = m1 = m2 = 0.0;
+            }
+            double lx = fabs(u) + c, ly = fabs(v) + c, lz = fabs(w) + c;
+            m0 = fmax(lx, m0);
+            m1 = fmax(ly, m1);
+            if (A.riemann == RIE_LF) m1 = fmax(lz, m1);  // src/physicsModel.cpp:653-654
+            else m2 = fmax(lz, m2);
+        }
+    }
+    if (A.do_lam) {
+        if (cur >= 0) {
+            atomicMax(&sl[3 * cur + 0], d2bits(m0));
+            atomicMax(&sl[3 * cur + 1], d2bits(m1));
+            atomicMax(&sl[3 * cur + 2], d2bits(m2));
+        }
+        __syncthreads();
+        for (int i = t; i < 3 * A.P; i += 256)
+            if (sl[i]) atomicMax(&A.lam[i], sl[i]);
+    }
+}
+
+// dt per rank and min over ranks; resets the eigenvalue slots for the next step
+__global__ void k_dt(unsigned long long *lam, int P, double hx, double hy, double hz, double cfl, int set_global,
+                     double *scal)
+{
+    if (threadIdx.x == 0) {
+        double dtmin = 0.0;
+        for (int r = 0; r < P; r++) {
+            double dt_xi = hx / bits2d(lam[3 * r + 0]);
+            double dt_eta = hy / bits2d(lam[3 * r + 1]);
+            double dt_zeta = hz / bits2d(lam[3 * r + 2]);
+            double dt = 1.0 / (1.0 / dt_xi + 1.0 / dt_eta + 1.0 / dt_zeta);
+            if (r == 0 || dt < dtmin) dtmin = dt;
+        }
+        scal[3] = dtmin;
+        if (set_global) scal[0] = dtmin * cfl;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * P; i += blockDim.x) lam[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// gradients + viscous fluxes
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+__global__ __launch_bounds__(256) void k_grad(GradArgs A)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    if (i >= A.nx || j >= A.ny) return;
+    const long sx = 1, sy = A.nx, sz = (long)A.nx * A.ny;
+    const long id = i + sy * j + sz * k;
+    double g[4][3];
+    const int pos[3] = {i, j, k};
+    const int ext[3] = {A.nx, A.ny, A.nz};
+    const long str[3] = {sx, sy, sz};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        long offs[9];
+#pragma unroll
+        for (int o = -4; o <= 4; o++) {
+            int p = wrapi(pos[d] + o, ext[d]);
+            offs[o + 4] = id + (long)(p - pos[d]) * str[d];
+        }
+        int cl = pos[d] % A.nloc[d];
+#pragma unroll
+        for (int f = 0; f < 4; f++) {
+            double L[9];
+#pragma unroll
+            for (int o = 0; o < 9; o++) L[o] = A.prim[f][offs[o]];
+            g[f][d] = central_diff<double>(L + 4, 0, cl, A.nloc[d], A.tsv[d], A.vis[d], A.inv_h[d]);
+        }
+    }
+    double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+    double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+    double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+    double tx = g[3][0], ty = g[3][1], tz = g[3][2];
+    double T = A.prim[3][id];
+    double miu0;
+    double S = A.tref / A.tinf;
+    if (A.law == 0) miu0 = (1.0 + S) / (T + S) * (T * sqrt(T)) / A.re;   // src/physicsModel.cpp:220
+    else miu0 = 1.0 / A.re;
+    double coeff0 = miu0 / ((A.gamma - 1.0) * A.mach * A.mach * A.pr);
+    double div = ux + vy + wz;
+    // src/NaiverStokes3d.cpp:265-283 (off-diagonals carry -2/3 div u as coded)
+    double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * div);
+    double txy = miu0 * (uy + vx - 2.0 / 3.0 * div);
+    double txz = miu0 * (uz + wx - 2.0 / 3.0 * div);
+    double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * div);
+    double tyz = miu0 * (vz + wy - 2.0 / 3.0 * div);
+    double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * div);
+    double u = A.prim[0][id], v = A.prim[1][id], w = A.prim[2][id];
+    A.out[0][id] = txx;
+    A.out[1][id] = tyy;
+    A.out[2][id] = tzz;
+    A.out[3][id] = txy;
+    A.out[4][id] = txz;
+    A.out[5][id] = tyz;
+    A.out[6][id] = u * txx + v * txy + w * txz + coeff0 * tx;
+    A.out[7][id] = u * txy + v * tyy + w * tyz + coeff0 * ty;
+    A.out[8][id] = u * txz + v * tyz + w * tzz + coeff0 * tz;
+}
+
+// ---------------------------------------------------------------------------
+// the direction sweep
+// ---------------------------------------------------------------------------
+template <int SCH, int CPT, int LPB>
+__global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
+{
+    constexpr int NP = 64 * CPT + 2 * HALO;
+    __shared__ double sP[LPB * NP];
+    __shared__ double sM[LPB * NP];
+    __shared__ double sRed[LPB];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const int n = A.n;
+    int ll, g;
+    if (A.dir == 0) { ll = wave; g = lane; }
+    else { ll = t % LPB; g = t / LPB; }
+    const long L = (long)blockIdx.x * LPB + ll;
+    const bool lv = L < A.nlines;
+    long base = 0, stride = 1;
+    if (A.dir == 0) { base = L * A.nx; stride = 1; }
+    else if (A.dir == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
+    else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
+    const int d = A.dir;
+    const double gm1 = A.gamma - 1.0;
+
+    double q[CPT][5], s[CPT][5], rhs[CPT][5];
+#pragma unroll
+    for (int kk = 0; kk < CPT; kk++) {
+        int c = g + 64 * kk;
+        bool ok = lv && c < n;
+        long id = base + (long)c * stride;
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+            q[kk][v] = ok ? A.q[v][id] : 1.0;
+            rhs[kk][v] = (ok && d > 0) ? A.rhs[v][id] : 0.0;
+        }
+        double rho = q[kk][0], ir = 1.0 / rho;
+        double u = q[kk][1] * ir, vv = q[kk][2] * ir, w = q[kk][3] * ir;
+        double p = fabs((q[kk][4] - 0.5 * rho * (u * u + vv * vv + w * w)) * gm1);
+        double cs = sqrt(A.gamma * p * ir);
+        double ud = (d == 0) ? u : (d == 1 ? vv : w);
+        if (A.riemann == RIE_LF) {
+            s[kk][0] = ud; s[kk][1] = fabs(ud) + cs; s[kk][2] = p; s[kk][3] = 0.0; s[kk][4] = 0.0;
+        } else {
+            // AUSM splittings, src/physicsModel.cpp:1181-1228
+            double mach = ud / cs;
+            double mp, mn, pp, pn;
+            if (fabs(mach) < 1.0) {
+                mp = 0.25 * sq(mach + 1.0) + 0.125 * sq(mach * mach - 1.0);
+                mn = -0.25 * sq(mach - 1.0) - 0.125 * sq(mach * mach - 1.0);
+                pp = p * (mach + 1.0) / 2.0;
+                pn = -p * (mach - 1.0) / 2.0;
+            } else {
+                mp = 0.5 * (mach + fabs(mach));
+                mn = 0.5 * (mach - fabs(mach));
+                pp = 0.5 * p * (mach + fabs(mach)) / mach;
+                pn = 0.5 * p * (mach - fabs(mach)) / mach;
+            }
+            s[kk][0] = mp * cs; s[kk][1] = mn * cs; s[kk][2] = pp; s[kk][3] = pn; s[kk][4] = p;
+        }
+    }
+
+    double *Pl = sP + ll * NP + HALO;   // load-mapping view
+    double *Ml = sM + ll * NP + HALO;
+    const double *Pw = sP + wave * NP + HALO;  // compute-mapping view (wave = line)
+    const double *Mw = sM + wave * NP + HALO;
+    const bool wv = (long)blockIdx.x * LPB + wave < A.nlines;
+    const Zone z{n, A.nloc, A.ts};
+
+#pragma unroll 1
+    for (int v = 0; v < 5; v++) {
+        // split fluxes f+/f- of variable v at the own cells (src/physicsModel.cpp:540-565, 790-820)
+#pragma unroll
+        for (int kk = 0; kk < CPT; kk++) {
+            int c = g + 64 * kk;
+            if (lv && c < n) {
+                double qv = q[kk][v], fp, fm;
+                if (A.riemann == RIE_LF) {
+                    double ud = s[kk][0], lam = s[kk][1], p = s[kk][2];
+                    double F = ud * qv;
+                    if (v == 1 + d) F = ud * qv + p;
+                    if (v == 4) F = ud * (qv + p);
+                    fp = 0.5 * (F + lam * qv);
+                    fm = 0.5 * (F - lam * qv);
+                } else {
+                    double mpa = s[kk][0], mna = s[kk][1];
+                    fp = mpa * qv;
+                    fm = mna * qv;
+                    if (v == 1 + d) { fp += s[kk][2]; fm += s[kk][3]; }
+                    if (v == 4) { fp = mpa * qv + mpa * s[kk][4]; fm = mna * qv + mna * s[kk][4]; }
+                }
+                Pl[c] = fp; Ml[c] = fm;
+                if (c < HALO) { Pl[c + n] = fp; Ml[c + n] = fm; }
+                if (c >= n - HALO) { Pl[c - n] = fp; Ml[c - n] = fm; }
+            }
+        }
+        __syncthreads();
+        double Fp[CPT], Fm[CPT];
+        if (wv) line_interfaces<SCH, CPT, double>(Pw, Mw, z, lane, Fp, Fm);
+        __syncthreads();
+        if (wv) {
+            double *Pwr = sP + wave * NP + HALO;
+#pragma unroll
+            for (int r = 0; r < CPT; r++) {
+                int m = lane * CPT + r;
+                if (m < n) {
+                    double F = Fp[r] + Fm[r];
+                    Pwr[m] = F;
+                    if (m == 0) Pwr[n] = F;
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < CPT; kk++) {
+            int c = g + 64 * kk;
+            if (lv && c < n) rhs[kk][v] += (Pl[c + 1] - Pl[c]) * A.inv_h;
+        }
+        __syncthreads();
+    }
+
+    if (A.vis != VIS_NONE) {
+        // -D_dir of the viscous flux components 1..4 (src/NaiverStokes3d.cpp:305-369)
+#pragma unroll 1
+        for (int v = 1; v < 5; v++) {
+            const double *fv = A.fv[v - 1];
+#pragma unroll
+            for (int kk = 0; kk < CPT; kk++) {
+                int c = g + 64 * kk;
+                if (lv && c < n) {
+                    double x = fv[base + (long)c * stride];
+                    Pl[c] = x;
+                    if (c < HALO) Pl[c + n] = x;
+                    if (c >= n - HALO) Pl[c - n] = x;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < CPT; kk++) {
+                int c = g + 64 * kk;
+                if (lv && c < n)
+                    rhs[kk][v] -= central_diff<double>(Pl, c, c % A.nloc, A.nloc, A.tsv, A.vis, A.inv_h);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (!A.epilogue) {
+#pragma unroll
+        for (int kk = 0; kk < CPT; kk++) {
+            int c = g + 64 * kk;
+            if (lv && c < n) {
+                long id = base + (long)c * stride;
+#pragma unroll
+                for (int v = 0; v < 5; v++) A.rhs[v][id] = rhs[kk][v];
+            }
+        }
+        return;
+    }
+    // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (stage 0), :97-101 (1), :140-144 (2)
+    const double dt = *A.dtp;
+    double res = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < CPT; kk++) {
+        int c = g + 64 * kk;
+        if (lv && c < n) {
+            long id = base + (long)c * stride;
+#pragma unroll
+            for (int v = 0; v < 5; v++) {
+                double rr = -rhs[kk][v];
+                double nu;
+                if (A.stage == 0) {
+                    nu = q[kk][v] + rr * dt;
+                } else {
+                    double un = A.qn[v][id];
+                    if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + q[kk][v]);
+                    else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + q[kk][v]);
+                    if (A.stage == 2) res += sq(nu - un);
+                }
+                A.qout[v][id] = nu;
+            }
+        }
+    }
+    if (A.stage == 2) {
+        // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) res += __shfl_down(res, o);
+        if (lane == 0) sRed[wave] = res;
+        __syncthreads();
+        if (t == 0) {
+            double acc = 0.0;
+            for (int w = 0; w < LPB; w++) acc += sRed[w];
+            A.res_part[blockIdx.x] = acc;
+        }
+    }
+}
+
+__global__ void k_res(const double *part, int nb, double inv_count, double *scal)
+{
+    __shared__ double sm[256];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) acc += part[i];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        scal[1] = sqrt(sm[0] * inv_count) / scal[0];
+        scal[2] += scal[0];  // time += dt (src/block.cpp:208-209)
+    }
+}
+
+// ---------------------------------------------------------------------------
+// statistics (postproc::solveQ set), per-block partial sums then a fixed-order reduce
+// ---------------------------------------------------------------------------
+constexpr int NSTAT = 23;
+struct StatArgs {
+    const double *prim[4];   // u v w T
+    const double *rho;
+    int nx, ny, nz;
+    int nloc[3], tsv[3], vis[3];
+    double inv_h[3];
+    double gamma, mach, tref, tinf;
+    double *part;
+};
+
+__global__ __launch_bounds__(256) void k_stats(StatArgs A)
+{
+    __shared__ double sm[NSTAT][4];
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    const int j = blockIdx.y * 4 + threadIdx.y;
+    const int k = blockIdx.z;
+    double acc[NSTAT];
+    for (int s = 0; s < NSTAT; s++) acc[s] = 0.0;
+    if (i < A.nx && j < A.ny) {
+        const long sy = A.nx, sz = (long)A.nx * A.ny;
+        const long id = i + sy * j + sz * k;
+        const int pos[3] = {i, j, k};
+        const int ext[3] = {A.nx, A.ny, A.nz};
+        const long str[3] = {1, sy, sz};
+        double g[3][3];
+        for (int d = 0; d < 3; d++) {
+            int cl = pos[d] % A.nloc[d];
+            for (int f = 0; f < 3; f++) {
+                double L[9];
+                for (int o = -4; o <= 4; o++) {
+                    int p = wrapi(pos[d] + o, ext[d]);
+                    L[o + 4] = A.prim[f][id + (long)(p - pos[d]) * str[d]];
+                }
+                g[f][d] = central_diff<double>(L + 4, 0, cl, A.nloc[d], A.tsv[d], A.vis[d], A.inv_h[d]);
+            }
+        }
+        double ux = g[0][0], uy = g[0][1], uz = g[0][2];
+        double vx = g[1][0], vy = g[1][1], vz = g[1][2];
+        double wx = g[2][0], wy = g[2][1], wz = g[2][2];
+        double u0 = A.prim[0][id], u1 = A.prim[1][id], u2 = A.prim[2][id], T = A.prim[3][id];
+        double rho = A.rho[id];
+        double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
+        double q0 = ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy;
+        // sound speed from T: c^2 = gamma p/rho = T/M^2
+        double alpha = sqrt(T) / A.mach;
+        double S = A.tref / A.tinf;
+        double vals[NSTAT] = {ux, uy, uz, vx, vy, vz, wx, wy, wz, w0, w1, w2,
+                              (w0 * w0 + w1 * w1 + w2 * w2) * 0.5, q0, w0 * u0 + w1 * u1 + w2 * u2,
+                              0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2),
+                              (u0 * u0 + u1 * u1 + u2 * u2) / 3.0,
+                              (ux * ux + vy * vy + wz * wz) / 3.0,
+                              (ux * ux * ux + vy * vy * vy + wz * wz * wz) / 3.0,
+                              alpha,
+                              (1.0 + S) / (T + S) * (T * sqrt(T)),
+                              rho, 0.0};
+        for (int s = 0; s < NSTAT; s++) acc[s] = vals[s];
+    }
+    const int t = threadIdx.x + 64 * threadIdx.y, lane = t & 63, wave = t >> 6;
+    for (int s = 0; s < NSTAT; s++) {
+        double x = acc[s];
+        for (int o = 32; o > 0; o >>= 1) x += __shfl_down(x, o);
+        if (lane == 0) sm[s][wave] = x;
+    }
+    __syncthreads();
+    if (t < NSTAT) {
+        long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        A.part[b * NSTAT + t] = ((sm[t][0] + sm[t][1]) + sm[t][2]) + sm[t][3];
+    }
+}
+
+__global__ void k_stats_reduce(const double *part, long nb, double *out)
+{
+    // one thread per statistic: fixed sequential order over blocks
+    int s = threadIdx.x;
+    if (s >= NSTAT) return;
+    double acc = 0.0;
+    for (long b = 0; b < nb; b++) acc += part[b * NSTAT + s];
+    out[s] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// operator hooks: one wave per line of a Field with explicit ghost strips
+// ---------------------------------------------------------------------------
+struct OpRecArgs {
+    const double *in, *gL, *gR;
+    double *out;
+    int n1, nlines, bf, ts, upwind, zones;
+};
+
+template <int SCH, int CPT>
+__global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
+{
+    constexpr int NP = 64 * CPT + 2 * HALO;
+    __shared__ double sL[NP];
+    const int lane = threadIdx.x;
+    const long line = blockIdx.x;
+    const int n = A.n1;
+    double *L = sL + HALO;
+    for (int c = lane - HALO; c < 64 * CPT + HALO; c += 64) {
+        double x = 0.0;
+        if (c >= 0 && c < n) x = A.in[line * n + c];
+        else if (c < 0 && A.gL && -c <= A.bf) x = A.gL[line * A.bf + (A.bf + c)];
+        else if (c >= n && A.gR && c - n < A.bf && c < n + HALO) x = A.gR[line * A.bf + (c - n)];
+        L[c] = x;
+    }
+    __syncthreads();
+    double Fp[CPT], Fm[CPT];
+    const Zone z{n + 1, n, A.ts};
+    line_interfaces<SCH, CPT, double>(L, L, z, lane, Fp, Fm);
+    for (int r = 0; r < CPT; r++) {
+        int m = lane * CPT + r;
+        if (m > n) continue;
+        int zbit = (m < A.ts) ? 2 : (m > n - A.ts ? 4 : 1);
+        if (A.zones & zbit) A.out[line * (n + 1) + m] = (A.upwind > 0) ? Fp[r] : Fm[r];
+    }
+}
+
+struct OpDiffArgs {
+    const double *in, *gL, *gR;
+    double *out;
+    int n1, bf, ts, vis, zones;
+    long nlines;
+    double inv_h;
+};
+
+__global__ void k_op_diff(OpDiffArgs A)
+{
+    long gid = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= A.nlines * A.n1) return;
+    long line = gid / A.n1;
+    int c = (int)(gid % A.n1);
+    int n = A.n1;
+    int zbit = (c < A.ts) ? 2 : (c >= n - A.ts ? 4 : 1);
+    if (!(A.zones & zbit)) return;
+    double Lb[9];
+    for (int o = -4; o <= 4; o++) {
+        int cc = c + o;
+        double x = 0.0;
+        if (cc >= 0 && cc < n) x = A.in[line * n + cc];
+        else if (cc < 0 && A.gL && -cc <= A.bf) x = A.gL[line * A.bf + (A.bf + cc)];
+        else if (cc >= n && A.gR && cc - n < A.bf) x = A.gR[line * A.bf + (cc - n)];
+        Lb[o + 4] = x;
+    }
+    A.out[gid] = central_diff<double>(Lb + 4, 0, c, n, A.ts, A.vis, A.inv_h);
+}
+
+__global__ void k_op_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n)
+{
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double rr = -rhs[i];
+    if (stage == 0) ui[i] = ui[i] + rr * dt;
+    else if (stage == 1) ui[i] = 0.75 * un[i] + 0.25 * (rr * dt + ui[i]);
+    else ui[i] = (1.0 / 3.0) * un[i] + (2.0 / 3.0) * (rr * dt + ui[i]);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch helpers
+// ---------------------------------------------------------------------------
+int cpt_for(int n)
+{
+    int c = (n + 63) / 64;
+    if (c <= 4) return c;
+    if (c <= 6) return 6;
+    if (c <= 8) return 8;
+    return -1;
+}
+
+template <int SCH, int CPT>
+void launch_sweep_cpt(const SweepArgs &A, hipStream_t s)
+{
+    constexpr int LPB = (CPT <= 4) ? 8 : 4;
+    long nb = (A.nlines + LPB - 1) / LPB;
+    hipLaunchKernelGGL((k_sweep<SCH, CPT, LPB>), dim3((unsigned)nb), dim3(64 * LPB), 0, s, A);
+}
+
+template <int SCH>
+void launch_sweep_sch(int cpt, const SweepArgs &A, hipStream_t s)
+{
+    switch (cpt) {
+    case 1: launch_sweep_cpt<SCH, 1>(A, s); break;
+    case 2: launch_sweep_cpt<SCH, 2>(A, s); break;
+    case 3: launch_sweep_cpt<SCH, 3>(A, s); break;
+    case 4: launch_sweep_cpt<SCH, 4>(A, s); break;
+    case 6: launch_sweep_cpt<SCH, 6>(A, s); break;
+    default: launch_sweep_cpt<SCH, 8>(A, s); break;
+    }
+}
+
+void launch_sweep(int scheme, int cpt, const SweepArgs &A, hipStream_t s)
+{
+    switch (scheme) {
+    case WENO5JS: launch_sweep_sch<WENO5JS>(cpt, A, s); break;
+    case WENO5Z: launch_sweep_sch<WENO5Z>(cpt, A, s); break;
+    case WENO7JS: launch_sweep_sch<WENO7JS>(cpt, A, s); break;
+    case CRWENO5: launch_sweep_sch<CRWENO5>(cpt, A, s); break;
+    default: launch_sweep_sch<HCCS>(cpt, A, s); break;
+    }
+}
+
+int sweep_lpb(int cpt) { return cpt <= 4 ? 8 : 4; }
+
+template <int SCH, int CPT>
+void launch_op_cpt(const OpRecArgs &A, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_op_recon<SCH, CPT>), dim3(A.nlines), dim3(64), 0, s, A);
+}
+
+template <int SCH>
+void launch_op_sch(int cpt, const OpRecArgs &A, hipStream_t s)
+{
+    switch (cpt) {
+    case 1: launch_op_cpt<SCH, 1>(A, s); break;
+    case 2: launch_op_cpt<SCH, 2>(A, s); break;
+    case 3: launch_op_cpt<SCH, 3>(A, s); break;
+    case 4: launch_op_cpt<SCH, 4>(A, s); break;
+    case 6: launch_op_cpt<SCH, 6>(A, s); break;
+    default: launch_op_cpt<SCH, 8>(A, s); break;
+    }
+}
+
+}  // namespace
+
+// ===========================================================================
+// context
+// ===========================================================================
+struct nuc_ctx {
+    nuc_config c;
+    hipStream_t stream = nullptr;
+    long N = 0;
+    int n[3], np[3], nloc[3], ts[3], tsv[3], cpt[3];
+    double h[3];
+    int P = 1;
+    double *q[3] = {nullptr, nullptr, nullptr};
+    double *rhs = nullptr, *prim = nullptr, *fv = nullptr;
+    unsigned long long *lam = nullptr;
+    double *scal = nullptr;
+    double *res_part = nullptr;
+    long res_nb = 0;
+    double *stat_part = nullptr;
+    long stat_nb = 0;
+    double *stat_out = nullptr;
+    std::string err;
+    long step = 0;
+    double time = 0.0;
+    int pending = 0;  // steps enqueued but not yet accounted in time
+};
+
+#define NUC_CHECK(ctx, call)                                                                  \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) {                                                               \
+            if (ctx) (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return NUC_EHIP;                                                                  \
+        }                                                                                     \
+    } while (0)
+
+static int validate(const nuc_config *c, std::string &err)
+{
+    int n[3] = {c->nx, c->ny, c->nz}, np[3] = {c->npx, c->npy, c->npz};
+    for (int d = 0; d < 3; d++) {
+        if (np[d] < 1 || n[d] % np[d] != 0) { err = "grid size must divide evenly by the rank grid (src/block.cpp:56-58)"; return NUC_EINVAL; }
+        if (c->scheme[d] < 0 || c->scheme[d] > 4) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
+        if (c->vis_scheme[d] < 0 || c->vis_scheme[d] > 1) { err = "unknown viscous scheme id"; return NUC_EINVAL; }
+        int ts = (c->scheme[d] == NUC_WENO7JS || c->scheme[d] == NUC_HCCS) ? 4 : 3;
+        if (n[d] / np[d] < 2 * ts + 2) { err = "rank-local line too short for the scheme's halo"; return NUC_EINVAL; }
+        if (n[d] > 512) { err = "line length above 512 is not supported"; return NUC_EINVAL; }
+    }
+    if ((long)c->npx * c->npy * c->npz > MAXRANK) { err = "at most 64 ranks"; return NUC_EINVAL; }
+    if (c->riemann < 0 || c->riemann > 1 || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
+    if (c->forcing != NUC_FORCING_NONE) { err = "Const forcing is not available on this build"; return NUC_EINVAL; }
+    if (c->precision != NUC_FP64) { err = "only the FP64 path is available on this build"; return NUC_EINVAL; }
+    if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) { err = "box lengths must be positive"; return NUC_EINVAL; }
+    return NUC_OK;
+}
+
+extern "C" long long nuc_footprint(const nuc_config *c)
+{
+    long long N = (long long)c->nx * c->ny * c->nz;
+    long long doubles = 15 * N + 5 * N + (c->model == NUC_NAVIERSTOKES3D ? 13 * N : 0);
+    return doubles * 8 + (N / 32 + 4096) * 8 + 4096;
+}
+
+extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
+{
+    if (!cfg || !out) return NUC_EINVAL;
+    *out = nullptr;
+    std::string err;
+    int rc = validate(cfg, err);
+    if (rc != NUC_OK) {
+        fprintf(stderr, "nuc_create: %s\n", err.c_str());
+        return rc;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return NUC_ENODEV;
+    nuc_ctx *x = new nuc_ctx();
+    x->c = *cfg;
+    x->n[0] = cfg->nx; x->n[1] = cfg->ny; x->n[2] = cfg->nz;
+    x->np[0] = cfg->npx; x->np[1] = cfg->npy; x->np[2] = cfg->npz;
+    double Lb[3] = {cfg->lx, cfg->ly, cfg->lz};
+    x->N = (long)cfg->nx * cfg->ny * cfg->nz;
+    x->P = cfg->npx * cfg->npy * cfg->npz;
+    bool any_cd8 = false;
+    for (int d = 0; d < 3; d++) {
+        x->nloc[d] = x->n[d] / x->np[d];
+        x->h[d] = Lb[d] / x->n[d];
+        x->ts[d] = (cfg->scheme[d] == NUC_WENO7JS || cfg->scheme[d] == NUC_HCCS) ? 4 : 3;
+        // cd8 needs four ghost cells; the reference hands it the inviscid halo
+        // (src/fieldOperator.cpp:520-526) and reads outside the strip when that is 3
+        x->tsv[d] = (cfg->vis_scheme[d] == NUC_CD8 && x->ts[d] < 4) ? 4 : x->ts[d];
+        x->cpt[d] = cpt_for(x->n[d]);
+        if (cfg->vis_scheme[d] == NUC_CD8) any_cd8 = true;
+    }
+    (void)any_cd8;
+    const size_t N = (size_t)x->N;
+    auto fail = [&](const char *what) {
+        x->err = what;
+        nuc_destroy(x);
+        return NUC_ENOMEM;
+    };
+    if (hipStreamCreate(&x->stream) != hipSuccess) { delete x; return NUC_EHIP; }
+    for (int b = 0; b < 3; b++)
+        if (hipMalloc(&x->q[b], 5 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc state");
+    if (hipMalloc(&x->rhs, 5 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc rhs");
+    if (cfg->model == NUC_NAVIERSTOKES3D) {
+        if (hipMalloc(&x->prim, 4 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc prim");
+        if (hipMalloc(&x->fv, 9 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc fv");
+    }
+    if (hipMalloc(&x->lam, 3 * MAXRANK * sizeof(unsigned long long)) != hipSuccess) return fail("hipMalloc lam");
+    if (hipMalloc(&x->scal, 8 * sizeof(double)) != hipSuccess) return fail("hipMalloc scal");
+    long lines_z = x->N / x->n[2];
+    x->res_nb = (lines_z + sweep_lpb(x->cpt[2]) - 1) / sweep_lpb(x->cpt[2]);
+    if (hipMalloc(&x->res_part, x->res_nb * sizeof(double)) != hipSuccess) return fail("hipMalloc res");
+    (void)hipMemset(x->lam, 0, 3 * MAXRANK * sizeof(unsigned long long));
+    (void)hipMemset(x->scal, 0, 8 * sizeof(double));
+    if (hipDeviceSynchronize() != hipSuccess) return fail("device init");
+    *out = x;
+    return NUC_OK;
+}
+
+extern "C" int nuc_destroy(nuc_ctx *x)
+{
+    if (!x) return NUC_OK;
+    if (x->stream) (void)hipStreamSynchronize(x->stream);
+    for (int b = 0; b < 3; b++) (void)hipFree(x->q[b]);
+    (void)hipFree(x->rhs); (void)hipFree(x->prim); (void)hipFree(x->fv); (void)hipFree(x->lam);
+    (void)hipFree(x->scal); (void)hipFree(x->res_part); (void)hipFree(x->stat_part); (void)hipFree(x->stat_out);
+    if (x->stream) (void)hipStreamDestroy(x->stream);
+    delete x;
+    return NUC_OK;
+}
+
+extern "C" const char *nuc_last_error(const nuc_ctx *x) { return x ? x->err.c_str() : "null context"; }
+
+extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
+{
+    if (!x || !q) return NUC_EINVAL;
+    NUC_CHECK(x, hipMemcpyAsync(x->q[0], q, 5 * x->N * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+extern "C" int nuc_download_state(nuc_ctx *x, double *q)
+{
+    if (!x || !q) return NUC_EINVAL;
+    NUC_CHECK(x, hipMemcpyAsync(q, x->q[0], 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+// one RHS evaluation of state buffer `in`; with epilogue the z sweep performs
+// RK stage `stage` from `in` (and q[0] = u^n) into `outb`
+static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
+{
+    const nuc_config &c = x->c;
+    const long N = x->N;
+    double *qi = x->q[in];
+    const bool ns = (c.model == NUC_NAVIERSTOKES3D);
+    const bool lam_needed = (stage == 0);
+    {
+        PrimArgs P{};
+        for (int v = 0; v < 5; v++) P.q[v] = qi + v * N;
+        if (ns) for (int f = 0; f < 4; f++) P.prim[f] = x->prim + f * N;
+        P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
+        P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2];
+        P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
+        P.riemann = c.riemann; P.do_lam = lam_needed ? 1 : 0; P.write_prim = ns ? 1 : 0;
+        P.gamma = c.gamma; P.mach = c.mach; P.lam = x->lam;
+        if (P.do_lam || P.write_prim) {
+            long nb = (N + 4095) / 4096;
+            if (nb > 65536) nb = 65536;
+            hipLaunchKernelGGL(k_prim, dim3((unsigned)nb), dim3(256), 0, x->stream, P);
+        }
+        if (lam_needed)
+            hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, x->stream, x->lam, x->P, x->h[0], x->h[1], x->h[2], c.cfl,
+                               epilogue ? 1 : 0, x->scal);
+    }
+    if (ns) {
+        GradArgs G{};
+        for (int f = 0; f < 4; f++) G.prim[f] = x->prim + f * N;
+        for (int o = 0; o < 9; o++) G.out[o] = x->fv + o * N;
+        G.nx = x->n[0]; G.ny = x->n[1]; G.nz = x->n[2];
+        for (int d = 0; d < 3; d++) {
+            G.nloc[d] = x->nloc[d]; G.tsv[d] = x->tsv[d];
+            G.vis[d] = (c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6;
+            G.inv_h[d] = 1.0 / x->h[d];
+        }
+        G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
+        G.tref = c.t_ref; G.tinf = c.t_inf;
+        dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
+        hipLaunchKernelGGL(k_grad, grid, dim3(64, 4, 1), 0, x->stream, G);
+    }
+    // fv component order per direction: x (txx,txy,txz,ex) y (txy,tyy,tyz,ey) z (txz,tyz,tzz,ez)
+    static const int fvmap[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+    for (int d = 0; d < 3; d++) {
+        SweepArgs S{};
+        S.dir = d; S.n = x->n[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
+        S.vis = ns ? ((c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6) : VIS_NONE;
+        S.riemann = c.riemann;
+        S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
+        S.nlines = N / x->n[d];
+        S.inv_h = 1.0 / x->h[d];
+        S.gamma = c.gamma;
+        for (int v = 0; v < 5; v++) { S.q[v] = qi + v * N; S.rhs[v] = x->rhs + v * N; }
+        if (ns) for (int o = 0; o < 4; o++) S.fv[o] = x->fv + (long)fvmap[d][o] * N;
+        S.stage = stage;
+        S.epilogue = (d == 2 && epilogue) ? 1 : 0;
+        if (S.epilogue) {
+            for (int v = 0; v < 5; v++) { S.qn[v] = x->q[0] + v * N; S.qout[v] = x->q[outb] + v * N; }
+            S.dtp = x->scal;
+            S.res_part = x->res_part;
+        }
+        launch_sweep(c.scheme[d], x->cpt[d], S, x->stream);
+    }
+    if (epilogue && stage == 2)
+        hipLaunchKernelGGL(k_res, dim3(1), dim3(256), 0, x->stream, x->res_part, (int)x->res_nb,
+                           1.0 / ((double)N * 5.0), x->scal);
+    return NUC_OK;
+}
+
+static int enqueue_steps(nuc_ctx *x, int nsteps)
+{
+    for (int it = 0; it < nsteps; it++) {
+        enqueue_stage(x, 0, 0, 1, true);
+        enqueue_stage(x, 1, 1, 2, true);
+        enqueue_stage(x, 2, 2, 0, true);
+    }
+    NUC_CHECK(x, hipGetLastError());
+    return NUC_OK;
+}
+
+extern "C" int nuc_step_async(nuc_ctx *x, int nsteps)
+{
+    if (!x || nsteps < 0) return NUC_EINVAL;
+    int rc = enqueue_steps(x, nsteps);
+    if (rc == NUC_OK) x->pending += nsteps;
+    return rc;
+}
+
+extern "C" int nuc_sync(nuc_ctx *x, nuc_step_info *info)
+{
+    if (!x) return NUC_EINVAL;
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    double sc[8];
+    NUC_CHECK(x, hipMemcpy(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost));
+    x->time = sc[2];
+    x->step += x->pending;
+    x->pending = 0;
+    if (info) { info->dt = sc[0]; info->res = sc[1]; info->time = x->time; info->step = x->step; }
+    if (!std::isfinite(sc[0]) || !std::isfinite(sc[1])) {
+        x->err = "non-finite dt or residual";
+        return NUC_ENONFINITE;
+    }
+    return NUC_OK;
+}
+
+extern "C" int nuc_step(nuc_ctx *x, int nsteps, nuc_step_info *info)
+{
+    if (!x || nsteps < 0) return NUC_EINVAL;
+    for (int it = 0; it < nsteps; it++) {
+        int rc = enqueue_steps(x, 1);
+        if (rc != NUC_OK) return rc;
+        x->pending = 1;
+        rc = nuc_sync(x, info);
+        if (rc != NUC_OK) return rc;
+    }
+    if (nsteps == 0 && info) return nuc_sync(x, info);
+    return NUC_OK;
+}
+
+extern "C" int nuc_rhs(nuc_ctx *x, double *rhs_host, double *dt_local)
+{
+    if (!x) return NUC_EINVAL;
+    enqueue_stage(x, 0, 0, 0, false);
+    NUC_CHECK(x, hipGetLastError());
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    if (rhs_host)
+        NUC_CHECK(x, hipMemcpy(rhs_host, x->rhs, 5 * x->N * sizeof(double), hipMemcpyDeviceToHost));
+    if (dt_local) {
+        double sc[8];
+        NUC_CHECK(x, hipMemcpy(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost));
+        *dt_local = sc[3];
+    }
+    return NUC_OK;
+}
+
+extern "C" int nuc_stats(nuc_ctx *x, double *out)
+{
+    if (!x || !out) return NUC_EINVAL;
+    const long N = x->N;
+    const nuc_config &c = x->c;
+    double *prim = x->prim;
+    bool own_prim = false;
+    if (!prim) {
+        NUC_CHECK(x, hipMalloc(&prim, 4 * N * sizeof(double)));
+        own_prim = true;
+    }
+    PrimArgs P{};
+    for (int v = 0; v < 5; v++) P.q[v] = x->q[0] + v * N;
+    for (int f = 0; f < 4; f++) P.prim[f] = prim + f * N;
+    P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
+    P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2]; P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
+    P.riemann = c.riemann; P.do_lam = 0; P.write_prim = 1; P.gamma = c.gamma; P.mach = c.mach; P.lam = x->lam;
+    long nbp = (N + 4095) / 4096;
+    if (nbp > 65536) nbp = 65536;
+    hipLaunchKernelGGL(k_prim, dim3((unsigned)nbp), dim3(256), 0, x->stream, P);
+    dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
+    long nb = (long)grid.x * grid.y * grid.z;
+    if (nb > x->stat_nb) {
+        (void)hipFree(x->stat_part);
+        NUC_CHECK(x, hipMalloc(&x->stat_part, nb * NSTAT * sizeof(double)));
+        x->stat_nb = nb;
+    }
+    if (!x->stat_out) NUC_CHECK(x, hipMalloc(&x->stat_out, NSTAT * sizeof(double)));
+    StatArgs S{};
+    for (int f = 0; f < 4; f++) S.prim[f] = prim + f * N;
+    S.rho = x->q[0];
+    S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
+    for (int d = 0; d < 3; d++) {
+        S.nloc[d] = x->nloc[d]; S.tsv[d] = x->tsv[d];
+        S.vis[d] = (c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6;
+        S.inv_h[d] = 1.0 / x->h[d];
+    }
+    S.gamma = c.gamma; S.mach = c.mach; S.tref = c.t_ref; S.tinf = c.t_inf; S.part = x->stat_part;
+    hipLaunchKernelGGL(k_stats, grid, dim3(64, 4, 1), 0, x->stream, S);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(64), 0, x->stream, x->stat_part, nb, x->stat_out);
+    NUC_CHECK(x, hipGetLastError());
+    double s[NSTAT];
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    NUC_CHECK(x, hipMemcpy(s, x->stat_out, sizeof(s), hipMemcpyDeviceToHost));
+    if (own_prim) (void)hipFree(prim);
+    const double Nd = (double)N;
+    for (int t = 0; t < 16; t++) out[t] = s[t] / Nd;
+    double u_rms_aver = s[16] / Nd, div_aver = s[17] / Nd, Skew3 = s[18] / Nd;
+    double alpha_aver = s[19] / Nd, miu_aver = s[20] / Nd, rho_bar = s[21] / Nd, Skew2 = div_aver;
+    // src/postproc.cpp:858-872
+    double skew = Skew3 / std::pow(Skew2, 1.5);
+    double lambda = std::sqrt(u_rms_aver / div_aver);
+    double Re_lambda = std::sqrt(u_rms_aver) * lambda * rho_bar / miu_aver * c.reynolds;
+    double tau_t = lambda / std::sqrt(u_rms_aver);
+    out[16] = std::sqrt(u_rms_aver);
+    out[17] = std::sqrt(u_rms_aver * 3.0) / alpha_aver;
+    out[18] = std::sqrt(div_aver);
+    out[19] = miu_aver;
+    out[20] = Re_lambda;
+    out[21] = lambda;
+    out[22] = rho_bar;
+    out[23] = tau_t;
+    out[24] = skew;
+    return NUC_OK;
+}
+
+// ===========================================================================
+// operator hooks
+// ===========================================================================
+namespace {
+struct DevBuf {
+    double *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    bool alloc(size_t n) { return hipMalloc(&p, n * sizeof(double)) == hipSuccess; }
+};
+}  // namespace
+
+extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const double *in, int n1, int n2, int n3,
+                                  const double *gL, const double *gR, int bf, double *out, int zones)
+{
+    if (!in || !out || n1 < 2 * tilesize + 1 || n2 < 1 || n3 < 1 || (upwind != 1 && upwind != -1)) return NUC_EINVAL;
+    if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < tilesize)) return NUC_EINVAL;
+    if (scheme < 0 || scheme > 4 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
+    int cpt = cpt_for(n1 + 1);
+    if (cpt < 0) return NUC_EINVAL;
+    long nl = (long)n2 * n3;
+    DevBuf di, dl, dr, dout;
+    if (!di.alloc(nl * n1) || !dout.alloc(nl * (n1 + 1))) return NUC_ENOMEM;
+    if (hipMemcpy(di.p, in, nl * n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(dout.p, out, nl * (n1 + 1) * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (gL && bf > 0) {
+        if (!dl.alloc(nl * bf)) return NUC_ENOMEM;
+        if (hipMemcpy(dl.p, gL, nl * bf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    }
+    if (gR && bf > 0) {
+        if (!dr.alloc(nl * bf)) return NUC_ENOMEM;
+        if (hipMemcpy(dr.p, gR, nl * bf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    }
+    OpRecArgs A{di.p, dl.p, dr.p, dout.p, n1, (int)nl, bf, tilesize, upwind, zones};
+    switch (scheme) {
+    case WENO5JS: launch_op_sch<WENO5JS>(cpt, A, 0); break;
+    case WENO5Z: launch_op_sch<WENO5Z>(cpt, A, 0); break;
+    case WENO7JS: launch_op_sch<WENO7JS>(cpt, A, 0); break;
+    case CRWENO5: launch_op_sch<CRWENO5>(cpt, A, 0); break;
+    default: launch_op_sch<HCCS>(cpt, A, 0); break;
+    }
+    if (hipGetLastError() != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(out, dout.p, nl * (n1 + 1) * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return NUC_EHIP;
+    return NUC_OK;
+}
+
+extern "C" int nuc_op_diff(int vis_scheme, int tilesize, const double *in, double h, int n1, int n2, int n3,
+                           const double *gL, const double *gR, int bf, double *out, int zones)
+{
+    if (!in || !out || n1 < 2 * tilesize || n2 < 1 || n3 < 1 || !(h != 0.0)) return NUC_EINVAL;
+    int need = (vis_scheme == NUC_CD8) ? 4 : 3;
+    if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < need)) return NUC_EINVAL;
+    if (vis_scheme == NUC_CD8 && tilesize < 4) return NUC_EINVAL;  // the reference's out-of-stencil read
+    long nl = (long)n2 * n3;
+    DevBuf di, dl, dr, dout;
+    if (!di.alloc(nl * n1) || !dout.alloc(nl * n1)) return NUC_ENOMEM;
+    if (hipMemcpy(di.p, in, nl * n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(dout.p, out, nl * n1 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (gL && bf > 0) {
+        if (!dl.alloc(nl * bf)) return NUC_ENOMEM;
+        if (hipMemcpy(dl.p, gL, nl * bf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    }
+    if (gR && bf > 0) {
+        if (!dr.alloc(nl * bf)) return NUC_ENOMEM;
+        if (hipMemcpy(dr.p, gR, nl * bf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    }
+    OpDiffArgs A{di.p, dl.p, dr.p, dout.p, n1, bf, tilesize, vis_scheme == NUC_CD8 ? VIS_CD8 : VIS_CD6, zones, nl,
+                 1.0 / h};
+    long tot = nl * n1;
+    hipLaunchKernelGGL(k_op_diff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, A);
+    if (hipGetLastError() != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(out, dout.p, nl * n1 * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return NUC_EHIP;
+    return NUC_OK;
+}
+
+extern "C" int nuc_op_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n)
+{
+    if (!rhs || !un || !ui || n < 1 || stage < 0 || stage > 2) return NUC_EINVAL;
+    DevBuf dr, du, di;
+    if (!dr.alloc(n) || !du.alloc(n) || !di.alloc(n)) return NUC_ENOMEM;
+    if (hipMemcpy(dr.p, rhs, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(du.p, un, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(di.p, ui, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
+    hipLaunchKernelGGL(k_op_rk3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, stage, dr.p, du.p, di.p, dt, n);
+    if (hipGetLastError() != hipSuccess) return NUC_EHIP;
+    if (hipMemcpy(ui, di.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return NUC_EHIP;
+    return NUC_OK;
+}

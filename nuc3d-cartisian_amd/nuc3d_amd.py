@@ -1,0 +1,251 @@
+"""Python binding of libnuc3d_amd.so (include/nuc3d_gpu.h) through ctypes.
+
+Loads the in-tree library and fails loudly when it is missing; there is no
+CPU fallback.  Arrays are numpy float64, x-fastest, conserved state as
+5*nx*ny*nz values (rho, rho*u, rho*v, rho*w, rho*E).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libnuc3d_amd.so")
+_dp = ctypes.POINTER(ctypes.c_double)
+
+WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS = range(5)
+CD6, CD8 = 0, 1
+LF, AUSM = 0, 1
+EULER3D, NAVIERSTOKES3D = 0, 1
+SUTHERLAND, CONSTANT = 0, 1
+FORCING_NONE, FORCING_CONST = 0, 1
+FP64, MIXED = 0, 1
+SCHEMES = {"weno5js": WENO5JS, "weno5z": WENO5Z, "weno7js": WENO7JS, "crweno5": CRWENO5, "hccs": HCCS}
+STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_ENONFINITE", 5: "NUC_ENODEV"}
+
+# every entry point include/nuc3d_gpu.h declares
+EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
+           "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_sync", "nuc_stats",
+           "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp", "nuc_init_tgv",
+           "nuc_init_hit"]
+
+
+class NucConfig(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+        ("npx", ctypes.c_int), ("npy", ctypes.c_int), ("npz", ctypes.c_int),
+        ("lx", ctypes.c_double), ("ly", ctypes.c_double), ("lz", ctypes.c_double),
+        ("scheme", ctypes.c_int * 3), ("vis_scheme", ctypes.c_int * 3),
+        ("riemann", ctypes.c_int), ("model", ctypes.c_int), ("viscosity", ctypes.c_int),
+        ("forcing", ctypes.c_int), ("precision", ctypes.c_int),
+        ("gamma", ctypes.c_double), ("mach", ctypes.c_double), ("reynolds", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("t_ref", ctypes.c_double), ("t_inf", ctypes.c_double),
+        ("forcing_amp", ctypes.c_double), ("forcing_amp_hel", ctypes.c_double),
+        ("forcing_k", ctypes.c_double), ("cfl", ctypes.c_double),
+    ]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("time", ctypes.c_double), ("res", ctypes.c_double),
+                ("step", ctypes.c_long)]
+
+
+class NucError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                               "(make -C nuc3d-cartisian_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        L.nuc_create.argtypes = [ctypes.POINTER(NucConfig), ctypes.POINTER(vp)]
+        L.nuc_destroy.argtypes = [vp]
+        L.nuc_last_error.argtypes = [vp]
+        L.nuc_last_error.restype = ctypes.c_char_p
+        L.nuc_footprint.argtypes = [ctypes.POINTER(NucConfig)]
+        L.nuc_footprint.restype = ctypes.c_longlong
+        L.nuc_upload_state.argtypes = [vp, _dp]
+        L.nuc_download_state.argtypes = [vp, _dp]
+        L.nuc_rhs.argtypes = [vp, _dp, _dp]
+        L.nuc_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
+        L.nuc_step_async.argtypes = [vp, ctypes.c_int]
+        L.nuc_sync.argtypes = [vp, ctypes.POINTER(StepInfo)]
+        L.nuc_stats.argtypes = [vp, _dp]
+        L.nuc_op_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.nuc_op_diff.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
+        L.nuc_op_rk3.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double, ctypes.c_long]
+        L.nuc_config_from_inp.argtypes = [ctypes.c_char_p, ctypes.POINTER(NucConfig),
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+        L.nuc_init_tgv.argtypes = [ctypes.POINTER(NucConfig), _dp]
+        L.nuc_init_hit.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_ulonglong, _dp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _check(rc, what, ctx=None):
+    if rc != 0:
+        msg = what
+        if ctx is not None:
+            msg += ": " + lib().nuc_last_error(ctx).decode()
+        raise NucError(rc, msg)
+
+
+def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, riemann=LF,
+                model=NAVIERSTOKES3D, viscosity=SUTHERLAND, forcing=FORCING_NONE, precision=FP64,
+                gamma=1.4, mach=0.1, reynolds=1600.0, prandtl=0.72, t_ref=110.4, t_inf=288.0,
+                forcing_amp=0.01, forcing_amp_hel=0.01, forcing_k=1.0, cfl=0.5):
+    c = NucConfig()
+    c.nx, c.ny, c.nz = n
+    c.npx, c.npy, c.npz = npr
+    if L is None:
+        L = (2 * np.pi,) * 3
+    c.lx, c.ly, c.lz = L
+    sch = scheme if isinstance(scheme, (tuple, list)) else (scheme,) * 3
+    vs = vis if isinstance(vis, (tuple, list)) else (vis,) * 3
+    for d in range(3):
+        c.scheme[d] = sch[d]
+        c.vis_scheme[d] = vs[d]
+    c.riemann, c.model, c.viscosity, c.forcing, c.precision = riemann, model, viscosity, forcing, precision
+    c.gamma, c.mach, c.reynolds, c.prandtl = gamma, mach, reynolds, prandtl
+    c.t_ref, c.t_inf = t_ref, t_inf
+    c.forcing_amp, c.forcing_amp_hel, c.forcing_k = forcing_amp, forcing_amp_hel, forcing_k
+    c.cfl = cfl
+    return c
+
+
+def config_from_inp(directory):
+    c = NucConfig()
+    s = ctypes.c_int()
+    e = ctypes.c_int()
+    _check(lib().nuc_config_from_inp(directory.encode(), ctypes.byref(c), ctypes.byref(s), ctypes.byref(e)),
+           "nuc_config_from_inp")
+    return c, s.value, e.value
+
+
+def init_tgv(cfg):
+    q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
+    _check(lib().nuc_init_tgv(ctypes.byref(cfg), _ptr(q)), "nuc_init_tgv")
+    return q
+
+
+def init_hit(cfg, k0=4.0, u_rms=1.0, seed=20261202):
+    q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
+    _check(lib().nuc_init_hit(ctypes.byref(cfg), float(k0), float(u_rms), int(seed), _ptr(q)), "nuc_init_hit")
+    return q
+
+
+def footprint(cfg):
+    return lib().nuc_footprint(ctypes.byref(cfg))
+
+
+class Context:
+    """One device context: state resident in HBM, the RHS / RK3 hot path on the GPU."""
+
+    def __init__(self, cfg):
+        self.cfg = cfg
+        self.N = cfg.nx * cfg.ny * cfg.nz
+        h = ctypes.c_void_p()
+        _check(lib().nuc_create(ctypes.byref(cfg), ctypes.byref(h)), "nuc_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nuc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        assert q.size == 5 * self.N
+        _check(lib().nuc_upload_state(self.h, _ptr(q)), "nuc_upload_state", self.h)
+
+    def download(self):
+        q = np.zeros(5 * self.N)
+        _check(lib().nuc_download_state(self.h, _ptr(q)), "nuc_download_state", self.h)
+        return q
+
+    def rhs(self):
+        r = np.zeros(5 * self.N)
+        dt = ctypes.c_double()
+        _check(lib().nuc_rhs(self.h, _ptr(r), ctypes.byref(dt)), "nuc_rhs", self.h)
+        return r, dt.value
+
+    def step(self, n=1):
+        info = StepInfo()
+        _check(lib().nuc_step(self.h, n, ctypes.byref(info)), "nuc_step", self.h)
+        return info.dt, info.res, info.time
+
+    def step_async(self, n):
+        _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    def sync(self):
+        info = StepInfo()
+        _check(lib().nuc_sync(self.h, ctypes.byref(info)), "nuc_sync", self.h)
+        return info.dt, info.res, info.time
+
+    def stats(self):
+        out = np.zeros(25)
+        _check(lib().nuc_stats(self.h, _ptr(out)), "nuc_stats", self.h)
+        return out
+
+
+def op_reconstruct(scheme, upwind, tilesize, fin, gL=None, gR=None, zones=7, out=None):
+    """fin shaped (n3, n2, n1); ghosts (n3, n2, bf).  Returns (n3, n2, n1+1)."""
+    n3, n2, n1 = fin.shape
+    bf = gL.shape[2] if gL is not None else (gR.shape[2] if gR is not None else 0)
+    if out is None:
+        out = np.zeros((n3, n2, n1 + 1))
+    fin = np.ascontiguousarray(fin, dtype=np.float64)
+    gL = None if gL is None else np.ascontiguousarray(gL, dtype=np.float64)
+    gR = None if gR is None else np.ascontiguousarray(gR, dtype=np.float64)
+    _check(lib().nuc_op_reconstruct(scheme, upwind, tilesize, _ptr(fin), n1, n2, n3, _ptr(gL), _ptr(gR), bf,
+                                    _ptr(out), zones), "nuc_op_reconstruct")
+    return out
+
+
+def op_diff(scheme, tilesize, fin, h, gL=None, gR=None, zones=7, out=None):
+    n3, n2, n1 = fin.shape
+    bf = gL.shape[2] if gL is not None else (gR.shape[2] if gR is not None else 0)
+    if out is None:
+        out = np.zeros((n3, n2, n1))
+    fin = np.ascontiguousarray(fin, dtype=np.float64)
+    gL = None if gL is None else np.ascontiguousarray(gL, dtype=np.float64)
+    gR = None if gR is None else np.ascontiguousarray(gR, dtype=np.float64)
+    _check(lib().nuc_op_diff(scheme, tilesize, _ptr(fin), float(h), n1, n2, n3, _ptr(gL), _ptr(gR), bf, _ptr(out),
+                             zones), "nuc_op_diff")
+    return out
+
+
+def op_rk3(stage, rhs, un, ui, dt):
+    ui = np.array(ui, dtype=np.float64, copy=True).ravel()
+    _check(lib().nuc_op_rk3(stage, _ptr(np.ascontiguousarray(rhs, dtype=np.float64).ravel()),
+                            _ptr(np.ascontiguousarray(un, dtype=np.float64).ravel()), _ptr(ui), float(dt), ui.size),
+           "nuc_op_rk3")
+    return ui

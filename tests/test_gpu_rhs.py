@@ -1,0 +1,141 @@
+"""Solver-level parity: the fused HIP stage (split, reconstruct, viscous,
+RHS, TVD-RK3 with on-device periodic wrap) against the oracle's restatement of
+NaiverStokesData3d::solve + PDEData3d::solve on the same seeded fields.
+
+Bar: relative L-infinity per conserved variable <= 1e-12 after 10 RK3 steps
+(the FP64 bar), measured as max|gpu - oracle| / max|oracle| per variable.
+"""
+import numpy as np
+import pytest
+
+import nuc3d_amd as G
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfgs(n, npr, scheme, vis, riemann, model, mach=0.1, reynolds=1600.0, cfl=0.5):
+    kw = dict(n=n, npr=npr, scheme=scheme, vis=vis, riemann=riemann, model=model, mach=mach,
+              reynolds=reynolds, cfl=cfl)
+    return G.make_config(**kw), O.make_config(**kw)
+
+
+def _perturbed_tgv(oc, seed, amp=0.02):
+    q = O.init_tgv(oc)
+    N = oc.nx * oc.ny * oc.nz
+    rng = np.random.default_rng(seed)
+    q = q.reshape(5, N)
+    # smooth-ish perturbation so that w != 0 and the flow is fully 3-D
+    q[3] += amp * np.sin(np.linspace(0, 40, N) + rng.random())
+    q[0] *= 1.0 + 0.01 * rng.standard_normal(N)
+    return q.ravel().copy()
+
+
+def _rel_linf(a, b, N):
+    a = a.reshape(5, N)
+    b = b.reshape(5, N)
+    return np.array([np.abs(a[v] - b[v]).max() / max(np.abs(b[v]).max(), 1e-300) for v in range(5)])
+
+
+CASES = [
+    # n, rank grid, scheme, vis, riemann, model
+    ((16, 16, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((16, 16, 16), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((16, 16, 16), (1, 1, 1), G.WENO7JS, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((16, 16, 16), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((16, 16, 16), (1, 1, 1), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((24, 20, 18), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((24, 20, 18), (2, 2, 2), G.CRWENO5, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((32, 32, 32), (2, 1, 2), G.WENO5Z, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((32, 32, 32), (1, 2, 1), G.HCCS, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((32, 24, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.EULER3D),
+    ((20, 20, 20), (1, 1, 1), G.WENO7JS, G.CD6, G.AUSM, G.EULER3D),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]}-s{c[2]}-v{c[3]}-r{c[4]}-m{c[5]}")
+def test_rhs_matches_oracle(case):
+    n, npr, scheme, vis, rie, model = case
+    print(f"rhs case {case}", flush=True)
+    gc, oc = _cfgs(n, npr, scheme, vis, rie, model)
+    q = _perturbed_tgv(oc, 1)
+    N = oc.nx * oc.ny * oc.nz
+    s = O.Solver(oc)
+    s.set_state(q)
+    r_ref, dt_ref = s.rhs()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        r_gpu, dt_gpu = ctx.rhs()
+    scale = np.abs(r_ref.reshape(5, N)).max(axis=1)
+    err = np.abs(r_gpu.reshape(5, N) - r_ref.reshape(5, N)).max(axis=1) / np.maximum(scale, 1e-300)
+    print("rhs rel err per variable", err, flush=True)
+    assert np.all(err <= 1e-11), err
+    assert abs(dt_gpu - dt_ref) <= 1e-14 * dt_ref
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]}-s{c[2]}-v{c[3]}-r{c[4]}-m{c[5]}")
+def test_ten_steps_match_oracle(case):
+    n, npr, scheme, vis, rie, model = case
+    print(f"10-step case {case}", flush=True)
+    gc, oc = _cfgs(n, npr, scheme, vis, rie, model)
+    q = _perturbed_tgv(oc, 2)
+    N = oc.nx * oc.ny * oc.nz
+    s = O.Solver(oc)
+    s.set_state(q)
+    dt_ref, res_ref, t_ref = s.step(10)
+    q_ref = s.get_state()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        dt_gpu, res_gpu, t_gpu = ctx.step(10)
+        q_gpu = ctx.download()
+    err = _rel_linf(q_gpu, q_ref, N)
+    print("10-step rel Linf per variable", err, flush=True)
+    assert np.all(err <= 1e-12), err
+    assert abs(dt_gpu - dt_ref) <= 1e-13 * dt_ref
+    assert abs(t_gpu - t_ref) <= 1e-13 * t_ref
+    assert abs(res_gpu - res_ref) <= 1e-9 * abs(res_ref)
+
+
+def test_tgv64_weno5js_ten_steps():
+    """The Taylor-Green configuration: 64^3, WENO5-JS, LF, cd6, Re 1600, M 0.1, CFL 0.5."""
+    gc, oc = _cfgs((64, 64, 64), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D)
+    q = O.init_tgv(oc)
+    assert np.array_equal(q, G.init_tgv(gc))
+    N = 64 ** 3
+    oc.nthreads = 8
+    s = O.Solver(oc)
+    s.set_state(q)
+    s.step(10)
+    q_ref = s.get_state()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        ctx.step(10)
+        q_gpu = ctx.download()
+    err = _rel_linf(q_gpu, q_ref, N)
+    print("TGV 64^3 10-step rel Linf", err, flush=True)
+    assert np.all(err <= 1e-12), err
+
+
+def test_stats_match_oracle():
+    gc, oc = _cfgs((32, 32, 32), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D)
+    q = _perturbed_tgv(oc, 4)
+    s = O.Solver(oc)
+    s.set_state(q)
+    ref = s.stats()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        got = ctx.stats()
+    err = np.abs(got - ref) / np.maximum(np.abs(ref), 1e-12)
+    print("stats rel err", err, flush=True)
+    assert np.all(err <= 1e-9), err
+
+
+def test_state_roundtrip_and_errors():
+    gc, _ = _cfgs((16, 16, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D)
+    q = np.random.default_rng(0).random(5 * 16 ** 3) + 1.0
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        assert np.array_equal(ctx.download(), q)
+    bad = G.make_config(n=(18, 16, 16), npr=(4, 1, 1))
+    with pytest.raises(G.NucError):
+        G.Context(bad)
