@@ -114,6 +114,14 @@ int nuc_op_diff(int vis_scheme, int tilesize, const double *in, double h, int n1
                 const double *gL, const double *gR, int bf, double *out, int zones);
 int nuc_op_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n);
 
+/* ---- timing ----------------------------------------------------------------
+ * With profiling on, every kernel launch of the context is bracketed by HIP
+ * events on the context's stream; nuc_sync accumulates the elapsed times per
+ * kernel slot (0 k_prim, 1 k_dt, 2 k_grad, 3/4/5 the x/y/z sweeps, 6 k_res).
+ * nuc_profile(ctx, on) also clears the totals. */
+int nuc_profile(nuc_ctx *ctx, int enable);
+int nuc_profile_read(const nuc_ctx *ctx, int slot, double *total_ms, long *launches, const char **name);
+
 /* ---- configuration -------------------------------------------------------
  * Fill cfg from a directory holding the reference's inp files
  * (inp/block.in, inp/Method.in, inp/PhysModel.in, inp/IOController.in) with the

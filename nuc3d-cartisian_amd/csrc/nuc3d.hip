@@ -31,6 +31,7 @@ using namespace nucdev;
 namespace {
 
 constexpr int MAXRANK = 64;
+constexpr int NUC_PROF_SLOTS = 7;
 
 // ---------------------------------------------------------------------------
 // kernel arguments
@@ -678,6 +679,36 @@ struct nuc_ctx {
     long step = 0;
     double time = 0.0;
     int pending = 0;  // steps enqueued but not yet accounted in time
+    // per-kernel timing with HIP events on the launch stream (nuc_profile_*)
+    bool prof = false;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev;
+    double prof_ms[NUC_PROF_SLOTS] = {0};
+    long prof_n[NUC_PROF_SLOTS] = {0};
+};
+
+static const char *kProfNames[NUC_PROF_SLOTS] = {"k_prim", "k_dt", "k_grad", "k_sweep_x", "k_sweep_y",
+                                                 "k_sweep_z", "k_res"};
+
+// brackets one launch with events when profiling is on
+struct ProfScope {
+    nuc_ctx *x;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(nuc_ctx *x_, int s_) : x(x_), slot(s_)
+    {
+        if (x->prof) {
+            (void)hipEventCreate(&a);
+            (void)hipEventCreate(&b);
+            (void)hipEventRecord(a, x->stream);
+        }
+    }
+    ~ProfScope()
+    {
+        if (x->prof) {
+            (void)hipEventRecord(b, x->stream);
+            x->ev.push_back({slot, {a, b}});
+        }
+    }
 };
 
 #define NUC_CHECK(ctx, call)                                                                  \
@@ -823,11 +854,14 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         if (P.do_lam || P.write_prim) {
             long nb = (N + 4095) / 4096;
             if (nb > 65536) nb = 65536;
+            ProfScope ps(x, 0);
             hipLaunchKernelGGL(k_prim, dim3((unsigned)nb), dim3(256), 0, x->stream, P);
         }
-        if (lam_needed)
+        if (lam_needed) {
+            ProfScope ps(x, 1);
             hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, x->stream, x->lam, x->P, x->h[0], x->h[1], x->h[2], c.cfl,
                                epilogue ? 1 : 0, x->scal);
+        }
     }
     if (ns) {
         GradArgs G{};
@@ -842,6 +876,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
         G.tref = c.t_ref; G.tinf = c.t_inf;
         dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
+        ProfScope ps(x, 2);
         hipLaunchKernelGGL(k_grad, grid, dim3(64, 4, 1), 0, x->stream, G);
     }
     // fv component order per direction: x (txx,txy,txz,ex) y (txy,tyy,tyz,ey) z (txz,tyz,tzz,ez)
@@ -864,11 +899,14 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
             S.dtp = x->scal;
             S.res_part = x->res_part;
         }
+        ProfScope ps(x, 3 + d);
         launch_sweep(c.scheme[d], x->cpt[d], S, x->stream);
     }
-    if (epilogue && stage == 2)
+    if (epilogue && stage == 2) {
+        ProfScope ps(x, 6);
         hipLaunchKernelGGL(k_res, dim3(1), dim3(256), 0, x->stream, x->res_part, (int)x->res_nb,
                            1.0 / ((double)N * 5.0), x->scal);
+    }
     return NUC_OK;
 }
 
@@ -895,6 +933,16 @@ extern "C" int nuc_sync(nuc_ctx *x, nuc_step_info *info)
 {
     if (!x) return NUC_EINVAL;
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    for (auto &e : x->ev) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, e.second.first, e.second.second) == hipSuccess) {
+            x->prof_ms[e.first] += ms;
+            x->prof_n[e.first] += 1;
+        }
+        (void)hipEventDestroy(e.second.first);
+        (void)hipEventDestroy(e.second.second);
+    }
+    x->ev.clear();
     double sc[8];
     NUC_CHECK(x, hipMemcpy(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost));
     x->time = sc[2];
@@ -1089,5 +1137,22 @@ extern "C" int nuc_op_rk3(int stage, const double *rhs, const double *un, double
     hipLaunchKernelGGL(k_op_rk3, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, stage, dr.p, du.p, di.p, dt, n);
     if (hipGetLastError() != hipSuccess) return NUC_EHIP;
     if (hipMemcpy(ui, di.p, n * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess) return NUC_EHIP;
+    return NUC_OK;
+}
+
+extern "C" int nuc_profile(nuc_ctx *x, int enable)
+{
+    if (!x) return NUC_EINVAL;
+    x->prof = enable != 0;
+    for (int i = 0; i < NUC_PROF_SLOTS; i++) { x->prof_ms[i] = 0.0; x->prof_n[i] = 0; }
+    return NUC_OK;
+}
+
+extern "C" int nuc_profile_read(const nuc_ctx *x, int slot, double *total_ms, long *launches, const char **name)
+{
+    if (!x || slot < 0 || slot >= NUC_PROF_SLOTS) return NUC_EINVAL;
+    if (total_ms) *total_ms = x->prof_ms[slot];
+    if (launches) *launches = x->prof_n[slot];
+    if (name) *name = kProfNames[slot];
     return NUC_OK;
 }

@@ -60,9 +60,15 @@ __device__ __forceinline__ T weno5z(T f0, T f1, T f2, T f3, T f4, bool typo)
 
 // 7-point WENO with Z weights, src/weno7js.cpp:79-133 (eps 1e-20); hccs
 // closures use the same formula with eps 1e-40 (src/hccs.cpp:85-135).
+// The smoothness indicators are expanded quadratic forms whose coefficients
+// (up to 17246) cancel to O(h^2) on smooth data, so any reassociation or FMA
+// moves them by ~1e-6 relative and shifts the Z weights.  This function is
+// therefore evaluated without contraction and in the reference's operation
+// order, which makes it bit-identical to the reference for the indicators.
 template <typename T>
 __device__ __forceinline__ T weno7z(const T *f, T eps)
 {
+#pragma clang fp contract(off)
     T is0 = f[0] * (T(547) * f[0] - T(3882) * f[1] + T(4642) * f[2] - T(1854) * f[3])
           + f[1] * (T(7043) * f[1] - T(17246) * f[2] + T(7042) * f[3])
           + f[2] * (T(11003) * f[2] - T(9402) * f[3]) + T(2107) * f[3] * f[3];
@@ -84,7 +90,8 @@ __device__ __forceinline__ T weno7z(const T *f, T eps)
     T a1 = T(12.0 / 35.0) * (T(1) + sq(tau / (eps + is1)));
     T a2 = T(18.0 / 35.0) * (T(1) + sq(tau / (eps + is2)));
     T a3 = T(4.0 / 35.0) * (T(1) + sq(tau / (eps + is3)));
-    return (a0 * q0 + a1 * q1 + a2 * q2 + a3 * q3) / (a0 + a1 + a2 + a3);
+    T asum = a0 + a1 + a2 + a3;
+    return a0 / asum * q0 + a1 / asum * q1 + a2 / asum * q2 + a3 / asum * q3;
 }
 
 // CRWENO5 interior row in upwind coordinates, src/crweno5.cpp:133-162:

@@ -201,105 +201,105 @@ void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vecto
 
 typedef std::complex<double> cplx;
 
-// in-place unnormalised DFT of a strided line, sign +1 (FFTW_BACKWARD);
-// mixed radix 2/3/5 recursion, direct sums for other factors
-static void dft_rec(cplx *x, int n, int sign, std::vector<cplx> &tmp)
-{
-    if (n == 1) return;
-    int p = (n % 2 == 0) ? 2 : (n % 3 == 0) ? 3 : (n % 5 == 0) ? 5 : n;
-    if (p == n) {
-        std::vector<cplx> y(n);
-        for (int k = 0; k < n; k++) {
-            cplx s = 0;
-            for (int t = 0; t < n; t++) {
-                double a = sign * 2.0 * M_PI * (double)((long)k * t % n) / n;
-                s += x[t] * cplx(std::cos(a), std::sin(a));
-            }
-            y[k] = s;
-        }
-        for (int k = 0; k < n; k++) x[k] = y[k];
-        return;
-    }
-    int m = n / p;
-    std::vector<cplx> sub((size_t)n);
-    for (int r = 0; r < p; r++)
-        for (int t = 0; t < m; t++) sub[(size_t)r * m + t] = x[t * p + r];
-    for (int r = 0; r < p; r++) dft_rec(sub.data() + (size_t)r * m, m, sign, tmp);
-    for (int k = 0; k < n; k++) {
-        cplx s = 0;
-        for (int r = 0; r < p; r++) {
-            double a = sign * 2.0 * M_PI * (double)((long)r * k % n) / n;
-            s += sub[(size_t)r * m + (k % m)] * cplx(std::cos(a), std::sin(a));
-        }
-        x[k] = s;
-    }
-}
-
-static void ifft3(std::vector<cplx> &a, int nx, int ny, int nz)
-{
-    std::vector<cplx> line, tmp;
-    line.resize(std::max(nx, std::max(ny, nz)));
-    for (int k = 0; k < nz; k++)
-        for (int j = 0; j < ny; j++) {
-            cplx *p = a.data() + (size_t)nx * (j + (size_t)ny * k);
-            dft_rec(p, nx, +1, tmp);
-        }
-    for (int k = 0; k < nz; k++)
-        for (int i = 0; i < nx; i++) {
-            for (int j = 0; j < ny; j++) line[j] = a[i + (size_t)nx * (j + (size_t)ny * k)];
-            dft_rec(line.data(), ny, +1, tmp);
-            for (int j = 0; j < ny; j++) a[i + (size_t)nx * (j + (size_t)ny * k)] = line[j];
-        }
-    for (int j = 0; j < ny; j++)
-        for (int i = 0; i < nx; i++) {
-            for (int k = 0; k < nz; k++) line[k] = a[i + (size_t)nx * (j + (size_t)ny * k)];
-            dft_rec(line.data(), nz, +1, tmp);
-            for (int k = 0; k < nz; k++) a[i + (size_t)nx * (j + (size_t)ny * k)] = line[k];
-        }
-}
-
+// Seeded random-phase solenoidal field.  The spectrum
+// E(k) = 16 sqrt(2/pi) (k/k0)^4 / k0 exp(-2 (k/k0)^2) (src/block.cpp:458) is
+// populated for |k_x|,|k_y|,|k_z| <= K = min(n/2-1, 4 k0) -- beyond 4 k0 it is
+// below 1e-13 of its peak -- and synthesised by a separable inverse DFT with
+// FFTW's backward sign (e^{+i}) and no normalisation, keeping the real part
+// (src/block.cpp:504-510).  Phases come from mt19937_64(seed), three draws per
+// mode in (kz, ky, kx) order, instead of std::random_device (src/block.cpp:416-418).
 void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long long seed, std::vector<double> &q)
 {
-    const int nx = c.nx, ny = c.ny, nz = c.nz;
-    const long N = (long)nx * ny * nz;
+    const int n[3] = {c.nx, c.ny, c.nz};
+    const long N = (long)n[0] * n[1] * n[2];
     const double pie = 4.0 * std::atan(1.0);
+    int K[3];
+    for (int d = 0; d < 3; d++) K[d] = std::max(1, std::min(n[d] / 2 - 1, (int)std::ceil(4.0 * k0)));
+    const int W[3] = {2 * K[0] + 1, 2 * K[1] + 1, 2 * K[2] + 1};
     std::mt19937_64 gen(seed);
     std::uniform_real_distribution<double> dis(0.0, 2.0 * pie);
-    std::vector<cplx> U(N), V(N), W(N);
-    // spectrum and solenoidal random-phase amplitudes, src/block.cpp:448-499
-    for (int k = 0; k < nz; k++)
-        for (int j = 0; j < ny; j++)
-            for (int i = 0; i < nx; i++) {
-                long id = i + (long)nx * (j + (long)ny * k);
-                double kx0 = (i >= nx / 2.0) ? i - nx : i;
-                double ky0 = (j >= ny / 2.0) ? j - ny : j;
-                double kz0 = (k >= nz / 2.0) ? k - nz : k;
+    // spectral amplitudes, comp c at [(kz*W1 + ky)*W0 + kx]
+    std::vector<cplx> spec[3];
+    for (int cc = 0; cc < 3; cc++) spec[cc].assign((size_t)W[0] * W[1] * W[2], 0.0);
+    for (int kz = -K[2]; kz <= K[2]; kz++)
+        for (int ky = -K[1]; ky <= K[1]; ky++)
+            for (int kx = -K[0]; kx <= K[0]; kx++) {
+                double kx0 = kx, ky0 = ky, kz0 = kz;
                 double kmag = std::sqrt(kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
                 double k12 = std::sqrt(kx0 * kx0 + ky0 * ky0);
                 if (kmag < 1e-6) kmag = 1.0;
-                double Ek0 = 16.0 * std::sqrt(2.0 / pie) * std::pow(kmag / k0, 4) / k0 * std::exp(-2.0 * kmag * kmag / k0 / k0);
+                double Ek0 = 16.0 * std::sqrt(2.0 / pie) * std::pow(kmag / k0, 4) / k0 *
+                             std::exp(-2.0 * kmag * kmag / k0 / k0);
                 double phi1 = dis(gen), phi2 = dis(gen), phi3 = dis(gen);
                 double amp = std::sqrt(2.0 * Ek0 / kmag / kmag / 4.0 / pie);
                 cplx a(amp * std::cos(phi1) * std::cos(phi3), amp * std::sin(phi1) * std::cos(phi3));
                 cplx b(amp * std::cos(phi2) * std::sin(phi3), amp * std::sin(phi2) * std::sin(phi3));
+                if (kx == 0 && ky == 0 && kz == 0) { a = 0; b = 0; }  // no mean flow
                 double k212 = (k12 < 1e-6) ? 1.0 : ky0 / k12;
                 double k112 = (k12 < 1e-6) ? 0.0 : kx0 / k12;
-                if (i == 0 && j == 0 && k == 0) { a = 0; b = 0; }  // no mean flow
-                U[id] = k212 * a + k112 * kz0 / kmag * b;
-                V[id] = k212 * kz0 / kmag * b - k112 * a;
-                W[id] = -k12 / kmag * b;
+                size_t id = ((size_t)(kz + K[2]) * W[1] + (ky + K[1])) * W[0] + (kx + K[0]);
+                spec[0][id] = k212 * a + k112 * kz0 / kmag * b;   // src/block.cpp:488-493
+                spec[1][id] = k212 * kz0 / kmag * b - k112 * a;
+                spec[2][id] = -k12 / kmag * b;
             }
-    ifft3(U, nx, ny, nz);
-    ifft3(V, nx, ny, nz);
-    ifft3(W, nx, ny, nz);
+    // twiddles e^{+i 2 pi k x / n}
+    std::vector<cplx> tw[3];
+    for (int d = 0; d < 3; d++) {
+        tw[d].resize((size_t)W[d] * n[d]);
+        for (int k = -K[d]; k <= K[d]; k++)
+            for (int x = 0; x < n[d]; x++) {
+                long ph = ((long)k * x) % n[d];
+                if (ph < 0) ph += n[d];
+                double ang = 2.0 * pie * (double)ph / n[d];
+                tw[d][(size_t)(k + K[d]) * n[d] + x] = cplx(std::cos(ang), std::sin(ang));
+            }
+    }
+    std::vector<double> vel[3];
+    for (int cc = 0; cc < 3; cc++) {
+        // x pass: g[kz][ky][x]
+        std::vector<cplx> g((size_t)W[2] * W[1] * n[0]);
+        for (int kz = 0; kz < W[2]; kz++)
+            for (int ky = 0; ky < W[1]; ky++)
+                for (int x = 0; x < n[0]; x++) {
+                    cplx acc = 0;
+                    const cplx *sp = &spec[cc][((size_t)kz * W[1] + ky) * W[0]];
+                    for (int kx = 0; kx < W[0]; kx++) acc += sp[kx] * tw[0][(size_t)kx * n[0] + x];
+                    g[((size_t)kz * W[1] + ky) * n[0] + x] = acc;
+                }
+        // y pass: h[kz][y][x]
+        std::vector<cplx> h((size_t)W[2] * n[1] * n[0]);
+#pragma omp parallel for schedule(static)
+        for (int kz = 0; kz < W[2]; kz++)
+            for (int y = 0; y < n[1]; y++)
+                for (int x = 0; x < n[0]; x++) {
+                    cplx acc = 0;
+                    for (int ky = 0; ky < W[1]; ky++)
+                        acc += g[((size_t)kz * W[1] + ky) * n[0] + x] * tw[1][(size_t)ky * n[1] + y];
+                    h[((size_t)kz * n[1] + y) * n[0] + x] = acc;
+                }
+        // z pass, real part
+        vel[cc].assign(N, 0.0);
+#pragma omp parallel for schedule(static)
+        for (int z = 0; z < n[2]; z++)
+            for (int y = 0; y < n[1]; y++)
+                for (int x = 0; x < n[0]; x++) {
+                    double acc = 0;
+                    for (int kz = 0; kz < W[2]; kz++) {
+                        const cplx hv = h[((size_t)kz * n[1] + y) * n[0] + x];
+                        const cplx t = tw[2][(size_t)kz * n[2] + z];
+                        acc += hv.real() * t.real() - hv.imag() * t.imag();
+                    }
+                    vel[cc][(size_t)x + (size_t)n[0] * (y + (size_t)n[1] * z)] = acc;
+                }
+    }
     double s2 = 0;
-    for (long id = 0; id < N; id++) s2 += U[id].real() * U[id].real() + V[id].real() * V[id].real() + W[id].real() * W[id].real();
-    double vel = std::sqrt(s2 / N / 3.0);
-    double scale = (vel > 0) ? u_rms / vel : 0.0;
+    for (long id = 0; id < N; id++) s2 += vel[0][id] * vel[0][id] + vel[1][id] * vel[1][id] + vel[2][id] * vel[2][id];
+    double v = std::sqrt(s2 / N / 3.0);
+    double scale = (v > 0) ? u_rms / v : 0.0;
     q.assign(5 * N, 0.0);
     double p0 = 1.0 / (c.gamma * c.mach * c.mach);
     for (long id = 0; id < N; id++) {
-        double u0 = U[id].real() * scale, v0 = V[id].real() * scale, w0 = W[id].real() * scale;
+        double u0 = vel[0][id] * scale, v0 = vel[1][id] * scale, w0 = vel[2][id] * scale;
         q[id] = 1.0;
         q[N + id] = u0;
         q[2 * N + id] = v0;

@@ -27,7 +27,7 @@ STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
            "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_sync", "nuc_stats",
            "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp", "nuc_init_tgv",
-           "nuc_init_hit"]
+           "nuc_init_hit", "nuc_profile", "nuc_profile_read"]
 
 
 class NucConfig(ctypes.Structure):
@@ -90,6 +90,9 @@ def lib():
         L.nuc_init_tgv.argtypes = [ctypes.POINTER(NucConfig), _dp]
         L.nuc_init_hit.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_double,
                                    ctypes.c_ulonglong, _dp]
+        L.nuc_profile.argtypes = [vp, ctypes.c_int]
+        L.nuc_profile_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long),
+                                       ctypes.POINTER(ctypes.c_char_p)]
         _lib = L
     return _lib
 
@@ -209,6 +212,21 @@ class Context:
         info = StepInfo()
         _check(lib().nuc_sync(self.h, ctypes.byref(info)), "nuc_sync", self.h)
         return info.dt, info.res, info.time
+
+    def profile(self, enable=True):
+        _check(lib().nuc_profile(self.h, 1 if enable else 0), "nuc_profile", self.h)
+
+    def profile_read(self):
+        """{kernel: (total_ms, launches)} accumulated since profile(True)."""
+        out = {}
+        for slot in range(7):
+            ms = ctypes.c_double()
+            n = ctypes.c_long()
+            name = ctypes.c_char_p()
+            _check(lib().nuc_profile_read(self.h, slot, ctypes.byref(ms), ctypes.byref(n), ctypes.byref(name)),
+                   "nuc_profile_read", self.h)
+            out[name.value.decode()] = (ms.value, n.value)
+        return out
 
     def stats(self):
         out = np.zeros(25)
