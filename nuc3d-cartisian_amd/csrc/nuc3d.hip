@@ -183,7 +183,7 @@ __global__ __launch_bounds__(256) void k_grad(GradArgs A)
             double L[9];
 #pragma unroll
             for (int o = 0; o < 9; o++) L[o] = A.prim[f][offs[o]];
-            g[f][d] = central_diff<double>(L + 4, 0, cl, A.nloc[d], A.tsv[d], A.vis[d], A.inv_h[d]);
+            g[f][d] = central_diff<double>(L + 4, 0, cl >= A.tsv[d] && cl < A.nloc[d] - A.tsv[d], A.vis[d], A.inv_h[d]);
         }
     }
     double ux = g[0][0], uy = g[0][1], uz = g[0][2];
@@ -218,187 +218,190 @@ __global__ __launch_bounds__(256) void k_grad(GradArgs A)
 
 // ---------------------------------------------------------------------------
 // the direction sweep
+//
+// Load mapping (global <-> LDS): for the x sweep thread t owns cells
+// lane + 64k of line t/64 (contiguous, coalesced); for y and z the LPB lines
+// of a workgroup are consecutive in x and thread t owns cells t/LPB + 64k of
+// line t%LPB, so each load instruction reads LPB consecutive doubles per cell.
+// Compute mapping: wave w owns line w (both signs, see nuc_lines.h).
 // ---------------------------------------------------------------------------
-template <int SCH, int CPT, int LPB>
+template <int RIE>
+__device__ __forceinline__ void split_params(const double q[5], int d, double gamma, double sp[5])
+{
+    const double gm1 = gamma - 1.0;
+    double rho = q[0], ir = 1.0 / rho;
+    double u = q[1] * ir, vv = q[2] * ir, w = q[3] * ir;
+    double p = fabs((q[4] - 0.5 * rho * (u * u + vv * vv + w * w)) * gm1);
+    double cs = sqrt(gamma * p * ir);
+    double ud = (d == 0) ? u : (d == 1 ? vv : w);
+    if constexpr (RIE == RIE_LF) {
+        sp[0] = ud; sp[1] = fabs(ud) + cs; sp[2] = p; sp[3] = 0.0; sp[4] = 0.0;
+    } else {
+        // AUSM Mach / pressure splittings, src/physicsModel.cpp:1181-1228
+        double mach = ud / cs;
+        double mp, mn, pp, pn;
+        if (fabs(mach) < 1.0) {
+            mp = 0.25 * sq(mach + 1.0) + 0.125 * sq(mach * mach - 1.0);
+            mn = -0.25 * sq(mach - 1.0) - 0.125 * sq(mach * mach - 1.0);
+            pp = p * (mach + 1.0) / 2.0;
+            pn = -p * (mach - 1.0) / 2.0;
+        } else {
+            mp = 0.5 * (mach + fabs(mach));
+            mn = 0.5 * (mach - fabs(mach));
+            pp = 0.5 * p * (mach + fabs(mach)) / mach;
+            pn = 0.5 * p * (mach - fabs(mach)) / mach;
+        }
+        sp[0] = mp * cs; sp[1] = mn * cs; sp[2] = pp; sp[3] = pn; sp[4] = p;
+    }
+}
+
+// split fluxes of variable v at one cell (LF src/physicsModel.cpp:540-565,
+// AUSM :790-820, per direction d)
+template <int RIE>
+__device__ __forceinline__ void split_var(const double *sp, double qv, int v, int d, double &fp, double &fm)
+{
+    if constexpr (RIE == RIE_LF) {
+        double ud = sp[0], lam = sp[1], p = sp[2];
+        double F = ud * qv;
+        if (v == 1 + d) F = ud * qv + p;
+        if (v == 4) F = ud * (qv + p);
+        fp = 0.5 * (F + lam * qv);
+        fm = 0.5 * (F - lam * qv);
+    } else {
+        fp = sp[0] * qv;
+        fm = sp[1] * qv;
+        if (v == 1 + d) { fp += sp[2]; fm += sp[3]; }
+        if (v == 4) { fp = sp[0] * qv + sp[0] * sp[4]; fm = sp[1] * qv + sp[1] * sp[4]; }
+    }
+}
+
+template <int SCH, int RIE, int R, int LPB>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
-    constexpr int NP = 64 * CPT + 2 * HALO;
+    constexpr int NP = 32 * R + 2 * HALO;
+    constexpr int CPL = R / 2;             // cells per thread in the load mapping
+    constexpr int NSP = (RIE == RIE_LF) ? 3 : 5;
+    constexpr int RAD = Radius<SCH>::value;
     __shared__ double sP[LPB * NP];
     __shared__ double sM[LPB * NP];
+    __shared__ double sV[LPB * NP];
     __shared__ double sRed[LPB];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int n = A.n;
+    const int n = A.n, d = A.dir;
     int ll, g;
-    if (A.dir == 0) { ll = wave; g = lane; }
+    if (d == 0) { ll = wave; g = lane; }
     else { ll = t % LPB; g = t / LPB; }
     const long L = (long)blockIdx.x * LPB + ll;
     const bool lv = L < A.nlines;
     long base = 0, stride = 1;
-    if (A.dir == 0) { base = L * A.nx; stride = 1; }
-    else if (A.dir == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
+    if (d == 0) { base = L * A.nx; stride = 1; }
+    else if (d == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
     else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
-    const int d = A.dir;
-    const double gm1 = A.gamma - 1.0;
 
-    double q[CPT][5], s[CPT][5], rhs[CPT][5];
+    // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344)
+    double sp[CPL][NSP];
+    unsigned innermask = 0;   // cd8 inner-form cells (rank-local position)
 #pragma unroll
-    for (int kk = 0; kk < CPT; kk++) {
-        int c = g + 64 * kk;
+    for (int k = 0; k < CPL; k++) {
+        int c = g + 64 * k;
         bool ok = lv && c < n;
         long id = base + (long)c * stride;
+        double q[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) {
-            q[kk][v] = ok ? A.q[v][id] : 1.0;
-            rhs[kk][v] = (ok && d > 0) ? A.rhs[v][id] : 0.0;
-        }
-        double rho = q[kk][0], ir = 1.0 / rho;
-        double u = q[kk][1] * ir, vv = q[kk][2] * ir, w = q[kk][3] * ir;
-        double p = fabs((q[kk][4] - 0.5 * rho * (u * u + vv * vv + w * w)) * gm1);
-        double cs = sqrt(A.gamma * p * ir);
-        double ud = (d == 0) ? u : (d == 1 ? vv : w);
-        if (A.riemann == RIE_LF) {
-            s[kk][0] = ud; s[kk][1] = fabs(ud) + cs; s[kk][2] = p; s[kk][3] = 0.0; s[kk][4] = 0.0;
-        } else {
-            // AUSM splittings, src/physicsModel.cpp:1181-1228
-            double mach = ud / cs;
-            double mp, mn, pp, pn;
-            if (fabs(mach) < 1.0) {
-                mp = 0.25 * sq(mach + 1.0) + 0.125 * sq(mach * mach - 1.0);
-                mn = -0.25 * sq(mach - 1.0) - 0.125 * sq(mach * mach - 1.0);
-                pp = p * (mach + 1.0) / 2.0;
-                pn = -p * (mach - 1.0) / 2.0;
-            } else {
-                mp = 0.5 * (mach + fabs(mach));
-                mn = 0.5 * (mach - fabs(mach));
-                pp = 0.5 * p * (mach + fabs(mach)) / mach;
-                pn = 0.5 * p * (mach - fabs(mach)) / mach;
-            }
-            s[kk][0] = mp * cs; s[kk][1] = mn * cs; s[kk][2] = pp; s[kk][3] = pn; s[kk][4] = p;
-        }
+        for (int v = 0; v < 5; v++) q[v] = ok ? A.q[v][id] : (v == 0 ? 1.0 : 0.0);
+        double tmp[5];
+        split_params<RIE>(q, d, A.gamma, tmp);
+#pragma unroll
+        for (int j = 0; j < NSP; j++) sp[k][j] = tmp[j];
+        int cl = c % A.nloc;
+        if (cl >= A.tsv && cl < A.nloc - A.tsv) innermask |= 1u << k;
     }
 
-    double *Pl = sP + ll * NP + HALO;   // load-mapping view
-    double *Ml = sM + ll * NP + HALO;
-    const double *Pw = sP + wave * NP + HALO;  // compute-mapping view (wave = line)
-    const double *Mw = sM + wave * NP + HALO;
+    double *Pl = sP + ll * NP + HALO, *Ml = sM + ll * NP + HALO, *Vl = sV + ll * NP + HALO;
+    double *Pw = sP + wave * NP + HALO, *Mw = sM + wave * NP + HALO;
     const bool wv = (long)blockIdx.x * LPB + wave < A.nlines;
     const Zone z{n, A.nloc, A.ts};
+    // compute-mapping geometry of this lane (nuc_lines.h)
+    const int h = lane & 31;
+    const bool neg = lane >= 32;
+    const int chunk = neg ? 31 - h : h;
+    const int m0 = chunk * R;
+    const double *Bw = neg ? (Mw + m0 + R - 1 + RAD) : (Pw + m0 - RAD - 1);
+    const int S = neg ? -1 : 1;
+    const int m0r = neg ? m0 + R - 1 : m0;
+    const int mstep = neg ? -1 : 1;
+    double res = 0.0;
+    const double dt = A.epilogue ? *A.dtp : 0.0;
 
 #pragma unroll 1
     for (int v = 0; v < 5; v++) {
-        // split fluxes f+/f- of variable v at the own cells (src/physicsModel.cpp:540-565, 790-820)
+        double qv[CPL];
 #pragma unroll
-        for (int kk = 0; kk < CPT; kk++) {
-            int c = g + 64 * kk;
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
+            qv[k] = 0.0;
             if (lv && c < n) {
-                double qv = q[kk][v], fp, fm;
-                if (A.riemann == RIE_LF) {
-                    double ud = s[kk][0], lam = s[kk][1], p = s[kk][2];
-                    double F = ud * qv;
-                    if (v == 1 + d) F = ud * qv + p;
-                    if (v == 4) F = ud * (qv + p);
-                    fp = 0.5 * (F + lam * qv);
-                    fm = 0.5 * (F - lam * qv);
-                } else {
-                    double mpa = s[kk][0], mna = s[kk][1];
-                    fp = mpa * qv;
-                    fm = mna * qv;
-                    if (v == 1 + d) { fp += s[kk][2]; fm += s[kk][3]; }
-                    if (v == 4) { fp = mpa * qv + mpa * s[kk][4]; fm = mna * qv + mna * s[kk][4]; }
-                }
+                long id = base + (long)c * stride;
+                qv[k] = A.q[v][id];
+                double fp, fm;
+                split_var<RIE>(sp[k], qv[k], v, d, fp, fm);
                 Pl[c] = fp; Ml[c] = fm;
                 if (c < HALO) { Pl[c + n] = fp; Ml[c + n] = fm; }
                 if (c >= n - HALO) { Pl[c - n] = fp; Ml[c - n] = fm; }
+                if (A.vis != VIS_NONE && v > 0) {
+                    double x = A.fv[v - 1][id];
+                    Vl[c] = x;
+                    if (c < HALO) Vl[c + n] = x;
+                    if (c >= n - HALO) Vl[c - n] = x;
+                }
             }
         }
         __syncthreads();
-        double Fp[CPT], Fm[CPT];
-        if (wv) line_interfaces<SCH, CPT, double>(Pw, Mw, z, lane, Fp, Fm);
+        double out[R];
+        if (wv) line_half<SCH, R, double>(Bw, S, m0r, mstep, z, h, neg, out);
         __syncthreads();
         if (wv) {
-            double *Pwr = sP + wave * NP + HALO;
+            double *Ow = neg ? Mw : Pw;
 #pragma unroll
-            for (int r = 0; r < CPT; r++) {
-                int m = lane * CPT + r;
+            for (int r = 0; r < R; r++) {
+                int m = m0r + mstep * r;
                 if (m < n) {
-                    double F = Fp[r] + Fm[r];
-                    Pwr[m] = F;
-                    if (m == 0) Pwr[n] = F;
+                    Ow[m] = out[r];
+                    if (m == 0) Ow[n] = out[r];
                 }
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < CPT; kk++) {
-            int c = g + 64 * kk;
-            if (lv && c < n) rhs[kk][v] += (Pl[c + 1] - Pl[c]) * A.inv_h;
-        }
-        __syncthreads();
-    }
-
-    if (A.vis != VIS_NONE) {
-        // -D_dir of the viscous flux components 1..4 (src/NaiverStokes3d.cpp:305-369)
-#pragma unroll 1
-        for (int v = 1; v < 5; v++) {
-            const double *fv = A.fv[v - 1];
-#pragma unroll
-            for (int kk = 0; kk < CPT; kk++) {
-                int c = g + 64 * kk;
-                if (lv && c < n) {
-                    double x = fv[base + (long)c * stride];
-                    Pl[c] = x;
-                    if (c < HALO) Pl[c + n] = x;
-                    if (c >= n - HALO) Pl[c - n] = x;
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < CPT; kk++) {
-                int c = g + 64 * kk;
-                if (lv && c < n)
-                    rhs[kk][v] -= central_diff<double>(Pl, c, c % A.nloc, A.nloc, A.tsv, A.vis, A.inv_h);
-            }
-            __syncthreads();
-        }
-    }
-
-    if (!A.epilogue) {
-#pragma unroll
-        for (int kk = 0; kk < CPT; kk++) {
-            int c = g + 64 * kk;
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
             if (lv && c < n) {
                 long id = base + (long)c * stride;
-#pragma unroll
-                for (int v = 0; v < 5; v++) A.rhs[v][id] = rhs[kk][v];
-            }
-        }
-        return;
-    }
-    // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (stage 0), :97-101 (1), :140-144 (2)
-    const double dt = *A.dtp;
-    double res = 0.0;
-#pragma unroll
-    for (int kk = 0; kk < CPT; kk++) {
-        int c = g + 64 * kk;
-        if (lv && c < n) {
-            long id = base + (long)c * stride;
-#pragma unroll
-            for (int v = 0; v < 5; v++) {
-                double rr = -rhs[kk][v];
-                double nu;
-                if (A.stage == 0) {
-                    nu = q[kk][v] + rr * dt;
+                double r = (d > 0) ? A.rhs[v][id] : 0.0;
+                // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
+                r += ((Pl[c + 1] + Ml[c + 1]) - (Pl[c] + Ml[c])) * A.inv_h;
+                if (A.vis != VIS_NONE && v > 0)
+                    r -= central_diff<double>(Vl, c, (innermask >> k) & 1u, A.vis, A.inv_h);
+                if (!A.epilogue) {
+                    A.rhs[v][id] = r;
                 } else {
-                    double un = A.qn[v][id];
-                    if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + q[kk][v]);
-                    else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + q[kk][v]);
-                    if (A.stage == 2) res += sq(nu - un);
+                    // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (0), :97-101 (1), :140-144 (2)
+                    double rr = -r, nu;
+                    if (A.stage == 0) {
+                        nu = qv[k] + rr * dt;
+                    } else {
+                        double un = A.qn[v][id];
+                        if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + qv[k]);
+                        else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qv[k]);
+                        if (A.stage == 2) res += sq(nu - un);
+                    }
+                    A.qout[v][id] = nu;
                 }
-                A.qout[v][id] = nu;
             }
         }
+        __syncthreads();
     }
-    if (A.stage == 2) {
+    if (A.epilogue && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) res += __shfl_down(res, o);
@@ -466,7 +469,7 @@ __global__ __launch_bounds__(256) void k_stats(StatArgs A)
                     int p = wrapi(pos[d] + o, ext[d]);
                     L[o + 4] = A.prim[f][id + (long)(p - pos[d]) * str[d]];
                 }
-                g[f][d] = central_diff<double>(L + 4, 0, cl, A.nloc[d], A.tsv[d], A.vis[d], A.inv_h[d]);
+                g[f][d] = central_diff<double>(L + 4, 0, cl >= A.tsv[d] && cl < A.nloc[d] - A.tsv[d], A.vis[d], A.inv_h[d]);
             }
         }
         double ux = g[0][0], uy = g[0][1], uz = g[0][2];
@@ -522,16 +525,17 @@ struct OpRecArgs {
     int n1, nlines, bf, ts, upwind, zones;
 };
 
-template <int SCH, int CPT>
+template <int SCH, int R>
 __global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
 {
-    constexpr int NP = 64 * CPT + 2 * HALO;
+    constexpr int NP = 32 * R + 2 * HALO;
+    constexpr int RAD = Radius<SCH>::value;
     __shared__ double sL[NP];
     const int lane = threadIdx.x;
     const long line = blockIdx.x;
     const int n = A.n1;
     double *L = sL + HALO;
-    for (int c = lane - HALO; c < 64 * CPT + HALO; c += 64) {
+    for (int c = lane - HALO; c < 32 * R + HALO; c += 64) {
         double x = 0.0;
         if (c >= 0 && c < n) x = A.in[line * n + c];
         else if (c < 0 && A.gL && -c <= A.bf) x = A.gL[line * A.bf + (A.bf + c)];
@@ -539,14 +543,22 @@ __global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
         L[c] = x;
     }
     __syncthreads();
-    double Fp[CPT], Fm[CPT];
+    const int h = lane & 31;
+    const bool neg = lane >= 32;
+    const int chunk = neg ? 31 - h : h;
+    const int m0 = chunk * R;
+    const double *B = neg ? (L + m0 + R - 1 + RAD) : (L + m0 - RAD - 1);
+    const int m0r = neg ? m0 + R - 1 : m0;
+    const int mstep = neg ? -1 : 1;
     const Zone z{n + 1, n, A.ts};
-    line_interfaces<SCH, CPT, double>(L, L, z, lane, Fp, Fm);
-    for (int r = 0; r < CPT; r++) {
-        int m = lane * CPT + r;
+    double out[R];
+    line_half<SCH, R, double>(B, neg ? -1 : 1, m0r, mstep, z, h, neg, out);
+    if (neg != (A.upwind < 0)) return;
+    for (int r = 0; r < R; r++) {
+        int m = m0r + mstep * r;
         if (m > n) continue;
         int zbit = (m < A.ts) ? 2 : (m > n - A.ts ? 4 : 1);
-        if (A.zones & zbit) A.out[line * (n + 1) + m] = (A.upwind > 0) ? Fp[r] : Fm[r];
+        if (A.zones & zbit) A.out[line * (n + 1) + m] = out[r];
     }
 }
 
@@ -576,7 +588,7 @@ __global__ void k_op_diff(OpDiffArgs A)
         else if (cc >= n && A.gR && cc - n < A.bf) x = A.gR[line * A.bf + (cc - n)];
         Lb[o + 4] = x;
     }
-    A.out[gid] = central_diff<double>(Lb + 4, 0, c, n, A.ts, A.vis, A.inv_h);
+    A.out[gid] = central_diff<double>(Lb + 4, 0, c >= A.ts && c < n - A.ts, A.vis, A.inv_h);
 }
 
 __global__ void k_op_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n)
@@ -592,65 +604,75 @@ __global__ void k_op_rk3(int stage, const double *rhs, const double *un, double 
 // ---------------------------------------------------------------------------
 // dispatch helpers
 // ---------------------------------------------------------------------------
-int cpt_for(int n)
+// rows per lane of a half wave for a line of nrows interfaces (32 lanes per sign)
+int rows_for(int nrows)
 {
-    int c = (n + 63) / 64;
-    if (c <= 4) return c;
-    if (c <= 6) return 6;
-    if (c <= 8) return 8;
+    int r = (nrows + 31) / 32;
+    if (r <= 2) return 2;
+    if (r <= 4) return 4;
+    if (r <= 6) return 6;
+    if (r <= 8) return 8;
+    if (r <= 16) return 16;
     return -1;
 }
 
-template <int SCH, int CPT>
-void launch_sweep_cpt(const SweepArgs &A, hipStream_t s)
+constexpr int lpb_for(int R) { return R <= 8 ? 8 : 4; }
+
+template <int SCH, int RIE, int R>
+void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
-    constexpr int LPB = (CPT <= 4) ? 8 : 4;
+    constexpr int LPB = lpb_for(R);
     long nb = (A.nlines + LPB - 1) / LPB;
-    hipLaunchKernelGGL((k_sweep<SCH, CPT, LPB>), dim3((unsigned)nb), dim3(64 * LPB), 0, s, A);
+    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, LPB>), dim3((unsigned)nb), dim3(64 * LPB), 0, s, A);
 }
 
-template <int SCH>
-void launch_sweep_sch(int cpt, const SweepArgs &A, hipStream_t s)
+template <int SCH, int RIE>
+void launch_sweep_sr(int R, const SweepArgs &A, hipStream_t s)
 {
-    switch (cpt) {
-    case 1: launch_sweep_cpt<SCH, 1>(A, s); break;
-    case 2: launch_sweep_cpt<SCH, 2>(A, s); break;
-    case 3: launch_sweep_cpt<SCH, 3>(A, s); break;
-    case 4: launch_sweep_cpt<SCH, 4>(A, s); break;
-    case 6: launch_sweep_cpt<SCH, 6>(A, s); break;
-    default: launch_sweep_cpt<SCH, 8>(A, s); break;
+    switch (R) {
+    case 2: launch_sweep_r<SCH, RIE, 2>(A, s); break;
+    case 4: launch_sweep_r<SCH, RIE, 4>(A, s); break;
+    case 6: launch_sweep_r<SCH, RIE, 6>(A, s); break;
+    case 8: launch_sweep_r<SCH, RIE, 8>(A, s); break;
+    default: launch_sweep_r<SCH, RIE, 16>(A, s); break;
     }
 }
 
-void launch_sweep(int scheme, int cpt, const SweepArgs &A, hipStream_t s)
+template <int SCH>
+void launch_sweep_s(int R, const SweepArgs &A, hipStream_t s)
+{
+    if (A.riemann == RIE_LF) launch_sweep_sr<SCH, RIE_LF>(R, A, s);
+    else launch_sweep_sr<SCH, RIE_AUSM>(R, A, s);
+}
+
+void launch_sweep(int scheme, int R, const SweepArgs &A, hipStream_t s)
 {
     switch (scheme) {
-    case WENO5JS: launch_sweep_sch<WENO5JS>(cpt, A, s); break;
-    case WENO5Z: launch_sweep_sch<WENO5Z>(cpt, A, s); break;
-    case WENO7JS: launch_sweep_sch<WENO7JS>(cpt, A, s); break;
-    case CRWENO5: launch_sweep_sch<CRWENO5>(cpt, A, s); break;
-    default: launch_sweep_sch<HCCS>(cpt, A, s); break;
+    case WENO5JS: launch_sweep_s<WENO5JS>(R, A, s); break;
+    case WENO5Z: launch_sweep_s<WENO5Z>(R, A, s); break;
+    case WENO7JS: launch_sweep_s<WENO7JS>(R, A, s); break;
+    case CRWENO5: launch_sweep_s<CRWENO5>(R, A, s); break;
+    default: launch_sweep_s<HCCS>(R, A, s); break;
     }
 }
 
-int sweep_lpb(int cpt) { return cpt <= 4 ? 8 : 4; }
+int sweep_lpb(int R) { return lpb_for(R); }
 
-template <int SCH, int CPT>
-void launch_op_cpt(const OpRecArgs &A, hipStream_t s)
+template <int SCH, int R>
+void launch_op_r(const OpRecArgs &A, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_op_recon<SCH, CPT>), dim3(A.nlines), dim3(64), 0, s, A);
+    hipLaunchKernelGGL((k_op_recon<SCH, R>), dim3(A.nlines), dim3(64), 0, s, A);
 }
 
 template <int SCH>
-void launch_op_sch(int cpt, const OpRecArgs &A, hipStream_t s)
+void launch_op_sch(int R, const OpRecArgs &A, hipStream_t s)
 {
-    switch (cpt) {
-    case 1: launch_op_cpt<SCH, 1>(A, s); break;
-    case 2: launch_op_cpt<SCH, 2>(A, s); break;
-    case 3: launch_op_cpt<SCH, 3>(A, s); break;
-    case 4: launch_op_cpt<SCH, 4>(A, s); break;
-    case 6: launch_op_cpt<SCH, 6>(A, s); break;
-    default: launch_op_cpt<SCH, 8>(A, s); break;
+    switch (R) {
+    case 2: launch_op_r<SCH, 2>(A, s); break;
+    case 4: launch_op_r<SCH, 4>(A, s); break;
+    case 6: launch_op_r<SCH, 6>(A, s); break;
+    case 8: launch_op_r<SCH, 8>(A, s); break;
+    default: launch_op_r<SCH, 16>(A, s); break;
     }
 }
 
@@ -773,7 +795,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
         // cd8 needs four ghost cells; the reference hands it the inviscid halo
         // (src/fieldOperator.cpp:520-526) and reads outside the strip when that is 3
         x->tsv[d] = (cfg->vis_scheme[d] == NUC_CD8 && x->ts[d] < 4) ? 4 : x->ts[d];
-        x->cpt[d] = cpt_for(x->n[d]);
+        x->cpt[d] = rows_for(x->n[d]);
         if (cfg->vis_scheme[d] == NUC_CD8) any_cd8 = true;
     }
     (void)any_cd8;
@@ -1069,7 +1091,7 @@ extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const do
     if (!in || !out || n1 < 2 * tilesize + 1 || n2 < 1 || n3 < 1 || (upwind != 1 && upwind != -1)) return NUC_EINVAL;
     if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < tilesize)) return NUC_EINVAL;
     if (scheme < 0 || scheme > 4 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
-    int cpt = cpt_for(n1 + 1);
+    int cpt = rows_for(n1 + 1);
     if (cpt < 0) return NUC_EINVAL;
     long nl = (long)n2 * n3;
     DevBuf di, dl, dr, dout;

@@ -1,12 +1,19 @@
-// nuc_lines.h -- the per-line interface computation shared by the production
+// nuc_lines.h -- per-line interface computation shared by the production
 // sweep kernels and the per-Field operator kernels.
 //
-// One wave owns one line held in LDS on [-HALO, n+HALO).  Lane k owns the
-// interfaces m = k*CPT + r.  Rank-local zones follow the reference's
-// inner/boundary split per MPI block of length nloc (src/fieldOperator.cpp:387-465):
+// One wave owns one line held in LDS on [-HALO, 32R+HALO).  Lanes 0..31
+// reconstruct f^+ and lanes 32..63 reconstruct f^- (src/euler3d.cpp:334-387
+// does the two as separate sweeps).  Lane h of a half owns a chunk of R
+// consecutive interfaces; the f^- half takes the chunks in mirrored order
+// (lane h <-> chunk 31-h, rows reversed), which turns the f^- stencils and its
+// tridiagonal system into the same shape as f^+, so both halves execute one
+// instruction stream.
+//
+// Rank-local zones follow the reference's inner/boundary split per MPI block
+// of length nloc (src/fieldOperator.cpp:387-465):
 //   explicit boundary interfaces  ml in [0, ts-1] U [nloc-ts+1, nloc]
 //   inner interfaces              ml in [ts, nloc-ts]
-//   compact schemes: rows ml = ts and ml = nloc-ts are explicit closures,
+//   compact schemes: ml = ts and ml = nloc-ts are explicit closure rows,
 //   ts < ml < nloc-ts are tridiagonal rows (src/crweno5.cpp:83-206).
 #pragma once
 #include "nuc_device.h"
@@ -16,104 +23,236 @@ namespace nucdev {
 constexpr int HALO = 4;
 
 struct Zone {
-    int nrows;  // interfaces computed on this line (n periodic, n+1 with ghosts)
+    int nrows;  // interfaces on this line (n periodic, n+1 with ghost strips)
     int nloc;   // rank-local line length
     int ts;     // inviscid tilesize of the scheme
 };
 
-template <int SCH, typename T>
-__device__ __forceinline__ T explicit_pos(const T *P, int m, bool inner)
+// reciprocal: hardware estimate refined by two Newton steps (|err| <= 1 ulp)
+__device__ __forceinline__ double frcp(double x)
 {
-    if constexpr (SCH == WENO5JS || SCH == CRWENO5) {
-        return weno5js<T>(P[m - 3], P[m - 2], P[m - 1], P[m], P[m + 1]);
-    } else if constexpr (SCH == WENO5Z) {
-        return weno5z<T>(P[m - 3], P[m - 2], P[m - 1], P[m], P[m + 1], inner);
-    } else {
-        T f[7] = {P[m - 4], P[m - 3], P[m - 2], P[m - 1], P[m], P[m + 1], P[m + 2]};
-        return weno7z<T>(f, SCH == WENO7JS ? T(1.0e-20) : T(1.0e-40));
-    }
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
 
-template <int SCH, typename T>
-__device__ __forceinline__ T explicit_neg(const T *M, int m)
-{
-    if constexpr (SCH == WENO5JS || SCH == CRWENO5) {
-        return weno5js<T>(M[m + 2], M[m + 1], M[m], M[m - 1], M[m - 2]);
-    } else if constexpr (SCH == WENO5Z) {
-        return weno5z<T>(M[m + 2], M[m + 1], M[m], M[m - 1], M[m - 2], false);
-    } else {
-        T f[7] = {M[m + 3], M[m + 2], M[m + 1], M[m], M[m - 1], M[m - 2], M[m - 3]};
-        return weno7z<T>(f, SCH == WENO7JS ? T(1.0e-20) : T(1.0e-40));
-    }
-}
+template <int SCH> struct Radius { static constexpr int value = (SCH == WENO7JS || SCH == HCCS) ? 3 : 2; };
 
-// Reconstructed f^+ (into Fp) and f^- (into Fm) at the lane's interfaces.
-// zone_mask: bit0 = inner interfaces, bit1 = boundary interfaces (used by the
-// operator hooks; the sweeps pass 3).  Interfaces not selected are left
-// untouched in Fp/Fm.
-template <int SCH, int CPT, typename T>
-__device__ __forceinline__ void line_interfaces(const T *P, const T *M, const Zone &z, int lane,
-                                                T Fp[CPT], T Fm[CPT])
+// normalised parallel cyclic reduction over the 32 lanes of a half wave:
+// row k: x_k + al*x_{k-1} + ga*x_{k+1} = de  ->  x_k = de on return
+template <typename T>
+__device__ __forceinline__ void pcr32(T &al, T &ga, T &de, int h)
 {
-    if constexpr (SCH == CRWENO5 || SCH == HCCS) {
-        T ap[CPT], bp[CPT], cp[CPT], dp[CPT];
-        T am[CPT], bm[CPT], cm[CPT], dm[CPT];
 #pragma unroll
-        for (int r = 0; r < CPT; r++) {
-            int m = lane * CPT + r;
-            ap[r] = T(0); bp[r] = T(1); cp[r] = T(0); dp[r] = T(0);
-            am[r] = T(0); bm[r] = T(1); cm[r] = T(0); dm[r] = T(0);
-            if (m < z.nrows) {
-                int ml = m % z.nloc;
-                if (ml <= z.ts || ml >= z.nloc - z.ts) {
-                    dp[r] = explicit_pos<SCH, T>(P, m, true);
-                    dm[r] = explicit_neg<SCH, T>(M, m);
+    for (int s = 1; s < 32; s <<= 1) {
+        T alm = __shfl_up(al, s, 32), gam = __shfl_up(ga, s, 32), dem = __shfl_up(de, s, 32);
+        T alp = __shfl_down(al, s, 32), gap = __shfl_down(ga, s, 32), dep = __shfl_down(de, s, 32);
+        if (h < s) { alm = T(0); gam = T(0); dem = T(0); }
+        if (h + s >= 32) { alp = T(0); gap = T(0); dep = T(0); }
+        T r = frcp(T(1) - al * gam - ga * alp);
+        T nal = -r * al * alm;
+        T nga = -r * ga * gap;
+        T nde = r * (de - al * dem - ga * dep);
+        al = nal; ga = nga; de = nde;
+    }
+}
+
+// Forward-eliminated rows of one lane's chunk (rows normalised):
+//   r = 0: Af x_{-1} + x_0 + Cf x_1 = Df ;  r >= 1: Af x_0 + x_r + Cf x_{r+1} = Df
+template <int R, typename T> struct Chunk {
+    T Af[R], Cf[R], Df[R];
+    __device__ __forceinline__ void row(int r, T a, T b, T c, T d)
+    {
+        if (r <= 1) {
+            T rb = frcp(b);
+            Af[r] = a * rb; Cf[r] = c * rb; Df[r] = d * rb;
+        } else {
+            T rr = frcp(b - a * Cf[r - 1]);
+            Df[r] = rr * (d - a * Df[r - 1]);
+            Af[r] = -rr * a * Af[r - 1];
+            Cf[r] = rr * c;
+        }
+    }
+    // Partition solve across the half wave (lane h holds chunk h in system
+    // order; the first row of lane 0 and the last row of lane 31 must not
+    // couple outward).  Solution returned in Df.  Equivalent to the
+    // reference's Thomas sweep (src/crweno5.cpp:208-219) up to rounding.
+    __device__ __forceinline__ void solve(int h)
+    {
+#pragma unroll
+        for (int r = R - 3; r >= 1; r--) {
+            T tD = Df[r] - Cf[r] * Df[r + 1];
+            T tA = Af[r] - Cf[r] * Af[r + 1];
+            Cf[r] = -Cf[r] * Cf[r + 1];
+            Df[r] = tD; Af[r] = tA;
+        }
+        T A0, C0, D0;
+        if constexpr (R == 2) {
+            A0 = Af[0]; C0 = Cf[0]; D0 = Df[0];
+        } else {
+            T r0 = frcp(T(1) - Cf[0] * Af[1]);
+            A0 = r0 * Af[0];
+            C0 = -r0 * Cf[0] * Cf[1];
+            D0 = r0 * (Df[0] - Cf[0] * Df[1]);
+        }
+        T A1 = Af[R - 1], C1 = Cf[R - 1], D1 = Df[R - 1];
+        T A0n = __shfl_down(A0, 1, 32), C0n = __shfl_down(C0, 1, 32), D0n = __shfl_down(D0, 1, 32);
+        if (h == 31) { A0n = T(0); C0n = T(0); D0n = T(0); }
+        T rb = frcp(T(1) - A1 * C0 - C1 * A0n);
+        T al = -rb * A1 * A0;
+        T ga = -rb * C1 * C0n;
+        T de = rb * (D1 - A1 * D0 - C1 * D0n);
+        pcr32(al, ga, de, h);
+        T X1 = de;
+        T X1m = __shfl_up(X1, 1, 32);
+        if (h == 0) X1m = T(0);
+        T X0 = D0 - A0 * X1m - C0 * X1;
+#pragma unroll
+        for (int r = 1; r <= R - 2; r++) Df[r] = Df[r] - Af[r] * X0 - Cf[r] * X1;
+        Df[0] = X0;
+        Df[R - 1] = X1;
+    }
+};
+
+// WENO5 smoothness indicators and candidate values of a 5-point upwind stencil
+template <typename T>
+__device__ __forceinline__ void w5_parts(const T *f, T &b0, T &b1, T &b2, T &q0, T &q1, T &q2)
+{
+    const T g0 = T(13.0 / 12.0), g1 = T(0.25);
+    b0 = g0 * sq(f[0] - T(2) * f[1] + f[2]) + g1 * sq(f[0] - T(4) * f[1] + T(3) * f[2]);
+    b1 = g0 * sq(f[1] - T(2) * f[2] + f[3]) + g1 * sq(f[1] - f[3]);
+    b2 = g0 * sq(f[2] - T(2) * f[3] + f[4]) + g1 * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
+    q0 = T(1.0 / 3.0) * f[0] + T(-7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
+    q1 = T(-1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
+    q2 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] + T(-1.0 / 6.0) * f[4];
+}
+
+// Reconstruct one sign of a line.  B points at the lane's window start with
+// stride S (+1 for f^+, -1 for f^-): win[t] = B[S*t], row r uses
+// win[r .. r+2*RAD] in upwind coordinates.  ml0/mstep give the rank-local
+// position of row 0 and its step (+1 for f^+, -1 for f^- whose rows run
+// backwards); m0r is the interface index of row 0.
+template <int SCH, int R, typename T>
+__device__ __forceinline__ void line_half(const T *B, int S, int m0r, int mstep, const Zone &z, int h, bool neg,
+                                          T out[R])
+{
+    constexpr int RAD = Radius<SCH>::value;
+    constexpr int W = R + 2 * RAD;
+    T win[W];
+#pragma unroll
+    for (int t = 0; t < W; t++) win[t] = B[S * t];
+    // rank-local position of each row, without integer division in the loop
+    int ml[R];
+    {
+        int m = m0r;
+        int l = m % z.nloc;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            ml[r] = l;
+            l += mstep;
+            if (l >= z.nloc) l -= z.nloc;
+            if (l < 0) l += z.nloc;
+        }
+    }
+    if constexpr (SCH == WENO5JS) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            T b0, b1, b2, q0, q1, q2;
+            w5_parts<T>(win + r, b0, b1, b2, q0, q1, q2);
+            const T eps = T(1.0e-6);
+            T s0 = sq(eps + b0), s1 = sq(eps + b1), s2 = sq(eps + b2);
+            // alpha_k = c_k / s_k; normalised with a single reciprocal
+            T n0 = T(0.1) * (s1 * s2), n1 = T(0.6) * (s0 * s2), n2 = T(0.3) * (s0 * s1);
+            out[r] = (n0 * q0 + n1 * q1 + n2 * q2) * frcp(n0 + n1 + n2);
+        }
+    } else if constexpr (SCH == WENO5Z) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T *f = win + r;
+            T b0, b1, b2, q0, q1, q2;
+            w5_parts<T>(f, b0, b1, b2, q0, q1, q2);
+            // the positive inner sweep's is2 reads f_{i+1} twice (src/weno5z.cpp:80)
+            bool typo = !neg && ml[r] >= z.ts && ml[r] <= z.nloc - z.ts;
+            if (typo) b2 = T(13.0 / 12.0) * sq(f[2] - f[3]) + T(0.25) * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
+            const T eps = T(1.0e-40);
+            T tau = fabs(b2 - b0);
+            T a0 = T(0.1) * (T(1) + sq(tau / (b0 + eps)));
+            T a1 = T(0.6) * (T(1) + sq(tau / (b1 + eps)));
+            T a2 = T(0.3) * (T(1) + sq(tau / (b2 + eps)));
+            out[r] = (a0 * q0 + a1 * q1 + a2 * q2) * frcp(a0 + a1 + a2);
+        }
+    } else if constexpr (SCH == WENO7JS) {
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] = weno7z<T>(win + r, T(1.0e-20));
+    } else if constexpr (SCH == CRWENO5) {
+        Chunk<R, T> ch;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T *f = win + r;
+            T b0, b1, b2, q0, q1, q2;
+            w5_parts<T>(f, b0, b1, b2, q0, q1, q2);
+            const T eps = T(1.0e-6);
+            T s0 = sq(eps + b0), s1 = sq(eps + b1), s2 = sq(eps + b2);
+            T P0 = s1 * s2, P1 = s0 * s2, P2 = s0 * s1;
+            // explicit WENO5-JS closure value (src/crweno5.cpp:87-129) from the same indicators
+            T n0 = T(0.1) * P0, n1 = T(0.6) * P1, n2 = T(0.3) * P2;
+            T wexp = (n0 * q0 + n1 * q1 + n2 * q2) * frcp(n0 + n1 + n2);
+            // compact row (src/crweno5.cpp:133-162) scaled by the weight
+            // denominator, which leaves the tridiagonal solution unchanged
+            T m0 = T(0.2) * P0, m1 = T(0.5) * P1, m2 = T(0.3) * P2;
+            T lo = T(2.0 / 3.0) * m0 + T(1.0 / 3.0) * m1;
+            T di = T(1.0 / 3.0) * m0 + T(2.0 / 3.0) * (m1 + m2);
+            T hi = T(1.0 / 3.0) * m2;
+            T rhs = (m0 * f[1] + (T(5) * (m0 + m1) + m2) * f[2] + (m1 + T(5) * m2) * f[3]) * T(1.0 / 6.0);
+            int m = m0r + mstep * r;
+            bool valid = m < z.nrows;
+            bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
+            T a = (valid && !expl) ? lo : T(0);
+            T b = (valid && !expl) ? di : T(1);
+            T c = (valid && !expl) ? hi : T(0);
+            T d = valid ? (expl ? wexp : rhs) : T(0);
+            ch.row(r, a, b, c, d);
+        }
+        ch.solve(h);
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] = ch.Df[r];
+    } else {  // HCCS
+        Chunk<R, T> ch;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T *f = win + r;   // 7-point window for the closures, 5-point from f+1
+            int m = m0r + mstep * r;
+            bool valid = m < z.nrows;
+            bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
+            T a = T(0), b = T(1), c = T(0), d = T(0);
+            if (valid) {
+                if (expl) {
+                    d = weno7z<T>(f, T(1.0e-40));
                 } else {
-                    T lo, di, hi, rhs;
-                    if constexpr (SCH == CRWENO5)
-                        crweno5_row<T>(P[m - 3], P[m - 2], P[m - 1], P[m], P[m + 1], lo, di, hi, rhs);
-                    else
-                        hccs_row<T>(P[m - 3], P[m - 2], P[m - 1], P[m], P[m + 1], lo, di, hi, rhs);
-                    ap[r] = lo; bp[r] = di; cp[r] = hi; dp[r] = rhs;
-                    if constexpr (SCH == CRWENO5)
-                        crweno5_row<T>(M[m + 2], M[m + 1], M[m], M[m - 1], M[m - 2], lo, di, hi, rhs);
-                    else
-                        hccs_row<T>(M[m + 2], M[m + 1], M[m], M[m - 1], M[m - 2], lo, di, hi, rhs);
-                    // negative sweep: roles of sub- and super-diagonal swap (src/crweno5.cpp:329-332)
-                    am[r] = hi; bm[r] = di; cm[r] = lo; dm[r] = rhs;
+                    hccs_row<T>(f[1], f[2], f[3], f[4], f[5], a, b, c, d);
                 }
             }
+            ch.row(r, a, b, c, d);
         }
-        tri_solve_wave<CPT, T>(ap, bp, cp, dp, lane);
-        tri_solve_wave<CPT, T>(am, bm, cm, dm, lane);
+        ch.solve(h);
 #pragma unroll
-        for (int r = 0; r < CPT; r++) { Fp[r] = dp[r]; Fm[r] = dm[r]; }
-    } else {
-#pragma unroll
-        for (int r = 0; r < CPT; r++) {
-            int m = lane * CPT + r;
-            Fp[r] = T(0); Fm[r] = T(0);
-            if (m < z.nrows) {
-                int ml = m % z.nloc;
-                bool inner = (ml >= z.ts && ml <= z.nloc - z.ts);
-                Fp[r] = explicit_pos<SCH, T>(P, m, inner);
-                Fm[r] = explicit_neg<SCH, T>(M, m);
-            }
-        }
+        for (int r = 0; r < R; r++) out[r] = ch.Df[r];
     }
 }
 
-// First derivative at cell c of a line buffer L (cd6 / cd8), zone by the
-// rank-local position cl (src/centraldifference6th.cpp:42-48,
+// First derivative at cell c of a line buffer L (cd6 / cd8); `inner` selects
+// the inner-loop form of cd8 (src/centraldifference6th.cpp:42-48,
 // src/centraldifference8th.cpp:52-66 inner, :127-131 edges).
 template <typename T>
-__device__ __forceinline__ T central_diff(const T *L, int c, int cl, int nloc, int tsv, int vis, T inv_h)
+__device__ __forceinline__ T central_diff(const T *L, int c, bool inner, int vis, T inv_h)
 {
     if (vis == VIS_CD6) {
         return (T(1.0 / 60.0) * (L[c + 3] - L[c - 3]) + T(-3.0 / 20.0) * (L[c + 2] - L[c - 2])
                 + T(3.0 / 4.0) * (L[c + 1] - L[c - 1])) * inv_h;
     }
-    if (cl >= tsv && cl < nloc - tsv) {
+    if (inner) {
         T an = T(-1.0 / 280.0) * L[c - 4] + T(4.0 / 105.0) * L[c - 3] + T(-1.0 / 5.0) * L[c - 2] + T(4.0 / 5.0) * L[c - 1];
         T ap = T(4.0 / 5.0) * L[c + 1] + T(-1.0 / 5.0) * L[c + 2] + T(4.0 / 105.0) * L[c + 3] + T(-1.0 / 280.0) * L[c + 4];
         return (ap - an) * inv_h;
