@@ -275,23 +275,50 @@ __device__ __forceinline__ void split_var(const double *sp, double qv, int v, in
     }
 }
 
+// Workgroup index with consecutive indices placed on the same XCD: the
+// dispatcher deals workgroups round-robin over the 8 XCDs (each with its own
+// L2), so physical block p runs on XCD p%8; giving that XCD the contiguous
+// logical range makes neighbouring line groups, which share cache lines in
+// the y/z sweeps, meet in one L2.
+__device__ __forceinline__ long xcd_block(unsigned p, unsigned nb)
+{
+    const unsigned q = nb / 8, rem = nb % 8, x = p % 8, i = p / 8;
+    return (long)x * q + (x < rem ? x : rem) + i;
+}
+
+// Line-buffer synchronisation: in the x sweep every wave loads and computes
+// its own line, so a wave-level barrier suffices (LDS operations of one wave
+// complete in order); the y/z load mapping spreads a line over all waves.
+__device__ __forceinline__ void sync_lines(int d)
+{
+    if (d == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
+}
+
 template <int SCH, int RIE, int R, int LPB>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
-    constexpr int NP = 32 * R + 2 * HALO;
+    constexpr int NP = Skew<R>::pitch(32 * R);
+    constexpr int NV = plain_pitch(32 * R);
     constexpr int CPL = R / 2;             // cells per thread in the load mapping
     constexpr int NSP = (RIE == RIE_LF) ? 3 : 5;
     constexpr int RAD = Radius<SCH>::value;
     __shared__ double sP[LPB * NP];
     __shared__ double sM[LPB * NP];
-    __shared__ double sV[LPB * NP];
+    __shared__ double sV[LPB * NV];
     __shared__ double sRed[LPB];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int n = A.n, d = A.dir;
     int ll, g;
     if (d == 0) { ll = wave; g = lane; }
     else { ll = t % LPB; g = t / LPB; }
-    const long L = (long)blockIdx.x * LPB + ll;
+    const long blk = xcd_block(blockIdx.x, gridDim.x);
+    const long L = blk * LPB + ll;
     const bool lv = L < A.nlines;
     long base = 0, stride = 1;
     if (d == 0) { base = L * A.nx; stride = 1; }
@@ -317,17 +344,18 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
         if (cl >= A.tsv && cl < A.nloc - A.tsv) innermask |= 1u << k;
     }
 
-    double *Pl = sP + ll * NP + HALO, *Ml = sM + ll * NP + HALO, *Vl = sV + ll * NP + HALO;
-    double *Pw = sP + wave * NP + HALO, *Mw = sM + wave * NP + HALO;
-    const bool wv = (long)blockIdx.x * LPB + wave < A.nlines;
+    // f^+ / f^- line buffers (skewed, f^- mirrored: Skew<R>::at / mir) and the
+    // plain viscous-flux line
+    double *Pl = sP + ll * NP, *Ml = sM + ll * NP, *Vl = sV + ll * NV + HALO;
+    double *Pw = sP + wave * NP, *Mw = sM + wave * NP;
+    const bool wv = blk * LPB + wave < A.nlines;
     const Zone z{n, A.nloc, A.ts};
     // compute-mapping geometry of this lane (nuc_lines.h)
     const int h = lane & 31;
     const bool neg = lane >= 32;
     const int chunk = neg ? 31 - h : h;
     const int m0 = chunk * R;
-    const double *Bw = neg ? (Mw + m0 + R - 1 + RAD) : (Pw + m0 - RAD - 1);
-    const int S = neg ? -1 : 1;
+    const double *Lw = (neg ? Mw : Pw) + h * (R + 1);
     const int m0r = neg ? m0 + R - 1 : m0;
     const int mstep = neg ? -1 : 1;
     double res = 0.0;
@@ -345,9 +373,9 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 qv[k] = A.q[v][id];
                 double fp, fm;
                 split_var<RIE>(sp[k], qv[k], v, d, fp, fm);
-                Pl[c] = fp; Ml[c] = fm;
-                if (c < HALO) { Pl[c + n] = fp; Ml[c + n] = fm; }
-                if (c >= n - HALO) { Pl[c - n] = fp; Ml[c - n] = fm; }
+                Pl[Skew<R>::at(c)] = fp; Ml[Skew<R>::mir(c)] = fm;
+                if (c < HALO) { Pl[Skew<R>::at(c + n)] = fp; Ml[Skew<R>::mir(c + n)] = fm; }
+                if (c >= n - HALO) { Pl[Skew<R>::at(c - n)] = fp; Ml[Skew<R>::mir(c - n)] = fm; }
                 if (A.vis != VIS_NONE && v > 0) {
                     double x = A.fv[v - 1][id];
                     Vl[c] = x;
@@ -356,22 +384,28 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 }
             }
         }
-        __syncthreads();
+        sync_lines(d);
         double out[R];
-        if (wv) line_half<SCH, R, double>(Bw, S, m0r, mstep, z, h, neg, out);
-        __syncthreads();
+        if (wv) line_half<SCH, R, double>(Lw, m0r, mstep, z, h, neg, out);
+        sync_lines(d);
         if (wv) {
-            double *Ow = neg ? Mw : Pw;
+            // interface m = m0r + mstep*r goes to slot at(m) of the f^+ buffer
+            // or mir(m) of the f^- buffer; relative to the lane base
+            // h*(R+1) these are at(r) and at(r-1), and at(r-1) = at(r)-1
+            // for 0 < r < R
+            double *Ob = (neg ? Mw : Pw) + h * (R + 1);
+            double *Ow = Ob - (neg ? 1 : 0);
+            if (m0r < n) Ob[neg ? Skew<R>::at(-1) : Skew<R>::at(0)] = out[0];
 #pragma unroll
-            for (int r = 0; r < R; r++) {
+            for (int r = 1; r < R; r++) {
                 int m = m0r + mstep * r;
-                if (m < n) {
-                    Ow[m] = out[r];
-                    if (m == 0) Ow[n] = out[r];
-                }
+                if (m < n) Ow[Skew<R>::at(r)] = out[r];
             }
+            // periodic copy of interface 0 to interface n
+            if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
+            if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
         }
-        __syncthreads();
+        sync_lines(d);
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
@@ -379,7 +413,8 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 long id = base + (long)c * stride;
                 double r = (d > 0) ? A.rhs[v][id] : 0.0;
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
-                r += ((Pl[c + 1] + Ml[c + 1]) - (Pl[c] + Ml[c])) * A.inv_h;
+                r += ((Pl[Skew<R>::at(c + 1)] + Ml[Skew<R>::mir(c + 1)]) - (Pl[Skew<R>::at(c)] + Ml[Skew<R>::mir(c)]))
+                     * A.inv_h;
                 if (A.vis != VIS_NONE && v > 0)
                     r -= central_diff<double>(Vl, c, (innermask >> k) & 1u, A.vis, A.inv_h);
                 if (!A.epilogue) {
@@ -399,7 +434,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 }
             }
         }
-        __syncthreads();
+        sync_lines(d);
     }
     if (A.epilogue && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
@@ -410,7 +445,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
         if (t == 0) {
             double acc = 0.0;
             for (int w = 0; w < LPB; w++) acc += sRed[w];
-            A.res_part[blockIdx.x] = acc;
+            A.res_part[blk] = acc;
         }
     }
 }
@@ -528,31 +563,32 @@ struct OpRecArgs {
 template <int SCH, int R>
 __global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
 {
-    constexpr int NP = 32 * R + 2 * HALO;
+    constexpr int NP = Skew<R>::pitch(32 * R);
     constexpr int RAD = Radius<SCH>::value;
     __shared__ double sL[NP];
+    __shared__ double sLm[NP];
     const int lane = threadIdx.x;
     const long line = blockIdx.x;
     const int n = A.n1;
-    double *L = sL + HALO;
+    double *L = sL;
     for (int c = lane - HALO; c < 32 * R + HALO; c += 64) {
         double x = 0.0;
         if (c >= 0 && c < n) x = A.in[line * n + c];
         else if (c < 0 && A.gL && -c <= A.bf) x = A.gL[line * A.bf + (A.bf + c)];
         else if (c >= n && A.gR && c - n < A.bf && c < n + HALO) x = A.gR[line * A.bf + (c - n)];
-        L[c] = x;
+        L[Skew<R>::at(c)] = x;
+        sLm[Skew<R>::mir(c)] = x;
     }
     __syncthreads();
     const int h = lane & 31;
     const bool neg = lane >= 32;
     const int chunk = neg ? 31 - h : h;
     const int m0 = chunk * R;
-    const double *B = neg ? (L + m0 + R - 1 + RAD) : (L + m0 - RAD - 1);
     const int m0r = neg ? m0 + R - 1 : m0;
     const int mstep = neg ? -1 : 1;
     const Zone z{n + 1, n, A.ts};
     double out[R];
-    line_half<SCH, R, double>(B, neg ? -1 : 1, m0r, mstep, z, h, neg, out);
+    line_half<SCH, R, double>((neg ? sLm : L) + h * (R + 1), m0r, mstep, z, h, neg, out);
     if (neg != (A.upwind < 0)) return;
     for (int r = 0; r < R; r++) {
         int m = m0r + mstep * r;
@@ -616,14 +652,19 @@ int rows_for(int nrows)
     return -1;
 }
 
-constexpr int lpb_for(int R) { return R <= 8 ? 8 : 4; }
+// lines per workgroup: 4 (one wave per line).  With the sweep's register
+// budget two such workgroups share a CU, so one workgroup's barrier waits
+// overlap the other's work; the y/z load mapping still reads 4 consecutive
+// x cells per line cell, and xcd_block keeps the neighbouring groups that
+// share the cache line on one XCD.
+constexpr int SWEEP_LPB = 4;
+int sweep_lpb(int) { return SWEEP_LPB; }
 
 template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
-    constexpr int LPB = lpb_for(R);
-    long nb = (A.nlines + LPB - 1) / LPB;
-    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, LPB>), dim3((unsigned)nb), dim3(64 * LPB), 0, s, A);
+    long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
+    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
 }
 
 template <int SCH, int RIE>
@@ -655,8 +696,6 @@ void launch_sweep(int scheme, int R, const SweepArgs &A, hipStream_t s)
     default: launch_sweep_s<HCCS>(R, A, s); break;
     }
 }
-
-int sweep_lpb(int R) { return lpb_for(R); }
 
 template <int SCH, int R>
 void launch_op_r(const OpRecArgs &A, hipStream_t s)

@@ -40,6 +40,30 @@ __device__ __forceinline__ double frcp(double x)
 
 template <int SCH> struct Radius { static constexpr int value = (SCH == WENO7JS || SCH == HCCS) ? 3 : 2; };
 
+// Skewed LDS line layout: position y (y >= -B) lives at slot (y+B) + (y+B)/R,
+// one pad slot after every R positions, B = HALO+1 rounded up to a multiple of
+// R.  The f^+ line stores cell x at y = x; the f^- line is stored mirrored,
+// y = 32R-2-x, so that both half waves read their windows upwards from
+// y = hR-RAD-1: the lane's window then starts at slot h*(R+1) + at(-RAD-1)
+// and every window element is a compile-time offset from one base.  Lanes R
+// positions apart sit R+1 slots apart (odd), so chunk reads and the
+// consecutive-cell writes of the load phase hit distinct LDS banks.
+// pitch(): line stride in slots, = 2 (mod 16) so that the LPB lines written
+// together by the y/z load mapping spread over the banks too.
+template <int R> struct Skew {
+    static constexpr int B = ((HALO + 1 + R - 1) / R) * R;
+    __host__ __device__ static constexpr int at(int y) { return (y + B) + (y + B) / R; }
+    __device__ __forceinline__ static int mir(int x) { return at(32 * R - 2 - x); }
+    static constexpr int span(int ncap) { return at(ncap + HALO - 1) + 1; }
+    static constexpr int pitch(int ncap) { return span(ncap) + ((2 - span(ncap) % 16) + 16) % 16; }
+};
+
+// pitch of an unskewed line of ncap cells with HALO ghosts on each side, = 2 (mod 16)
+__host__ __device__ constexpr int plain_pitch(int ncap)
+{
+    return (ncap + 2 * HALO) + ((2 - (ncap + 2 * HALO) % 16) + 16) % 16;
+}
+
 // normalised parallel cyclic reduction over the 32 lanes of a half wave:
 // row k: x_k + al*x_{k-1} + ga*x_{k+1} = de  ->  x_k = de on return
 template <typename T>
@@ -129,20 +153,21 @@ __device__ __forceinline__ void w5_parts(const T *f, T &b0, T &b1, T &b2, T &q0,
     q2 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] + T(-1.0 / 6.0) * f[4];
 }
 
-// Reconstruct one sign of a line.  B points at the lane's window start with
-// stride S (+1 for f^+, -1 for f^-): win[t] = B[S*t], row r uses
+// Reconstruct one sign of a line.  Lw points at the lane's window in the
+// skewed line buffer (slot h*(R+1); the f^- buffer is mirrored, see Skew):
+// win[t] = Lw[at(t-RAD-1)] in upwind order, row r uses
 // win[r .. r+2*RAD] in upwind coordinates.  ml0/mstep give the rank-local
 // position of row 0 and its step (+1 for f^+, -1 for f^- whose rows run
 // backwards); m0r is the interface index of row 0.
 template <int SCH, int R, typename T>
-__device__ __forceinline__ void line_half(const T *B, int S, int m0r, int mstep, const Zone &z, int h, bool neg,
+__device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const Zone &z, int h, bool neg,
                                           T out[R])
 {
     constexpr int RAD = Radius<SCH>::value;
     constexpr int W = R + 2 * RAD;
     T win[W];
 #pragma unroll
-    for (int t = 0; t < W; t++) win[t] = B[S * t];
+    for (int t = 0; t < W; t++) win[t] = Lw[Skew<R>::at(t - RAD - 1)];
     // rank-local position of each row, without integer division in the loop
     int ml[R];
     {
