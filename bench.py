@@ -20,6 +20,19 @@ stage: 35.6 B per cell per sweep launch.
 
 cpu_baseline: the oracle (a C restatement of the reference algorithm) timed on
 this host's cores on one RHS evaluation of the same 256^3 workload.
+
+mixed: the same workload with NUC_MIXED (FP32 split fluxes, reconstruction and
+viscous stencils; FP64 weights, tridiagonal solves, RK3 and reductions), timed
+the same way, with the relative error of every global statistic (kinetic
+energy, dissipation, enstrophy, u_rms, Re_lambda, ...) against the FP64 run
+from the same initial state after the same steps.
+
+serving: a stream of independent boxes (Taylor-Green or decaying turbulence,
+N log-uniform per side, Mach U(0.1,0.5), Re U(400,3200), scheme uniform over
+WENO5-JS/WENO5-Z/CRWENO5, LF, cd6, CFL 0.5) through the serving loop:
+throughput with every job submitted at once against the same jobs run one at a
+time on plain contexts, and p50/p99 latency (arrival -> result) under Poisson
+arrivals at 20 jobs/s.
 """
 import argparse
 import json
@@ -62,6 +75,91 @@ def cpu_baseline(q, n, threads):
                       f"{t1 - t0:.1f} s"}
 
 
+def stage_stats(G, cfg, q, steps, warmup):
+    """(seconds for `steps` timed steps, stats_ex after warmup+steps, profile)"""
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        if warmup:
+            ctx.step_async(warmup)
+            ctx.sync()
+        ctx.profile(True)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ctx.step_async(steps)
+        dt, res, tsim = ctx.sync()
+        t1 = time.perf_counter()
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        st = ctx.stats_ex()
+        q_end = ctx.download()
+    return t1 - t0, st, prof, q_end, dt
+
+
+STAT_NAMES = ["ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "om_x", "om_y", "om_z", "enstrophy", "Q",
+              "helicity", "KE", "u_rms", "M_t", "theta_rms", "mu_mean", "Re_lambda", "lambda", "rho_mean",
+              "tau_t", "skewness", "eps_s", "eps_d"]
+
+
+def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(njobs):
+        n = int(round(np.exp(rng.uniform(np.log(nmin), np.log(nmax)))))
+        n += n % 2
+        mach = rng.uniform(0.1, 0.5)
+        re = rng.uniform(400.0, 3200.0)
+        scheme = [G.WENO5JS, G.WENO5Z, G.CRWENO5][rng.integers(3)]
+        cfg = G.make_config(n=(n, n, n), scheme=scheme, vis=G.CD6, riemann=G.LF, model=G.NAVIERSTOKES3D,
+                            mach=mach, reynolds=re, cfl=0.5)
+        if rng.random() < 0.5:
+            q = G.init_tgv(cfg)
+        else:
+            q = G.init_hit(cfg, k0=4.0, u_rms=min(1.0, 0.3 / (np.sqrt(3.0) * mach)), seed=int(rng.integers(1 << 30)))
+        jobs.append((cfg, q, n))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
+    return jobs, arrivals
+
+
+def run_serving(G, jobs, arrivals=None):
+    """All jobs through one server; arrivals None = all at once. Returns (wall s, latencies s)."""
+    import numpy as np
+    with G.Server(hbm_budget=96 << 30, max_active=16) as srv:
+        ids = []
+        t0 = time.perf_counter()
+        for k, (cfg, q, _) in enumerate(jobs):
+            if arrivals is not None:
+                wait = arrivals[k] - (time.perf_counter() - t0)
+                if wait > 0:
+                    time.sleep(wait)
+            ids.append(srv.submit(cfg, q, SERVE_STEPS))
+        lat = []
+        for jid in ids:
+            while True:
+                r = srv.result(jid, want_state=False)
+                if r is not None:
+                    break
+                time.sleep(0.0005)
+            if r[1].state != G.JOB_DONE:
+                raise RuntimeError(f"serving job {jid} failed: state {r[1].state} error {r[1].error}")
+            lat.append(r[1].done_s - r[1].submit_s)
+        wall = time.perf_counter() - t0
+    return wall, np.array(lat)
+
+
+def run_sequential(G, jobs):
+    t0 = time.perf_counter()
+    for cfg, q, _ in jobs:
+        with G.Context(cfg) as ctx:
+            ctx.upload(q)
+            ctx.step(SERVE_STEPS)
+            ctx.download()
+    return time.perf_counter() - t0
+
+
+SERVE_STEPS = 100
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -71,6 +169,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-slots", action="store_true", help="print per-kernel times to stderr")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box per GPU and measures one MI355X; "
@@ -84,24 +185,11 @@ def main():
     cfg = workload_config(G, n)
     q = G.init_hit(cfg, k0=4.0, u_rms=1.0, seed=20261202)
     cells = n ** 3
-    with G.Context(cfg) as ctx:
-        ctx.upload(q)
-        if args.warmup:
-            ctx.step_async(args.warmup)
-            ctx.sync()
-        ctx.profile(True)
-        ctx.sync()
-        t0 = time.perf_counter()
-        ctx.step_async(args.steps)
-        dt, res, tsim = ctx.sync()
-        t1 = time.perf_counter()
-        prof = ctx.profile_read()
-        ctx.profile(False)
-        q_end = ctx.download()
+    print(f"bench: FP64 {n}^3, {args.steps} steps", file=sys.stderr, flush=True)
+    secs, st64, prof, q_end, dt = stage_stats(G, cfg, q, args.steps, args.warmup)
     if not np.all(np.isfinite(q_end)):
         print("bench.py: state became non-finite", file=sys.stderr)
         sys.exit(3)
-    secs = t1 - t0
     ms_per_step = secs / args.steps * 1e3
     value = cells * 3 * args.steps / secs / 1e6
 
@@ -120,6 +208,39 @@ def main():
     if args.profile_slots:
         for k, (t, c) in prof.items():
             print(f"  {k:10s} total {t:9.3f} ms  launches {c:5d}  avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
+
+    mixed = None
+    if not args.no_mixed:
+        print("bench: mixed", file=sys.stderr, flush=True)
+        msecs, stmx, mprof, _, _ = stage_stats(G, workload_config(G, n, G.MIXED), q, args.steps, args.warmup)
+        rel = np.abs(stmx - st64) / np.maximum(np.abs(st64), 1e-300)
+        eps64, epsmx = st64[25] + st64[26], stmx[25] + stmx[26]
+        mixed = {"value": round(cells * 3 * args.steps / msecs / 1e6, 3),
+                 "ms_per_step": round(msecs / args.steps * 1e3, 4),
+                 "speedup_vs_fp64": round(secs / msecs, 4),
+                 "rel_err_vs_fp64": {"KE": float(rel[15]), "dissipation": float(abs(epsmx - eps64) / eps64),
+                                     **{STAT_NAMES[i]: float(rel[i]) for i in (12, 16, 17, 18, 19, 20, 21, 24, 25, 26)}},
+                 "after_steps": args.warmup + args.steps}
+        if args.profile_slots:
+            for k, (t, c) in mprof.items():
+                print(f"  mixed {k:10s} avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
+
+    serving = None
+    if not args.no_serving:
+        print("bench: serving", file=sys.stderr, flush=True)
+        jobs, arrivals = serving_stream(G, args.serve_jobs, 32, 96, SERVE_STEPS, 20.0, 20261202 + 500)
+        run_serving(G, jobs[:4])                      # warm the kernels and the allocator
+        wall_sat, _ = run_serving(G, jobs)
+        wall_seq = run_sequential(G, jobs)
+        wall_poi, lat = run_serving(G, jobs, arrivals)
+        serving = {"jobs": len(jobs), "steps_per_job": SERVE_STEPS, "grid_n": [32, 96],
+                   "jobs_per_s": round(len(jobs) / wall_sat, 3),
+                   "sequential_jobs_per_s": round(len(jobs) / wall_seq, 3),
+                   "speedup_vs_sequential": round(wall_seq / wall_sat, 3),
+                   "poisson_rate_per_s": 20.0,
+                   "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
+                   "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
+                   "poisson_wall_s": round(wall_poi, 3)}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -145,6 +266,8 @@ def main():
                    "stage_ms": round(stage_ms, 4), "dt_last": dt},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "mixed": mixed,
+        "serving": serving,
     }
     print(json.dumps(out))
 

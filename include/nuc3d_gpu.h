@@ -26,7 +26,8 @@ enum nuc_status {
     NUC_ENOMEM = 2,       /* device allocation failed */
     NUC_EHIP = 3,         /* a HIP runtime call failed */
     NUC_ENONFINITE = 4,   /* state became NaN/Inf */
-    NUC_ENODEV = 5        /* no usable gfx950 device */
+    NUC_ENODEV = 5,       /* no usable gfx950 device */
+    NUC_EBUSY = 6         /* work still in flight (nuc_query, serving results) */
 };
 
 /* inviscid schemes, names as accepted in inp/Method.in (src/fieldOperator.cpp:128-220) */
@@ -93,12 +94,25 @@ int nuc_rhs(nuc_ctx *ctx, double *rhs_host, double *dt_local);
 int nuc_step(nuc_ctx *ctx, int nsteps, nuc_step_info *info);
 /* enqueue nsteps steps without waiting (timing); nuc_sync waits */
 int nuc_step_async(nuc_ctx *ctx, int nsteps);
+/* Replay steps from one captured hipGraph of a whole RK3 step (kernel nodes
+ * only) instead of launching each kernel: removes the per-launch host cost that
+ * dominates small boxes.  No counterpart in the reference (its step loop is
+ * src/block.cpp:191-215).  Profiling (nuc_profile) falls back to launches. */
+int nuc_set_graph(nuc_ctx *ctx, int enable);
+/* NUC_OK when all enqueued work of ctx has finished, NUC_EBUSY otherwise; never blocks. */
+int nuc_query(nuc_ctx *ctx);
 int nuc_sync(nuc_ctx *ctx, nuc_step_info *info);
 /* Global statistics of postproc::solveQ (src/postproc.cpp:736-929), FP64:
  * out[0..15] means of ux,uy,uz,vx,vy,vz,wx,wy,wz, omega_x,y,z, enstrophy, Q,
  * helicity, kinetic energy; out[16..24] u_rms, M_t, sqrt(<(du_i/dx_i)^2>/3),
  * mu_mean, Re_lambda, lambda, rho_mean, tau_t, skewness. */
 int nuc_stats(nuc_ctx *ctx, double *out25);
+/* The same 25 values followed by the dissipation rates the mixed path reports
+ * its error on: out[25] = <mu |omega|^2>/Re (solenoidal), out[26] =
+ * 4/3 <mu (div u)^2>/Re (dilatational), mu the Sutherland law of getMiu
+ * (src/physicsModel.cpp:193-225).  Writes min(nout, NUC_NSTATS_EX) values. */
+#define NUC_NSTATS_EX 27
+int nuc_stats_ex(nuc_ctx *ctx, double *out, int nout);
 
 /* ---- operator hooks (reference plug-in API, one Field per call) -----------
  * These mirror interoplation::interpolationInner/BoundaryL/BoundaryR

@@ -18,8 +18,10 @@
 #include "nuc3d_gpu.h"
 #include "nuc_device.h"
 #include "nuc_lines.h"
+#include "nuc_force.h"
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -31,7 +33,7 @@ using namespace nucdev;
 namespace {
 
 constexpr int MAXRANK = 64;
-constexpr int NUC_PROF_SLOTS = 7;
+constexpr int NUC_PROF_SLOTS = 8;
 
 // ---------------------------------------------------------------------------
 // kernel arguments
@@ -42,9 +44,12 @@ struct SweepArgs {
     long nlines;
     double inv_h, gamma;
     const double *q[5];
-    const double *fv[4];
+    const void *fv[4];      // viscous flux components, T of the sweep
     double *rhs[5];
     int stage, epilogue;
+    int accum;              // x sweep adds to rhs (forcing initialised it)
+    int mixed;              // FP32 stencils (host-side dispatch only)
+    const double *fmean;    // z sweep: <f.u> added back to the energy RHS
     const double *qn[5];
     double *qout[5];
     const double *dtp;
@@ -63,7 +68,7 @@ struct PrimArgs {
 
 struct GradArgs {
     const double *prim[4];
-    double *out[9];
+    void *out[9];           // 9 viscous flux components, T of the kernel
     int nx, ny, nz;
     int nloc[3], tsv[3], vis[3];
     double inv_h[3];
@@ -157,6 +162,8 @@ __global__ void k_dt(unsigned long long *lam, int P, double hx, double hy, doubl
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// T: stencil and flux type (double, or float for the mixed path)
+template <typename T>
 __global__ __launch_bounds__(256) void k_grad(GradArgs A)
 {
     const int i = blockIdx.x * 64 + threadIdx.x;
@@ -165,7 +172,7 @@ __global__ __launch_bounds__(256) void k_grad(GradArgs A)
     if (i >= A.nx || j >= A.ny) return;
     const long sx = 1, sy = A.nx, sz = (long)A.nx * A.ny;
     const long id = i + sy * j + sz * k;
-    double g[4][3];
+    T g[4][3];
     const int pos[3] = {i, j, k};
     const int ext[3] = {A.nx, A.ny, A.nz};
     const long str[3] = {sx, sy, sz};
@@ -180,40 +187,41 @@ __global__ __launch_bounds__(256) void k_grad(GradArgs A)
         int cl = pos[d] % A.nloc[d];
 #pragma unroll
         for (int f = 0; f < 4; f++) {
-            double L[9];
+            T L[9];
 #pragma unroll
-            for (int o = 0; o < 9; o++) L[o] = A.prim[f][offs[o]];
-            g[f][d] = central_diff<double>(L + 4, 0, cl >= A.tsv[d] && cl < A.nloc[d] - A.tsv[d], A.vis[d], A.inv_h[d]);
+            for (int o = 0; o < 9; o++) L[o] = T(A.prim[f][offs[o]]);
+            g[f][d] = central_diff<T>(L + 4, 0, cl >= A.tsv[d] && cl < A.nloc[d] - A.tsv[d], A.vis[d], T(A.inv_h[d]));
         }
     }
-    double ux = g[0][0], uy = g[0][1], uz = g[0][2];
-    double vx = g[1][0], vy = g[1][1], vz = g[1][2];
-    double wx = g[2][0], wy = g[2][1], wz = g[2][2];
-    double tx = g[3][0], ty = g[3][1], tz = g[3][2];
-    double T = A.prim[3][id];
-    double miu0;
-    double S = A.tref / A.tinf;
-    if (A.law == 0) miu0 = (1.0 + S) / (T + S) * (T * sqrt(T)) / A.re;   // src/physicsModel.cpp:220
-    else miu0 = 1.0 / A.re;
-    double coeff0 = miu0 / ((A.gamma - 1.0) * A.mach * A.mach * A.pr);
-    double div = ux + vy + wz;
+    T ux = g[0][0], uy = g[0][1], uz = g[0][2];
+    T vx = g[1][0], vy = g[1][1], vz = g[1][2];
+    T wx = g[2][0], wy = g[2][1], wz = g[2][2];
+    T tx = g[3][0], ty = g[3][1], tz = g[3][2];
+    T Tt = T(A.prim[3][id]);
+    T miu0;
+    T S = T(A.tref / A.tinf);
+    if (A.law == 0) miu0 = (T(1) + S) / (Tt + S) * (Tt * sqrt(Tt)) / T(A.re);   // src/physicsModel.cpp:220
+    else miu0 = T(1) / T(A.re);
+    T coeff0 = miu0 / T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
+    T div = ux + vy + wz;
     // src/NaiverStokes3d.cpp:265-283 (off-diagonals carry -2/3 div u as coded)
-    double txx = miu0 * (2.0 * ux - 2.0 / 3.0 * div);
-    double txy = miu0 * (uy + vx - 2.0 / 3.0 * div);
-    double txz = miu0 * (uz + wx - 2.0 / 3.0 * div);
-    double tyy = miu0 * (2.0 * vy - 2.0 / 3.0 * div);
-    double tyz = miu0 * (vz + wy - 2.0 / 3.0 * div);
-    double tzz = miu0 * (2.0 * wz - 2.0 / 3.0 * div);
-    double u = A.prim[0][id], v = A.prim[1][id], w = A.prim[2][id];
-    A.out[0][id] = txx;
-    A.out[1][id] = tyy;
-    A.out[2][id] = tzz;
-    A.out[3][id] = txy;
-    A.out[4][id] = txz;
-    A.out[5][id] = tyz;
-    A.out[6][id] = u * txx + v * txy + w * txz + coeff0 * tx;
-    A.out[7][id] = u * txy + v * tyy + w * tyz + coeff0 * ty;
-    A.out[8][id] = u * txz + v * tyz + w * tzz + coeff0 * tz;
+    T txx = miu0 * (T(2) * ux - T(2.0 / 3.0) * div);
+    T txy = miu0 * (uy + vx - T(2.0 / 3.0) * div);
+    T txz = miu0 * (uz + wx - T(2.0 / 3.0) * div);
+    T tyy = miu0 * (T(2) * vy - T(2.0 / 3.0) * div);
+    T tyz = miu0 * (vz + wy - T(2.0 / 3.0) * div);
+    T tzz = miu0 * (T(2) * wz - T(2.0 / 3.0) * div);
+    T u = T(A.prim[0][id]), v = T(A.prim[1][id]), w = T(A.prim[2][id]);
+    T *const *out = reinterpret_cast<T *const *>(A.out);
+    out[0][id] = txx;
+    out[1][id] = tyy;
+    out[2][id] = tzz;
+    out[3][id] = txy;
+    out[4][id] = txz;
+    out[5][id] = tyz;
+    out[6][id] = u * txx + v * txy + w * txz + coeff0 * tx;
+    out[7][id] = u * txy + v * tyy + w * tyz + coeff0 * ty;
+    out[8][id] = u * txz + v * tyz + w * tzz + coeff0 * tz;
 }
 
 // ---------------------------------------------------------------------------
@@ -225,31 +233,31 @@ __global__ __launch_bounds__(256) void k_grad(GradArgs A)
 // line t%LPB, so each load instruction reads LPB consecutive doubles per cell.
 // Compute mapping: wave w owns line w (both signs, see nuc_lines.h).
 // ---------------------------------------------------------------------------
-template <int RIE>
-__device__ __forceinline__ void split_params(const double q[5], int d, double gamma, double sp[5])
+template <int RIE, typename T>
+__device__ __forceinline__ void split_params(const T q[5], int d, T gamma, T sp[5])
 {
-    const double gm1 = gamma - 1.0;
-    double rho = q[0], ir = 1.0 / rho;
-    double u = q[1] * ir, vv = q[2] * ir, w = q[3] * ir;
-    double p = fabs((q[4] - 0.5 * rho * (u * u + vv * vv + w * w)) * gm1);
-    double cs = sqrt(gamma * p * ir);
-    double ud = (d == 0) ? u : (d == 1 ? vv : w);
+    const T gm1 = gamma - T(1);
+    T rho = q[0], ir = T(1) / rho;
+    T u = q[1] * ir, vv = q[2] * ir, w = q[3] * ir;
+    T p = fabs((q[4] - T(0.5) * rho * (u * u + vv * vv + w * w)) * gm1);
+    T cs = sqrt(gamma * p * ir);
+    T ud = (d == 0) ? u : (d == 1 ? vv : w);
     if constexpr (RIE == RIE_LF) {
-        sp[0] = ud; sp[1] = fabs(ud) + cs; sp[2] = p; sp[3] = 0.0; sp[4] = 0.0;
+        sp[0] = ud; sp[1] = fabs(ud) + cs; sp[2] = p; sp[3] = T(0); sp[4] = T(0);
     } else {
         // AUSM Mach / pressure splittings, src/physicsModel.cpp:1181-1228
-        double mach = ud / cs;
-        double mp, mn, pp, pn;
-        if (fabs(mach) < 1.0) {
-            mp = 0.25 * sq(mach + 1.0) + 0.125 * sq(mach * mach - 1.0);
-            mn = -0.25 * sq(mach - 1.0) - 0.125 * sq(mach * mach - 1.0);
-            pp = p * (mach + 1.0) / 2.0;
-            pn = -p * (mach - 1.0) / 2.0;
+        T mach = ud / cs;
+        T mp, mn, pp, pn;
+        if (fabs(mach) < T(1)) {
+            mp = T(0.25) * sq(mach + T(1)) + T(0.125) * sq(mach * mach - T(1));
+            mn = T(-0.25) * sq(mach - T(1)) - T(0.125) * sq(mach * mach - T(1));
+            pp = p * (mach + T(1)) / T(2);
+            pn = -p * (mach - T(1)) / T(2);
         } else {
-            mp = 0.5 * (mach + fabs(mach));
-            mn = 0.5 * (mach - fabs(mach));
-            pp = 0.5 * p * (mach + fabs(mach)) / mach;
-            pn = 0.5 * p * (mach - fabs(mach)) / mach;
+            mp = T(0.5) * (mach + fabs(mach));
+            mn = T(0.5) * (mach - fabs(mach));
+            pp = T(0.5) * p * (mach + fabs(mach)) / mach;
+            pn = T(0.5) * p * (mach - fabs(mach)) / mach;
         }
         sp[0] = mp * cs; sp[1] = mn * cs; sp[2] = pp; sp[3] = pn; sp[4] = p;
     }
@@ -257,16 +265,16 @@ __device__ __forceinline__ void split_params(const double q[5], int d, double ga
 
 // split fluxes of variable v at one cell (LF src/physicsModel.cpp:540-565,
 // AUSM :790-820, per direction d)
-template <int RIE>
-__device__ __forceinline__ void split_var(const double *sp, double qv, int v, int d, double &fp, double &fm)
+template <int RIE, typename T>
+__device__ __forceinline__ void split_var(const T *sp, T qv, int v, int d, T &fp, T &fm)
 {
     if constexpr (RIE == RIE_LF) {
-        double ud = sp[0], lam = sp[1], p = sp[2];
-        double F = ud * qv;
+        T ud = sp[0], lam = sp[1], p = sp[2];
+        T F = ud * qv;
         if (v == 1 + d) F = ud * qv + p;
         if (v == 4) F = ud * (qv + p);
-        fp = 0.5 * (F + lam * qv);
-        fm = 0.5 * (F - lam * qv);
+        fp = T(0.5) * (F + lam * qv);
+        fm = T(0.5) * (F - lam * qv);
     } else {
         fp = sp[0] * qv;
         fm = sp[1] * qv;
@@ -300,7 +308,10 @@ __device__ __forceinline__ void sync_lines(int d)
     }
 }
 
-template <int SCH, int RIE, int R, int LPB>
+// T: type of the split fluxes, line buffers and stencils (double, or float
+// for the mixed path); weights, tridiagonal solves, interface values, the
+// flux difference and the RHS/RK3 accumulation are FP64 in both.
+template <int SCH, int RIE, int R, int LPB, typename T>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
     constexpr int NP = Skew<R>::pitch(32 * R);
@@ -308,9 +319,11 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     constexpr int CPL = R / 2;             // cells per thread in the load mapping
     constexpr int NSP = (RIE == RIE_LF) ? 3 : 5;
     constexpr int RAD = Radius<SCH>::value;
+    // sP/sM hold the split fluxes (as T) and then the interface values (as
+    // double) of the same lines; the phases are separated by barriers
     __shared__ double sP[LPB * NP];
     __shared__ double sM[LPB * NP];
-    __shared__ double sV[LPB * NV];
+    __shared__ T sV[LPB * NV];
     __shared__ double sRed[LPB];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
     const int n = A.n, d = A.dir;
@@ -326,18 +339,18 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
 
     // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344)
-    double sp[CPL][NSP];
+    T sp[CPL][NSP];
     unsigned innermask = 0;   // cd8 inner-form cells (rank-local position)
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
         int c = g + 64 * k;
         bool ok = lv && c < n;
         long id = base + (long)c * stride;
-        double q[5];
+        T q[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) q[v] = ok ? A.q[v][id] : (v == 0 ? 1.0 : 0.0);
-        double tmp[5];
-        split_params<RIE>(q, d, A.gamma, tmp);
+        for (int v = 0; v < 5; v++) q[v] = ok ? T(A.q[v][id]) : (v == 0 ? T(1) : T(0));
+        T tmp[5];
+        split_params<RIE, T>(q, d, T(A.gamma), tmp);
 #pragma unroll
         for (int j = 0; j < NSP; j++) sp[k][j] = tmp[j];
         int cl = c % A.nloc;
@@ -346,8 +359,10 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 
     // f^+ / f^- line buffers (skewed, f^- mirrored: Skew<R>::at / mir) and the
     // plain viscous-flux line
-    double *Pl = sP + ll * NP, *Ml = sM + ll * NP, *Vl = sV + ll * NV + HALO;
-    double *Pw = sP + wave * NP, *Mw = sM + wave * NP;
+    T *Pl = reinterpret_cast<T *>(sP) + ll * NP, *Ml = reinterpret_cast<T *>(sM) + ll * NP;
+    T *Vl = sV + ll * NV + HALO;
+    double *Pw = sP + wave * NP, *Mw = sM + wave * NP;                 // interface values
+    double *Pd = sP + ll * NP, *Md = sM + ll * NP;
     const bool wv = blk * LPB + wave < A.nlines;
     const Zone z{n, A.nloc, A.ts};
     // compute-mapping geometry of this lane (nuc_lines.h)
@@ -355,7 +370,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     const bool neg = lane >= 32;
     const int chunk = neg ? 31 - h : h;
     const int m0 = chunk * R;
-    const double *Lw = (neg ? Mw : Pw) + h * (R + 1);
+    const T *Lw = reinterpret_cast<const T *>(neg ? sM : sP) + wave * NP + h * (R + 1);
     const int m0r = neg ? m0 + R - 1 : m0;
     const int mstep = neg ? -1 : 1;
     double res = 0.0;
@@ -371,13 +386,13 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
             if (lv && c < n) {
                 long id = base + (long)c * stride;
                 qv[k] = A.q[v][id];
-                double fp, fm;
-                split_var<RIE>(sp[k], qv[k], v, d, fp, fm);
+                T fp, fm;
+                split_var<RIE, T>(sp[k], T(qv[k]), v, d, fp, fm);
                 Pl[Skew<R>::at(c)] = fp; Ml[Skew<R>::mir(c)] = fm;
                 if (c < HALO) { Pl[Skew<R>::at(c + n)] = fp; Ml[Skew<R>::mir(c + n)] = fm; }
                 if (c >= n - HALO) { Pl[Skew<R>::at(c - n)] = fp; Ml[Skew<R>::mir(c - n)] = fm; }
                 if (A.vis != VIS_NONE && v > 0) {
-                    double x = A.fv[v - 1][id];
+                    T x = static_cast<const T *>(A.fv[v - 1])[id];
                     Vl[c] = x;
                     if (c < HALO) Vl[c + n] = x;
                     if (c >= n - HALO) Vl[c - n] = x;
@@ -386,7 +401,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
         }
         sync_lines(d);
         double out[R];
-        if (wv) line_half<SCH, R, double>(Lw, m0r, mstep, z, h, neg, out);
+        if (wv) line_half<SCH, R, T, double>(Lw, m0r, mstep, z, h, neg, out);
         sync_lines(d);
         if (wv) {
             // interface m = m0r + mstep*r goes to slot at(m) of the f^+ buffer
@@ -411,12 +426,13 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
             int c = g + 64 * k;
             if (lv && c < n) {
                 long id = base + (long)c * stride;
-                double r = (d > 0) ? A.rhs[v][id] : 0.0;
+                double r = (d > 0 || A.accum) ? A.rhs[v][id] : 0.0;
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
-                r += ((Pl[Skew<R>::at(c + 1)] + Ml[Skew<R>::mir(c + 1)]) - (Pl[Skew<R>::at(c)] + Ml[Skew<R>::mir(c)]))
+                r += ((Pd[Skew<R>::at(c + 1)] + Md[Skew<R>::mir(c + 1)]) - (Pd[Skew<R>::at(c)] + Md[Skew<R>::mir(c)]))
                      * A.inv_h;
                 if (A.vis != VIS_NONE && v > 0)
-                    r -= central_diff<double>(Vl, c, (innermask >> k) & 1u, A.vis, A.inv_h);
+                    r -= double(central_diff<T>(Vl, c, (innermask >> k) & 1u, A.vis, T(A.inv_h)));
+                if (A.fmean && v == 4) r += *A.fmean;
                 if (!A.epilogue) {
                     A.rhs[v][id] = r;
                 } else {
@@ -470,7 +486,7 @@ __global__ void k_res(const double *part, int nb, double inv_count, double *scal
 // ---------------------------------------------------------------------------
 // statistics (postproc::solveQ set), per-block partial sums then a fixed-order reduce
 // ---------------------------------------------------------------------------
-constexpr int NSTAT = 23;
+constexpr int NSTAT = 24;
 struct StatArgs {
     const double *prim[4];   // u v w T
     const double *rho;
@@ -517,6 +533,8 @@ __global__ __launch_bounds__(256) void k_stats(StatArgs A)
         // sound speed from T: c^2 = gamma p/rho = T/M^2
         double alpha = sqrt(T) / A.mach;
         double S = A.tref / A.tinf;
+        double mu = (1.0 + S) / (T + S) * (T * sqrt(T));
+        double div = ux + vy + wz;
         double vals[NSTAT] = {ux, uy, uz, vx, vy, vz, wx, wy, wz, w0, w1, w2,
                               (w0 * w0 + w1 * w1 + w2 * w2) * 0.5, q0, w0 * u0 + w1 * u1 + w2 * u2,
                               0.5 * rho * (u0 * u0 + u1 * u1 + u2 * u2),
@@ -524,8 +542,9 @@ __global__ __launch_bounds__(256) void k_stats(StatArgs A)
                               (ux * ux + vy * vy + wz * wz) / 3.0,
                               (ux * ux * ux + vy * vy * vy + wz * wz * wz) / 3.0,
                               alpha,
-                              (1.0 + S) / (T + S) * (T * sqrt(T)),
-                              rho, 0.0};
+                              mu, rho,
+                              // solenoidal and dilatational dissipation integrands
+                              mu * (w0 * w0 + w1 * w1 + w2 * w2), mu * div * div};
         for (int s = 0; s < NSTAT; s++) acc[s] = vals[s];
     }
     const int t = threadIdx.x + 64 * threadIdx.y, lane = t & 63, wave = t >> 6;
@@ -588,7 +607,7 @@ __global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
     const int mstep = neg ? -1 : 1;
     const Zone z{n + 1, n, A.ts};
     double out[R];
-    line_half<SCH, R, double>((neg ? sLm : L) + h * (R + 1), m0r, mstep, z, h, neg, out);
+    line_half<SCH, R, double, double>((neg ? sLm : L) + h * (R + 1), m0r, mstep, z, h, neg, out);
     if (neg != (A.upwind < 0)) return;
     for (int r = 0; r < R; r++) {
         int m = m0r + mstep * r;
@@ -664,7 +683,10 @@ template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
-    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+    if (A.mixed)
+        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+    else
+        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
 }
 
 template <int SCH, int RIE>
@@ -736,6 +758,13 @@ struct nuc_ctx {
     double *stat_part = nullptr;
     long stat_nb = 0;
     double *stat_out = nullptr;
+    bool forcing = false;
+    bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
+    // one captured RK3 step, replayed by nuc_step/nuc_step_async (nuc_set_graph)
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    nucforce::Plan fplan;
     std::string err;
     long step = 0;
     double time = 0.0;
@@ -748,7 +777,7 @@ struct nuc_ctx {
 };
 
 static const char *kProfNames[NUC_PROF_SLOTS] = {"k_prim", "k_dt", "k_grad", "k_sweep_x", "k_sweep_y",
-                                                 "k_sweep_z", "k_res"};
+                                                 "k_sweep_z", "k_res", "k_force"};
 
 // brackets one launch with events when profiling is on
 struct ProfScope {
@@ -794,8 +823,8 @@ static int validate(const nuc_config *c, std::string &err)
     }
     if ((long)c->npx * c->npy * c->npz > MAXRANK) { err = "at most 64 ranks"; return NUC_EINVAL; }
     if (c->riemann < 0 || c->riemann > 1 || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
-    if (c->forcing != NUC_FORCING_NONE) { err = "Const forcing is not available on this build"; return NUC_EINVAL; }
-    if (c->precision != NUC_FP64) { err = "only the FP64 path is available on this build"; return NUC_EINVAL; }
+    if (c->forcing != NUC_FORCING_NONE && c->forcing != NUC_FORCING_CONST) { err = "unknown forcing id"; return NUC_EINVAL; }
+    if (c->precision != NUC_FP64 && c->precision != NUC_MIXED) { err = "unknown precision id"; return NUC_EINVAL; }
     if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) { err = "box lengths must be positive"; return NUC_EINVAL; }
     return NUC_OK;
 }
@@ -804,7 +833,9 @@ extern "C" long long nuc_footprint(const nuc_config *c)
 {
     long long N = (long long)c->nx * c->ny * c->nz;
     long long doubles = 15 * N + 5 * N + (c->model == NUC_NAVIERSTOKES3D ? 13 * N : 0);
-    return doubles * 8 + (N / 32 + 4096) * 8 + 4096;
+    if (c->model != NUC_NAVIERSTOKES3D && c->forcing == NUC_FORCING_CONST) doubles += 4 * N;
+    long long fb = (c->forcing == NUC_FORCING_CONST) ? nucforce::plan_bytes(*c) : 0;
+    return doubles * 8 + (N / 32 + 4096) * 8 + 4096 + fb;
 }
 
 extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
@@ -844,22 +875,40 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
         nuc_destroy(x);
         return NUC_ENOMEM;
     };
-    if (hipStreamCreate(&x->stream) != hipSuccess) { delete x; return NUC_EHIP; }
+    // a non-blocking stream and stream-ordered allocations: contexts created
+    // and destroyed while others run (the serving loop) neither wait for nor
+    // stall the other contexts' streams
+    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return NUC_EHIP; }
+    auto dalloc = [&](auto **ptr, size_t bytes) {
+        return hipMallocAsync(reinterpret_cast<void **>(ptr), bytes, x->stream) == hipSuccess;
+    };
     for (int b = 0; b < 3; b++)
-        if (hipMalloc(&x->q[b], 5 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc state");
-    if (hipMalloc(&x->rhs, 5 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc rhs");
-    if (cfg->model == NUC_NAVIERSTOKES3D) {
-        if (hipMalloc(&x->prim, 4 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc prim");
-        if (hipMalloc(&x->fv, 9 * N * sizeof(double)) != hipSuccess) return fail("hipMalloc fv");
+        if (!dalloc(&x->q[b], 5 * N * sizeof(double))) return fail("device allocation (state)");
+    if (!dalloc(&x->rhs, 5 * N * sizeof(double))) return fail("device allocation (rhs)");
+    x->forcing = (cfg->forcing == NUC_FORCING_CONST);
+    x->mixed = (cfg->precision == NUC_MIXED);
+    if (cfg->model == NUC_NAVIERSTOKES3D || x->forcing)
+        if (!dalloc(&x->prim, 4 * N * sizeof(double))) return fail("device allocation (prim)");
+    if (cfg->model == NUC_NAVIERSTOKES3D)
+        if (!dalloc(&x->fv, 9 * N * sizeof(double))) return fail("device allocation (fv)");
+    if (x->forcing) {
+        std::string ferr;
+        int frc = nucforce::plan_create(x->fplan, *cfg, x->stream, ferr);
+        if (frc != NUC_OK) {
+            fprintf(stderr, "nuc_create: %s\n", ferr.c_str());
+            x->forcing = false;
+            nuc_destroy(x);
+            return frc;
+        }
     }
-    if (hipMalloc(&x->lam, 3 * MAXRANK * sizeof(unsigned long long)) != hipSuccess) return fail("hipMalloc lam");
-    if (hipMalloc(&x->scal, 8 * sizeof(double)) != hipSuccess) return fail("hipMalloc scal");
+    if (!dalloc(&x->lam, 3 * MAXRANK * sizeof(unsigned long long))) return fail("device allocation (lam)");
+    if (!dalloc(&x->scal, 8 * sizeof(double))) return fail("device allocation (scal)");
     long lines_z = x->N / x->n[2];
     x->res_nb = (lines_z + sweep_lpb(x->cpt[2]) - 1) / sweep_lpb(x->cpt[2]);
-    if (hipMalloc(&x->res_part, x->res_nb * sizeof(double)) != hipSuccess) return fail("hipMalloc res");
-    (void)hipMemset(x->lam, 0, 3 * MAXRANK * sizeof(unsigned long long));
-    (void)hipMemset(x->scal, 0, 8 * sizeof(double));
-    if (hipDeviceSynchronize() != hipSuccess) return fail("device init");
+    if (!dalloc(&x->res_part, x->res_nb * sizeof(double))) return fail("device allocation (residual)");
+    (void)hipMemsetAsync(x->lam, 0, 3 * MAXRANK * sizeof(unsigned long long), x->stream);
+    (void)hipMemsetAsync(x->scal, 0, 8 * sizeof(double), x->stream);
+    if (hipStreamSynchronize(x->stream) != hipSuccess) return fail("device init");
     *out = x;
     return NUC_OK;
 }
@@ -867,11 +916,17 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
 extern "C" int nuc_destroy(nuc_ctx *x)
 {
     if (!x) return NUC_OK;
-    if (x->stream) (void)hipStreamSynchronize(x->stream);
-    for (int b = 0; b < 3; b++) (void)hipFree(x->q[b]);
-    (void)hipFree(x->rhs); (void)hipFree(x->prim); (void)hipFree(x->fv); (void)hipFree(x->lam);
-    (void)hipFree(x->scal); (void)hipFree(x->res_part); (void)hipFree(x->stat_part); (void)hipFree(x->stat_out);
-    if (x->stream) (void)hipStreamDestroy(x->stream);
+    if (!x->stream) { delete x; return NUC_OK; }
+    (void)hipStreamSynchronize(x->stream);
+    if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
+    if (x->graph) (void)hipGraphDestroy(x->graph);
+    void *bufs[] = {x->q[0], x->q[1], x->q[2], x->rhs, x->prim, x->fv, x->lam, x->scal, x->res_part,
+                    x->stat_part, x->stat_out};
+    for (void *b : bufs)
+        if (b) (void)hipFreeAsync(b, x->stream);
+    if (x->forcing) nucforce::plan_destroy(x->fplan, x->stream);
+    (void)hipStreamSynchronize(x->stream);
+    (void)hipStreamDestroy(x->stream);
     delete x;
     return NUC_OK;
 }
@@ -906,11 +961,12 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     {
         PrimArgs P{};
         for (int v = 0; v < 5; v++) P.q[v] = qi + v * N;
-        if (ns) for (int f = 0; f < 4; f++) P.prim[f] = x->prim + f * N;
+        const bool wp = ns || x->forcing;
+        if (wp) for (int f = 0; f < 4; f++) P.prim[f] = x->prim + f * N;
         P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
         P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2];
         P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
-        P.riemann = c.riemann; P.do_lam = lam_needed ? 1 : 0; P.write_prim = ns ? 1 : 0;
+        P.riemann = c.riemann; P.do_lam = lam_needed ? 1 : 0; P.write_prim = wp ? 1 : 0;
         P.gamma = c.gamma; P.mach = c.mach; P.lam = x->lam;
         if (P.do_lam || P.write_prim) {
             long nb = (N + 4095) / 4096;
@@ -938,7 +994,16 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         G.tref = c.t_ref; G.tinf = c.t_inf;
         dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
         ProfScope ps(x, 2);
-        hipLaunchKernelGGL(k_grad, grid, dim3(64, 4, 1), 0, x->stream, G);
+        if (x->mixed) hipLaunchKernelGGL(k_grad<float>, grid, dim3(64, 4, 1), 0, x->stream, G);
+        else hipLaunchKernelGGL(k_grad<double>, grid, dim3(64, 4, 1), 0, x->stream, G);
+    }
+    if (x->forcing) {
+        // Const forcing (src/physicsModel.cpp:1368-1677) initialises rhs; the
+        // x sweep accumulates onto it and the z sweep adds <f.u> back
+        double *r[5];
+        for (int v = 0; v < 5; v++) r[v] = x->rhs + v * N;
+        ProfScope ps(x, 7);
+        nucforce::enqueue(x->fplan, x->prim, x->prim + N, x->prim + 2 * N, r, x->scal + 4, x->stream);
     }
     // fv component order per direction: x (txx,txy,txz,ex) y (txy,tyy,tyz,ey) z (txz,tyz,tzz,ez)
     static const int fvmap[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
@@ -954,6 +1019,9 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         for (int v = 0; v < 5; v++) { S.q[v] = qi + v * N; S.rhs[v] = x->rhs + v * N; }
         if (ns) for (int o = 0; o < 4; o++) S.fv[o] = x->fv + (long)fvmap[d][o] * N;
         S.stage = stage;
+        S.accum = (d == 0 && x->forcing) ? 1 : 0;
+        S.mixed = x->mixed ? 1 : 0;
+        S.fmean = (d == 2 && x->forcing) ? x->scal + 4 : nullptr;
         S.epilogue = (d == 2 && epilogue) ? 1 : 0;
         if (S.epilogue) {
             for (int v = 0; v < 5; v++) { S.qn[v] = x->q[0] + v * N; S.qout[v] = x->q[outb] + v * N; }
@@ -971,15 +1039,49 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     return NUC_OK;
 }
 
+static void enqueue_one_step(nuc_ctx *x)
+{
+    enqueue_stage(x, 0, 0, 1, true);
+    enqueue_stage(x, 1, 1, 2, true);
+    enqueue_stage(x, 2, 2, 0, true);
+}
+
 static int enqueue_steps(nuc_ctx *x, int nsteps)
 {
-    for (int it = 0; it < nsteps; it++) {
-        enqueue_stage(x, 0, 0, 1, true);
-        enqueue_stage(x, 1, 1, 2, true);
-        enqueue_stage(x, 2, 2, 0, true);
+    if (x->use_graph && !x->prof) {
+        // the step's launches depend only on the context's buffers and
+        // configuration (dt, residual and time live on the device), so one
+        // capture serves every later step; it holds kernel nodes only
+        if (!x->gexec) {
+            NUC_CHECK(x, hipStreamBeginCapture(x->stream, hipStreamCaptureModeRelaxed));
+            enqueue_one_step(x);
+            hipError_t e = hipStreamEndCapture(x->stream, &x->graph);
+            if (e != hipSuccess) { x->err = std::string("graph capture: ") + hipGetErrorString(e); return NUC_EHIP; }
+            NUC_CHECK(x, hipGraphInstantiate(&x->gexec, x->graph, nullptr, nullptr, 0));
+        }
+        for (int it = 0; it < nsteps; it++) NUC_CHECK(x, hipGraphLaunch(x->gexec, x->stream));
+        return NUC_OK;
     }
+    for (int it = 0; it < nsteps; it++) enqueue_one_step(x);
     NUC_CHECK(x, hipGetLastError());
     return NUC_OK;
+}
+
+extern "C" int nuc_set_graph(nuc_ctx *x, int enable)
+{
+    if (!x) return NUC_EINVAL;
+    x->use_graph = enable != 0;
+    return NUC_OK;
+}
+
+extern "C" int nuc_query(nuc_ctx *x)
+{
+    if (!x) return NUC_EINVAL;
+    hipError_t e = hipStreamQuery(x->stream);
+    if (e == hipSuccess) return NUC_OK;
+    if (e == hipErrorNotReady) return NUC_EBUSY;
+    x->err = std::string("hipStreamQuery: ") + hipGetErrorString(e);
+    return NUC_EHIP;
 }
 
 extern "C" int nuc_step_async(nuc_ctx *x, int nsteps)
@@ -1005,7 +1107,8 @@ extern "C" int nuc_sync(nuc_ctx *x, nuc_step_info *info)
     }
     x->ev.clear();
     double sc[8];
-    NUC_CHECK(x, hipMemcpy(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost));
+    NUC_CHECK(x, hipMemcpyAsync(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
     x->time = sc[2];
     x->step += x->pending;
     x->pending = 0;
@@ -1038,24 +1141,32 @@ extern "C" int nuc_rhs(nuc_ctx *x, double *rhs_host, double *dt_local)
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     if (rhs_host)
-        NUC_CHECK(x, hipMemcpy(rhs_host, x->rhs, 5 * x->N * sizeof(double), hipMemcpyDeviceToHost));
-    if (dt_local) {
-        double sc[8];
-        NUC_CHECK(x, hipMemcpy(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost));
-        *dt_local = sc[3];
-    }
+        NUC_CHECK(x, hipMemcpyAsync(rhs_host, x->rhs, 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    double sc[8];
+    NUC_CHECK(x, hipMemcpyAsync(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    if (dt_local) *dt_local = sc[3];
     return NUC_OK;
 }
 
 extern "C" int nuc_stats(nuc_ctx *x, double *out)
 {
-    if (!x || !out) return NUC_EINVAL;
+    if (!out) return NUC_EINVAL;
+    double ex[NUC_NSTATS_EX];
+    int rc = nuc_stats_ex(x, ex, NUC_NSTATS_EX);
+    if (rc == NUC_OK) memcpy(out, ex, 25 * sizeof(double));
+    return rc;
+}
+
+extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
+{
+    if (!x || !out || nout < 1) return NUC_EINVAL;
     const long N = x->N;
     const nuc_config &c = x->c;
     double *prim = x->prim;
     bool own_prim = false;
     if (!prim) {
-        NUC_CHECK(x, hipMalloc(&prim, 4 * N * sizeof(double)));
+        NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&prim), 4 * N * sizeof(double), x->stream));
         own_prim = true;
     }
     PrimArgs P{};
@@ -1070,11 +1181,13 @@ extern "C" int nuc_stats(nuc_ctx *x, double *out)
     dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
     long nb = (long)grid.x * grid.y * grid.z;
     if (nb > x->stat_nb) {
-        (void)hipFree(x->stat_part);
-        NUC_CHECK(x, hipMalloc(&x->stat_part, nb * NSTAT * sizeof(double)));
+        if (x->stat_part) (void)hipFreeAsync(x->stat_part, x->stream);
+        x->stat_part = nullptr;
+        NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&x->stat_part), nb * NSTAT * sizeof(double), x->stream));
         x->stat_nb = nb;
     }
-    if (!x->stat_out) NUC_CHECK(x, hipMalloc(&x->stat_out, NSTAT * sizeof(double)));
+    if (!x->stat_out)
+        NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&x->stat_out), NSTAT * sizeof(double), x->stream));
     StatArgs S{};
     for (int f = 0; f < 4; f++) S.prim[f] = prim + f * N;
     S.rho = x->q[0];
@@ -1089,11 +1202,12 @@ extern "C" int nuc_stats(nuc_ctx *x, double *out)
     hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(64), 0, x->stream, x->stat_part, nb, x->stat_out);
     NUC_CHECK(x, hipGetLastError());
     double s[NSTAT];
+    NUC_CHECK(x, hipMemcpyAsync(s, x->stat_out, sizeof(s), hipMemcpyDeviceToHost, x->stream));
+    if (own_prim) (void)hipFreeAsync(prim, x->stream);
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
-    NUC_CHECK(x, hipMemcpy(s, x->stat_out, sizeof(s), hipMemcpyDeviceToHost));
-    if (own_prim) (void)hipFree(prim);
     const double Nd = (double)N;
-    for (int t = 0; t < 16; t++) out[t] = s[t] / Nd;
+    double o[NUC_NSTATS_EX];
+    for (int t = 0; t < 16; t++) o[t] = s[t] / Nd;
     double u_rms_aver = s[16] / Nd, div_aver = s[17] / Nd, Skew3 = s[18] / Nd;
     double alpha_aver = s[19] / Nd, miu_aver = s[20] / Nd, rho_bar = s[21] / Nd, Skew2 = div_aver;
     // src/postproc.cpp:858-872
@@ -1101,15 +1215,20 @@ extern "C" int nuc_stats(nuc_ctx *x, double *out)
     double lambda = std::sqrt(u_rms_aver / div_aver);
     double Re_lambda = std::sqrt(u_rms_aver) * lambda * rho_bar / miu_aver * c.reynolds;
     double tau_t = lambda / std::sqrt(u_rms_aver);
-    out[16] = std::sqrt(u_rms_aver);
-    out[17] = std::sqrt(u_rms_aver * 3.0) / alpha_aver;
-    out[18] = std::sqrt(div_aver);
-    out[19] = miu_aver;
-    out[20] = Re_lambda;
-    out[21] = lambda;
-    out[22] = rho_bar;
-    out[23] = tau_t;
-    out[24] = skew;
+    o[16] = std::sqrt(u_rms_aver);
+    o[17] = std::sqrt(u_rms_aver * 3.0) / alpha_aver;
+    o[18] = std::sqrt(div_aver);
+    o[19] = miu_aver;
+    o[20] = Re_lambda;
+    o[21] = lambda;
+    o[22] = rho_bar;
+    o[23] = tau_t;
+    o[24] = skew;
+    // dissipation eps = <mu |omega|^2>/Re + 4/3 <mu (div u)^2>/Re with the
+    // Sutherland viscosity of getMiu (src/physicsModel.cpp:193-225)
+    o[25] = s[22] / Nd / c.reynolds;
+    o[26] = 4.0 / 3.0 * s[23] / Nd / c.reynolds;
+    memcpy(out, o, (size_t)std::min(nout, NUC_NSTATS_EX) * sizeof(double));
     return NUC_OK;
 }
 

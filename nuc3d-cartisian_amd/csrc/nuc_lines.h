@@ -155,13 +155,15 @@ __device__ __forceinline__ void w5_parts(const T *f, T &b0, T &b1, T &b2, T &q0,
 
 // Reconstruct one sign of a line.  Lw points at the lane's window in the
 // skewed line buffer (slot h*(R+1); the f^- buffer is mirrored, see Skew):
-// win[t] = Lw[at(t-RAD-1)] in upwind order, row r uses
-// win[r .. r+2*RAD] in upwind coordinates.  ml0/mstep give the rank-local
-// position of row 0 and its step (+1 for f^+, -1 for f^- whose rows run
-// backwards); m0r is the interface index of row 0.
-template <int SCH, int R, typename T>
+// win[t] = Lw[at(t-RAD-1)] in upwind order, row r uses win[r .. r+2*RAD] in
+// upwind coordinates.  m0r is the interface index of row 0 and mstep its step
+// (+1 for f^+, -1 for f^- whose rows run backwards).
+// T is the type of the stencils (smoothness indicators, candidate values);
+// TW the type of the nonlinear weights, the compact rows, their tridiagonal
+// solve and the result (FP64 in both precision paths).
+template <int SCH, int R, typename T, typename TW>
 __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const Zone &z, int h, bool neg,
-                                          T out[R])
+                                          TW out[R])
 {
     constexpr int RAD = Radius<SCH>::value;
     constexpr int W = R + 2 * RAD;
@@ -186,11 +188,11 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         for (int r = 0; r < R; r++) {
             T b0, b1, b2, q0, q1, q2;
             w5_parts<T>(win + r, b0, b1, b2, q0, q1, q2);
-            const T eps = T(1.0e-6);
-            T s0 = sq(eps + b0), s1 = sq(eps + b1), s2 = sq(eps + b2);
+            const TW eps = TW(1.0e-6);
+            TW s0 = sq(eps + TW(b0)), s1 = sq(eps + TW(b1)), s2 = sq(eps + TW(b2));
             // alpha_k = c_k / s_k; normalised with a single reciprocal
-            T n0 = T(0.1) * (s1 * s2), n1 = T(0.6) * (s0 * s2), n2 = T(0.3) * (s0 * s1);
-            out[r] = (n0 * q0 + n1 * q1 + n2 * q2) * frcp(n0 + n1 + n2);
+            TW n0 = TW(0.1) * (s1 * s2), n1 = TW(0.6) * (s0 * s2), n2 = TW(0.3) * (s0 * s1);
+            out[r] = (n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2);
         }
     } else if constexpr (SCH == WENO5Z) {
 #pragma unroll
@@ -201,62 +203,64 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             // the positive inner sweep's is2 reads f_{i+1} twice (src/weno5z.cpp:80)
             bool typo = !neg && ml[r] >= z.ts && ml[r] <= z.nloc - z.ts;
             if (typo) b2 = T(13.0 / 12.0) * sq(f[2] - f[3]) + T(0.25) * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
-            const T eps = T(1.0e-40);
-            T tau = fabs(b2 - b0);
-            T a0 = T(0.1) * (T(1) + sq(tau / (b0 + eps)));
-            T a1 = T(0.6) * (T(1) + sq(tau / (b1 + eps)));
-            T a2 = T(0.3) * (T(1) + sq(tau / (b2 + eps)));
-            out[r] = (a0 * q0 + a1 * q1 + a2 * q2) * frcp(a0 + a1 + a2);
+            const TW eps = TW(1.0e-40);
+            const TW B0 = b0, B1 = b1, B2 = b2;
+            TW tau = fabs(B2 - B0);
+            TW a0 = TW(0.1) * (TW(1) + sq(tau / (B0 + eps)));
+            TW a1 = TW(0.6) * (TW(1) + sq(tau / (B1 + eps)));
+            TW a2 = TW(0.3) * (TW(1) + sq(tau / (B2 + eps)));
+            out[r] = (a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2);
         }
     } else if constexpr (SCH == WENO7JS) {
 #pragma unroll
-        for (int r = 0; r < R; r++) out[r] = weno7z<T>(win + r, T(1.0e-20));
+        for (int r = 0; r < R; r++) out[r] = weno7z<T, TW>(win + r, TW(1.0e-20));
     } else if constexpr (SCH == CRWENO5) {
-        Chunk<R, T> ch;
+        Chunk<R, TW> ch;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const T *f = win + r;
             T b0, b1, b2, q0, q1, q2;
             w5_parts<T>(f, b0, b1, b2, q0, q1, q2);
-            const T eps = T(1.0e-6);
-            T s0 = sq(eps + b0), s1 = sq(eps + b1), s2 = sq(eps + b2);
-            T P0 = s1 * s2, P1 = s0 * s2, P2 = s0 * s1;
+            const TW eps = TW(1.0e-6);
+            TW s0 = sq(eps + TW(b0)), s1 = sq(eps + TW(b1)), s2 = sq(eps + TW(b2));
+            TW P0 = s1 * s2, P1 = s0 * s2, P2 = s0 * s1;
             // explicit WENO5-JS closure value (src/crweno5.cpp:87-129) from the same indicators
-            T n0 = T(0.1) * P0, n1 = T(0.6) * P1, n2 = T(0.3) * P2;
-            T wexp = (n0 * q0 + n1 * q1 + n2 * q2) * frcp(n0 + n1 + n2);
+            TW n0 = TW(0.1) * P0, n1 = TW(0.6) * P1, n2 = TW(0.3) * P2;
+            TW wexp = (n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2);
             // compact row (src/crweno5.cpp:133-162) scaled by the weight
             // denominator, which leaves the tridiagonal solution unchanged
-            T m0 = T(0.2) * P0, m1 = T(0.5) * P1, m2 = T(0.3) * P2;
-            T lo = T(2.0 / 3.0) * m0 + T(1.0 / 3.0) * m1;
-            T di = T(1.0 / 3.0) * m0 + T(2.0 / 3.0) * (m1 + m2);
-            T hi = T(1.0 / 3.0) * m2;
-            T rhs = (m0 * f[1] + (T(5) * (m0 + m1) + m2) * f[2] + (m1 + T(5) * m2) * f[3]) * T(1.0 / 6.0);
+            TW m0 = TW(0.2) * P0, m1 = TW(0.5) * P1, m2 = TW(0.3) * P2;
+            TW lo = TW(2.0 / 3.0) * m0 + TW(1.0 / 3.0) * m1;
+            TW di = TW(1.0 / 3.0) * m0 + TW(2.0 / 3.0) * (m1 + m2);
+            TW hi = TW(1.0 / 3.0) * m2;
+            TW rhs = (m0 * TW(f[1]) + (TW(5) * (m0 + m1) + m2) * TW(f[2]) + (m1 + TW(5) * m2) * TW(f[3]))
+                     * TW(1.0 / 6.0);
             int m = m0r + mstep * r;
             bool valid = m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
-            T a = (valid && !expl) ? lo : T(0);
-            T b = (valid && !expl) ? di : T(1);
-            T c = (valid && !expl) ? hi : T(0);
-            T d = valid ? (expl ? wexp : rhs) : T(0);
+            TW a = (valid && !expl) ? lo : TW(0);
+            TW b = (valid && !expl) ? di : TW(1);
+            TW c = (valid && !expl) ? hi : TW(0);
+            TW d = valid ? (expl ? wexp : rhs) : TW(0);
             ch.row(r, a, b, c, d);
         }
         ch.solve(h);
 #pragma unroll
         for (int r = 0; r < R; r++) out[r] = ch.Df[r];
     } else {  // HCCS
-        Chunk<R, T> ch;
+        Chunk<R, TW> ch;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const T *f = win + r;   // 7-point window for the closures, 5-point from f+1
             int m = m0r + mstep * r;
             bool valid = m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
-            T a = T(0), b = T(1), c = T(0), d = T(0);
+            TW a = TW(0), b = TW(1), c = TW(0), d = TW(0);
             if (valid) {
                 if (expl) {
-                    d = weno7z<T>(f, T(1.0e-40));
+                    d = weno7z<T, TW>(f, TW(1.0e-40));
                 } else {
-                    hccs_row<T>(f[1], f[2], f[3], f[4], f[5], a, b, c, d);
+                    hccs_row<T, TW>(f[1], f[2], f[3], f[4], f[5], a, b, c, d);
                 }
             }
             ch.row(r, a, b, c, d);

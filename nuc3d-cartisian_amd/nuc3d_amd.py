@@ -1,4 +1,5 @@
-"""Python binding of libnuc3d_amd.so (include/nuc3d_gpu.h) through ctypes.
+"""Python binding of libnuc3d_amd.so (include/nuc3d_gpu.h, include/nuc3d_serve.h)
+through ctypes.
 
 Loads the in-tree library and fails loudly when it is missing; there is no
 CPU fallback.  Arrays are numpy float64, x-fastest, conserved state as
@@ -19,15 +20,21 @@ LF, AUSM = 0, 1
 EULER3D, NAVIERSTOKES3D = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
 FORCING_NONE, FORCING_CONST = 0, 1
+NSTATS_EX = 27
 FP64, MIXED = 0, 1
 SCHEMES = {"weno5js": WENO5JS, "weno5z": WENO5Z, "weno7js": WENO7JS, "crweno5": CRWENO5, "hccs": HCCS}
-STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_ENONFINITE", 5: "NUC_ENODEV"}
+STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_ENONFINITE", 5: "NUC_ENODEV",
+          6: "NUC_EBUSY"}
+NUC_OK, NUC_EBUSY = 0, 6
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
 
-# every entry point include/nuc3d_gpu.h declares
+# every entry point include/nuc3d_gpu.h and include/nuc3d_serve.h declare
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
-           "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_sync", "nuc_stats",
-           "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp", "nuc_init_tgv",
-           "nuc_init_hit", "nuc_profile", "nuc_profile_read"]
+           "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_set_graph", "nuc_query", "nuc_sync",
+           "nuc_stats", "nuc_stats_ex", "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp",
+           "nuc_init_tgv", "nuc_init_hit", "nuc_profile", "nuc_profile_read",
+           "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
+           "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error"]
 
 
 class NucConfig(ctypes.Structure):
@@ -48,6 +55,21 @@ class NucConfig(ctypes.Structure):
 class StepInfo(ctypes.Structure):
     _fields_ = [("dt", ctypes.c_double), ("time", ctypes.c_double), ("res", ctypes.c_double),
                 ("step", ctypes.c_long)]
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_longlong), ("max_active", ctypes.c_int),
+                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+
+
+class JobResult(ctypes.Structure):
+    _fields_ = [("info", StepInfo), ("submit_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("state", ctypes.c_int), ("error", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_longlong) for k in ("completed", "failed", "queued", "active", "peak_active",
+                                                  "bytes_in_use", "peak_bytes")]
 
 
 class NucError(RuntimeError):
@@ -78,8 +100,11 @@ def lib():
         L.nuc_rhs.argtypes = [vp, _dp, _dp]
         L.nuc_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
         L.nuc_step_async.argtypes = [vp, ctypes.c_int]
+        L.nuc_set_graph.argtypes = [vp, ctypes.c_int]
+        L.nuc_query.argtypes = [vp]
         L.nuc_sync.argtypes = [vp, ctypes.POINTER(StepInfo)]
         L.nuc_stats.argtypes = [vp, _dp]
+        L.nuc_stats_ex.argtypes = [vp, _dp, ctypes.c_int]
         L.nuc_op_reconstruct.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int, _dp, ctypes.c_int]
         L.nuc_op_diff.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, ctypes.c_int,
@@ -93,6 +118,17 @@ def lib():
         L.nuc_profile.argtypes = [vp, ctypes.c_int]
         L.nuc_profile_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(ctypes.c_char_p)]
+        L.nuc_server_create.argtypes = [ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)]
+        L.nuc_server_destroy.argtypes = [vp]
+        L.nuc_server_submit.argtypes = [vp, ctypes.POINTER(NucConfig), _dp, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_longlong)]
+        L.nuc_server_result.argtypes = [vp, ctypes.c_longlong, _dp, ctypes.POINTER(JobResult)]
+        L.nuc_server_drain.argtypes = [vp, ctypes.c_double]
+        L.nuc_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
+        L.nuc_server_now.argtypes = [vp]
+        L.nuc_server_now.restype = ctypes.c_double
+        L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
+        L.nuc_server_job_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -208,6 +244,18 @@ class Context:
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
 
+    def set_graph(self, enable=True):
+        """Replay one captured hipGraph per RK3 step instead of per-kernel launches."""
+        _check(lib().nuc_set_graph(self.h, 1 if enable else 0), "nuc_set_graph", self.h)
+
+    def query(self):
+        """True when all enqueued work has finished (never blocks)."""
+        rc = lib().nuc_query(self.h)
+        if rc == NUC_EBUSY:
+            return False
+        _check(rc, "nuc_query", self.h)
+        return True
+
     def sync(self):
         info = StepInfo()
         _check(lib().nuc_sync(self.h, ctypes.byref(info)), "nuc_sync", self.h)
@@ -219,7 +267,7 @@ class Context:
     def profile_read(self):
         """{kernel: (total_ms, launches)} accumulated since profile(True)."""
         out = {}
-        for slot in range(7):
+        for slot in range(8):
             ms = ctypes.c_double()
             n = ctypes.c_long()
             name = ctypes.c_char_p()
@@ -231,6 +279,12 @@ class Context:
     def stats(self):
         out = np.zeros(25)
         _check(lib().nuc_stats(self.h, _ptr(out)), "nuc_stats", self.h)
+        return out
+
+    def stats_ex(self):
+        """nuc_stats values followed by the solenoidal and dilatational dissipation."""
+        out = np.zeros(NSTATS_EX)
+        _check(lib().nuc_stats_ex(self.h, _ptr(out), NSTATS_EX), "nuc_stats_ex", self.h)
         return out
 
 
@@ -267,3 +321,61 @@ def op_rk3(stage, rhs, un, ui, dt):
                             _ptr(np.ascontiguousarray(un, dtype=np.float64).ravel()), _ptr(ui), float(dt), ui.size),
            "nuc_op_rk3")
     return ui
+
+
+class Server:
+    """The serving loop (include/nuc3d_serve.h): submit jobs, fetch results."""
+
+    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_slice=0, use_graphs=True):
+        c = ServerConfig(int(hbm_budget), int(max_active), int(steps_per_slice), 1 if use_graphs else 0)
+        h = ctypes.c_void_p()
+        _check(lib().nuc_server_create(ctypes.byref(c), ctypes.byref(h)), "nuc_server_create")
+        self.h = h
+        self._shapes = {}
+
+    def close(self):
+        if self.h:
+            lib().nuc_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def now(self):
+        return lib().nuc_server_now(self.h)
+
+    def submit(self, cfg, q0, nsteps):
+        q0 = np.ascontiguousarray(q0, dtype=np.float64)
+        jid = ctypes.c_longlong()
+        _check(lib().nuc_server_submit(self.h, ctypes.byref(cfg), _ptr(q0), int(nsteps), ctypes.byref(jid)),
+               "nuc_server_submit")
+        self._shapes[jid.value] = q0.size
+        return jid.value
+
+    def result(self, jid, want_state=True):
+        """None while the job is queued or running; else (state array or None, JobResult)."""
+        r = JobResult()
+        q = np.empty(self._shapes[jid]) if want_state else None
+        rc = lib().nuc_server_result(self.h, jid, _ptr(q), ctypes.byref(r))
+        if rc == NUC_EBUSY:
+            return None
+        _check(rc, "nuc_server_result")
+        self._shapes.pop(jid, None)
+        return (q if r.state == JOB_DONE else None), r
+
+    def drain(self, timeout_s=-1.0):
+        return lib().nuc_server_drain(self.h, float(timeout_s)) == NUC_OK
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().nuc_server_stats_get(self.h, ctypes.byref(st)), "nuc_server_stats_get")
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

@@ -1,0 +1,123 @@
+"""The serving loop on the GPU: every job's result is bit-identical to the same
+job run alone on a plain context (per-kernel launches, no graphs), whatever
+else runs beside it; admission respects the HBM budget; jobs submitted while
+others run are admitted as slots free up and overlap them on the device.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import nuc3d_amd as G
+
+pytestmark = pytest.mark.gpu
+
+
+def _job(n, scheme, mach, re, seed, tgv):
+    cfg = G.make_config(n=n, scheme=scheme, vis=G.CD6, riemann=G.LF, model=G.NAVIERSTOKES3D, mach=mach,
+                        reynolds=re, cfl=0.5)
+    q = G.init_tgv(cfg) if tgv else G.init_hit(cfg, k0=3.0, u_rms=0.5, seed=seed)
+    return cfg, q
+
+
+JOBS = [
+    ((32, 32, 32), G.WENO5JS, 0.1, 1600.0, 1, True, 7),
+    ((48, 40, 32), G.CRWENO5, 0.3, 800.0, 2, False, 5),
+    ((24, 24, 24), G.WENO5Z, 0.5, 400.0, 3, False, 9),
+    ((64, 64, 64), G.WENO5JS, 0.2, 3200.0, 4, False, 4),
+    ((40, 40, 40), G.CRWENO5, 0.4, 1200.0, 5, True, 6),
+    ((32, 48, 24), G.WENO5Z, 0.15, 2000.0, 6, True, 8),
+]
+
+
+def _alone(cfg, q, steps):
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        ctx.step(steps)
+        return ctx.download()
+
+
+def _wait(srv, jid, timeout=120):
+    t0 = time.time()
+    while time.time() - t0 < timeout:
+        r = srv.result(jid)
+        if r is not None:
+            return r
+        time.sleep(0.002)
+    raise TimeoutError(f"job {jid}")
+
+
+def test_results_identical_alone_and_in_the_mix():
+    print("serving mix", flush=True)
+    ref, ids = {}, {}
+    with G.Server(hbm_budget=8 << 30, max_active=4) as srv:
+        for i, (n, sch, mach, re, seed, tgv, steps) in enumerate(JOBS):
+            cfg, q = _job(n, sch, mach, re, seed, tgv)
+            ids[i] = srv.submit(cfg, q, steps)
+            ref[i] = (cfg, q, steps)
+            if i == 2:
+                time.sleep(0.05)   # later jobs arrive while the first ones run
+        out = {i: _wait(srv, jid) for i, jid in ids.items()}
+        st = srv.stats()
+    print("server stats", st, flush=True)
+    assert st["completed"] == len(JOBS) and st["failed"] == 0
+    assert st["peak_active"] >= 2
+    for i, (state, r) in out.items():
+        cfg, q, steps = ref[i]
+        alone = _alone(cfg, q, steps)
+        assert r.state == G.JOB_DONE and r.info.step == steps
+        assert np.array_equal(state, alone), (i, np.abs(state - alone).max())
+    # overlap: some job was admitted before an earlier one had finished
+    rs = [out[i][1] for i in range(len(JOBS))]
+    assert any(rs[j].admit_s < rs[i].done_s for i in range(len(JOBS)) for j in range(i + 1, len(JOBS)))
+
+
+def test_admission_respects_hbm_budget():
+    print("serving budget", flush=True)
+    cfgs = [_job((48, 48, 48), G.WENO5JS, 0.2, 1000.0, k, False) for k in range(5)]
+    fp = G.footprint(cfgs[0][0])
+    budget = int(2.5 * fp)
+    with G.Server(hbm_budget=budget, max_active=8) as srv:
+        ids = [srv.submit(c, q, 4) for c, q in cfgs]
+        out = [_wait(srv, j) for j in ids]
+        st = srv.stats()
+    print("server stats", st, flush=True)
+    assert st["peak_bytes"] <= budget and st["peak_active"] <= 2
+    assert all(r.state == G.JOB_DONE for _, r in out)
+
+
+def test_graph_replay_matches_launches():
+    """Every replay of the captured step, with eager work and synchronisation
+    between replays, equals the eager launches bit for bit."""
+    cfg, q = _job((32, 32, 32), G.CRWENO5, 0.3, 1000.0, 9, False)
+    eager = []
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        for _ in range(5):
+            ctx.step(1)
+            eager.append(ctx.download())
+    with G.Context(cfg) as ctx:
+        ctx.set_graph(True)
+        ctx.upload(q)
+        for k in range(5):
+            ctx.step(1)
+            st = ctx.stats()           # eager kernels between replays
+            assert np.isfinite(st).all()
+            assert np.array_equal(ctx.download(), eager[k]), k
+        ctx.upload(eager[1])
+        ctx.step_async(3)
+        ctx.sync()
+        assert np.array_equal(ctx.download(), eager[4])
+
+
+def test_failed_job_does_not_disturb_others():
+    good, q = _job((32, 32, 32), G.WENO5JS, 0.2, 1000.0, 3, True)
+    bad = G.make_config(n=(18, 16, 16), npr=(4, 1, 1))
+    with G.Server(hbm_budget=4 << 30, max_active=4) as srv:
+        j1 = srv.submit(good, q, 3)
+        j2 = srv.submit(bad, np.ones(5 * 18 * 16 * 16), 3)
+        j3 = srv.submit(good, q, 3)
+        r1, r2, r3 = _wait(srv, j1), _wait(srv, j2), _wait(srv, j3)
+    assert r2[1].state == G.JOB_FAILED
+    assert r1[1].state == G.JOB_DONE and r3[1].state == G.JOB_DONE
+    assert np.array_equal(r1[0], r3[0])

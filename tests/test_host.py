@@ -1,0 +1,120 @@
+"""Host-side checks that need no GPU: the C ABI library loads and exports every
+symbol its headers declare, the inp/*.in readers follow the reference's parsing
+rules, footprints, and the serving loop's handling of a job that cannot start.
+
+The inp fixtures under tests/golden/ are written in the reference's formats:
+block.in (src/block.cpp:49-61), Method.in (src/fieldOperator.cpp:21-125),
+PhysModel.in (six positional pairs, then key/value; src/physicsModel.cpp:72-156)
+and IOController.in (src/IOcontroller.cpp:12-110).
+"""
+import os
+import re
+import time
+
+import numpy as np
+import pytest
+
+import nuc3d_amd as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = os.path.join(ROOT, "tests", "golden")
+
+
+def _declared():
+    names = set()
+    for h in ("nuc3d_gpu.h", "nuc3d_serve.h"):
+        src = open(os.path.join(ROOT, "include", h)).read()
+        src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+        for m in re.finditer(r"^(?:const\s+)?(?:int|long long|double|char|void)\s*\*?\s*(nuc_\w+)\s*\(", src, re.M):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    declared = _declared()
+    assert len(declared) >= 30
+    assert declared == set(G.EXPORTS), declared.symmetric_difference(G.EXPORTS)
+    L = G.lib()
+    for name in sorted(declared):
+        assert hasattr(L, name), name
+
+
+def test_inp_tgv64():
+    c, s, e = G.config_from_inp(os.path.join(GOLD, "inp_tgv64"))
+    assert (c.npx, c.npy, c.npz, c.nx, c.ny, c.nz) == (1, 1, 1, 64, 64, 64)
+    assert abs(c.lx - 2 * np.pi) < 1e-15
+    assert list(c.scheme) == [G.WENO5JS] * 3 and list(c.vis_scheme) == [G.CD6] * 3
+    assert (c.riemann, c.model, c.viscosity, c.forcing) == (G.LF, G.NAVIERSTOKES3D, G.SUTHERLAND, G.FORCING_NONE)
+    assert (c.reynolds, c.mach, c.prandtl, c.gamma) == (1600.0, 0.1, 0.72, 1.4)
+    assert (s, e, c.cfl) == (0, 200, 0.5)
+    assert c.precision == G.FP64
+
+
+def test_inp_sixth_positional_pair_is_the_forcing_slot():
+    """PhysModel.in's first six lines are read positionally; a file whose sixth
+    pair is 'Reynolds 1600.0' (no Forcing line) loses that Reynolds number: the
+    value goes to the forcing slot (unknown name -> None) and Reynolds keeps its
+    default 1000 (src/physicsModel.cpp:22-43,72-77,104-110)."""
+    c, s, e = G.config_from_inp(os.path.join(GOLD, "inp_reynolds_as_sixth_pair"))
+    assert c.forcing == G.FORCING_NONE
+    assert c.reynolds == 1000.0
+    assert (c.mach, c.prandtl) == (0.5, 0.7)
+    assert (c.riemann, c.model) == (G.AUSM, G.EULER3D)
+    assert (c.npx, c.npy, c.npz, c.nx, c.ny, c.nz) == (2, 1, 2, 32, 48, 64)
+    assert (c.lx, c.ly, c.lz) == (6.0, 7.0, 8.0)
+    assert list(c.scheme) == [G.WENO5Z, G.CRWENO5, G.WENO7JS]
+    # scheme_y_vis absent -> default cd6 (src/fieldOperator.cpp:22-33)
+    assert list(c.vis_scheme) == [G.CD8, G.CD6, G.CD8]
+    assert (s, e, c.cfl) == (5, 1000, 0.6)
+
+
+def test_inp_forced_hccs():
+    c, s, e = G.config_from_inp(os.path.join(GOLD, "inp_forced_hccs"))
+    assert list(c.scheme) == [G.HCCS] * 3
+    assert (c.riemann, c.viscosity, c.forcing) == (G.AUSM, G.CONSTANT, G.FORCING_CONST)
+    assert (c.forcing_amp, c.forcing_amp_hel, c.forcing_k) == (0.5, 0.0, 2.0)
+    assert c.reynolds == 800.0
+    assert (c.nx, c.npx) == (512, 4)
+
+
+def test_inp_missing_directory_is_an_error():
+    with pytest.raises(G.NucError):
+        G.config_from_inp(os.path.join(GOLD, "does_not_exist"))
+
+
+def test_footprint_scales_with_cells_and_options():
+    a = G.footprint(G.make_config(n=(64, 64, 64)))
+    b = G.footprint(G.make_config(n=(128, 64, 64)))
+    assert 1.9 < b / a < 2.1
+    # 29 FP64 fields per cell for Navier-Stokes
+    assert abs(a / 64 ** 3 / 8 - 33) < 1.5
+    e = G.footprint(G.make_config(n=(64, 64, 64), model=G.EULER3D))
+    assert e < a
+    f = G.footprint(G.make_config(n=(64, 64, 64), forcing=G.FORCING_CONST, forcing_k=2.0))
+    assert f > a
+
+
+def test_server_reports_a_job_that_cannot_start_and_keeps_serving():
+    """A job whose configuration nuc_create rejects (grid not divisible by the
+    rank grid) is reported FAILED; the loop stays up and can be destroyed."""
+    bad = G.make_config(n=(18, 16, 16), npr=(4, 1, 1))
+    q = np.ones(5 * 18 * 16 * 16)
+    with G.Server(hbm_budget=1 << 30, max_active=2) as srv:
+        jid = srv.submit(bad, q, 3)
+        t0 = time.time()
+        res = None
+        while res is None and time.time() - t0 < 30:
+            res = srv.result(jid)
+            time.sleep(0.01)
+        assert res is not None
+        state, r = res
+        assert state is None and r.state == G.JOB_FAILED and r.error != 0
+        st = srv.stats()
+        assert st["failed"] == 1 and st["active"] == 0 and st["bytes_in_use"] == 0
+
+
+def test_server_rejects_job_above_budget():
+    cfg = G.make_config(n=(64, 64, 64))
+    with G.Server(hbm_budget=G.footprint(cfg) - 1, max_active=2) as srv:
+        with pytest.raises(G.NucError):
+            srv.submit(cfg, np.ones(5 * 64 ** 3), 1)
