@@ -2,12 +2,12 @@
 // per-stage right-hand side and TVD-RK3 advance.
 //
 // Per RK stage (FP64 path):
-//   k_prim   : con2prim of the stage input; at stage 0 the per-rank maximum
-//              eigenvalues (src/physicsModel.cpp:546-654, LF z-into-eta quirk kept)
+//   k_prim   : at stage 0 the per-rank maximum eigenvalues (src/physicsModel.cpp:546-654,
+//              LF z-into-eta quirk kept); with forcing also u,v,w,T of the stage input
 //   k_dt     : dt_local = min over ranks of getDt (src/euler3d.cpp:459-466), dt_global = cfl*dt_local
-//   k_grad   : (Navier-Stokes) gradients of u,v,w,T with cd6/cd8, Sutherland
-//              viscosity, stress and heat flux (src/NaiverStokes3d.cpp:87-303);
-//              writes the 9 distinct viscous flux components
+//   k_gradq  : (Navier-Stokes) con2prim, gradients of u,v,w,T with cd6/cd8,
+//              Sutherland viscosity, stress and heat flux (src/NaiverStokes3d.cpp:87-303)
+//              straight from Q; writes the 9 distinct viscous flux components
 //   k_sweep  : one kernel per direction.  Each workgroup holds LPB whole lines
 //              in LDS; per variable it splits the flux, reconstructs f^+ and
 //              f^- (explicit WENO or compact tridiagonal solved per wave),
@@ -44,7 +44,8 @@ struct SweepArgs {
     long nlines;
     double inv_h, gamma;
     const double *q[5];
-    const void *fv[4];      // viscous flux components, T of the sweep
+    const void *fv[4];      // viscous flux components (float when fv_f32, else double)
+    int fv_f32;
     double *rhs[5];
     int stage, epilogue;
     int accum;              // x sweep adds to rhs (forcing initialised it)
@@ -67,6 +68,7 @@ struct PrimArgs {
 };
 
 struct GradArgs {
+    const double *q[5];     // conserved state (k_gradq)
     const double *prim[4];
     void *out[9];           // 9 viscous flux components, T of the kernel
     int nx, ny, nz;
@@ -162,66 +164,126 @@ __global__ void k_dt(unsigned long long *lam, int P, double hx, double hy, doubl
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
-// T: stencil and flux type (double, or float for the mixed path)
-template <typename T>
-__global__ __launch_bounds__(256) void k_grad(GradArgs A)
+// Gradients and viscous fluxes straight from the conserved state, as a
+// plane-marching stencil: a workgroup owns a GTX x GTY column of cells and
+// walks GKZ planes along z.  x/y derivatives read a (GTX+8) x (GTY+8) tile of
+// the current plane's primitives in LDS (halos wrapped periodically); the z
+// derivative reads a 9-plane register queue of the thread's own column.  Each
+// state value is fetched from HBM about once (plus the z overlap of
+// neighbouring workgroups), instead of the 27 stencil reads per field of a
+// gather kernel, and no primitive field round-trips through HBM.
+constexpr int GTX = 64, GTY = 8, GKZ = 32;
+
+__device__ __forceinline__ void con2prim_uvwT(const double *const *q, long id, double gamma, double mach,
+                                              double &u, double &v, double &w, double &T)
 {
-    const int i = blockIdx.x * 64 + threadIdx.x;
-    const int j = blockIdx.y * 4 + threadIdx.y;
-    const int k = blockIdx.z;
-    if (i >= A.nx || j >= A.ny) return;
-    const long sx = 1, sy = A.nx, sz = (long)A.nx * A.ny;
-    const long id = i + sy * j + sz * k;
-    T g[4][3];
-    const int pos[3] = {i, j, k};
-    const int ext[3] = {A.nx, A.ny, A.nz};
-    const long str[3] = {sx, sy, sz};
+    // src/physicsModel.cpp:331-344, the same expression as k_prim
+    const double gm1 = gamma - 1.0;
+    double rho = q[0][id];
+    double ir = 1.0 / rho;
+    u = q[1][id] * ir; v = q[2][id] * ir; w = q[3][id] * ir;
+    double E = q[4][id];
+    double p = fabs((E - 0.5 * rho * (u * u + v * v + w * w)) * gm1);
+    T = gamma * mach * mach * p * ir;
+}
+
+template <typename T>
+__global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
+{
+    constexpr int PX = GTX + 8, PY = GTY + 8;
+    __shared__ T tile[4][PY][PX];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = tx + GTX * ty;
+    const int i0 = blockIdx.x * GTX, j0 = blockIdx.y * GTY, k0 = blockIdx.z * GKZ;
+    const int i = i0 + tx, j = j0 + ty;
+    const int nx = A.nx, ny = A.ny, nz = A.nz;
+    const long sy = nx, sz = (long)nx * ny;
+    const int iw = wrapi(i % nx, nx), jw = wrapi(j % ny, ny);
+    const bool own = i < nx && j < ny;
+    const double gamma = A.gamma, mach = A.mach;
+    // register queue: planes k-4 .. k+4 of u, v, w, T at (iw, jw)
+    T qz[4][9];
 #pragma unroll
-    for (int d = 0; d < 3; d++) {
-        long offs[9];
-#pragma unroll
-        for (int o = -4; o <= 4; o++) {
-            int p = wrapi(pos[d] + o, ext[d]);
-            offs[o + 4] = id + (long)(p - pos[d]) * str[d];
-        }
-        int cl = pos[d] % A.nloc[d];
-#pragma unroll
-        for (int f = 0; f < 4; f++) {
-            T L[9];
-#pragma unroll
-            for (int o = 0; o < 9; o++) L[o] = T(A.prim[f][offs[o]]);
-            g[f][d] = central_diff<T>(L + 4, 0, cl >= A.tsv[d] && cl < A.nloc[d] - A.tsv[d], A.vis[d], T(A.inv_h[d]));
-        }
+    for (int o = 0; o < 8; o++) {
+        int kk = wrapi((k0 - 4 + o) % nz, nz);
+        double u, v, w, Tt;
+        con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
+        qz[0][o + 1] = T(u); qz[1][o + 1] = T(v); qz[2][o + 1] = T(w); qz[3][o + 1] = T(Tt);
     }
-    T ux = g[0][0], uy = g[0][1], uz = g[0][2];
-    T vx = g[1][0], vy = g[1][1], vz = g[1][2];
-    T wx = g[2][0], wy = g[2][1], wz = g[2][2];
-    T tx = g[3][0], ty = g[3][1], tz = g[3][2];
-    T Tt = T(A.prim[3][id]);
-    T miu0;
-    T S = T(A.tref / A.tinf);
-    if (A.law == 0) miu0 = (T(1) + S) / (Tt + S) * (Tt * sqrt(Tt)) / T(A.re);   // src/physicsModel.cpp:220
-    else miu0 = T(1) / T(A.re);
-    T coeff0 = miu0 / T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
-    T div = ux + vy + wz;
-    // src/NaiverStokes3d.cpp:265-283 (off-diagonals carry -2/3 div u as coded)
-    T txx = miu0 * (T(2) * ux - T(2.0 / 3.0) * div);
-    T txy = miu0 * (uy + vx - T(2.0 / 3.0) * div);
-    T txz = miu0 * (uz + wx - T(2.0 / 3.0) * div);
-    T tyy = miu0 * (T(2) * vy - T(2.0 / 3.0) * div);
-    T tyz = miu0 * (vz + wy - T(2.0 / 3.0) * div);
-    T tzz = miu0 * (T(2) * wz - T(2.0 / 3.0) * div);
-    T u = T(A.prim[0][id]), v = T(A.prim[1][id]), w = T(A.prim[2][id]);
+    const int kend = min(k0 + GKZ, nz);
+    const T S = T(A.tref / A.tinf);
+    const T coeff_den = T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
+    const int cli = iw % A.nloc[0], clj = jw % A.nloc[1];
+    const bool inx = cli >= A.tsv[0] && cli < A.nloc[0] - A.tsv[0];
+    const bool iny = clj >= A.tsv[1] && clj < A.nloc[1] - A.tsv[1];
     T *const *out = reinterpret_cast<T *const *>(A.out);
-    out[0][id] = txx;
-    out[1][id] = tyy;
-    out[2][id] = tzz;
-    out[3][id] = txy;
-    out[4][id] = txz;
-    out[5][id] = tyz;
-    out[6][id] = u * txx + v * txy + w * txz + coeff0 * tx;
-    out[7][id] = u * txy + v * tyy + w * tyz + coeff0 * ty;
-    out[8][id] = u * txz + v * tyz + w * tzz + coeff0 * tz;
+    for (int k = k0; k < kend; k++) {
+        // advance the queue by one plane
+#pragma unroll
+        for (int f = 0; f < 4; f++)
+#pragma unroll
+            for (int o = 0; o < 8; o++) qz[f][o] = qz[f][o + 1];
+        {
+            int kk = wrapi((k + 4) % nz, nz);
+            double u, v, w, Tt;
+            con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
+            qz[0][8] = T(u); qz[1][8] = T(v); qz[2][8] = T(w); qz[3][8] = T(Tt);
+        }
+        // tile of plane k with wrapped x/y halos
+        for (int c = t; c < PX * PY; c += GTX * GTY) {
+            int cx = c % PX, cy = c / PX;
+            int gi = wrapi((i0 + cx - 4 + 4 * nx) % nx, nx), gj = wrapi((j0 + cy - 4 + 4 * ny) % ny, ny);
+            double u, v, w, Tt;
+            con2prim_uvwT(A.q, gi + sy * gj + sz * k, gamma, mach, u, v, w, Tt);
+            tile[0][cy][cx] = T(u); tile[1][cy][cx] = T(v); tile[2][cy][cx] = T(w); tile[3][cy][cx] = T(Tt);
+        }
+        __syncthreads();
+        if (own) {
+            const int clk = k % A.nloc[2];
+            const bool inz = clk >= A.tsv[2] && clk < A.nloc[2] - A.tsv[2];
+            T g[4][3];
+#pragma unroll
+            for (int f = 0; f < 4; f++) {
+                T Lx[9], Ly[9];
+#pragma unroll
+                for (int o = 0; o < 9; o++) {
+                    Lx[o] = tile[f][ty + 4][tx + o];
+                    Ly[o] = tile[f][ty + o][tx + 4];
+                }
+                g[f][0] = central_diff<T>(Lx + 4, 0, inx, A.vis[0], T(A.inv_h[0]));
+                g[f][1] = central_diff<T>(Ly + 4, 0, iny, A.vis[1], T(A.inv_h[1]));
+                g[f][2] = central_diff<T>(qz[f] + 4, 0, inz, A.vis[2], T(A.inv_h[2]));
+            }
+            T ux = g[0][0], uy = g[0][1], uz = g[0][2];
+            T vx = g[1][0], vy = g[1][1], vz = g[1][2];
+            T wx = g[2][0], wy = g[2][1], wz = g[2][2];
+            T tx_ = g[3][0], ty_ = g[3][1], tz_ = g[3][2];
+            T Tt = qz[3][4];
+            T miu0;
+            if (A.law == 0) miu0 = (T(1) + S) / (Tt + S) * (Tt * sqrt(Tt)) / T(A.re);   // src/physicsModel.cpp:220
+            else miu0 = T(1) / T(A.re);
+            T coeff0 = miu0 / coeff_den;
+            T div = ux + vy + wz;
+            // src/NaiverStokes3d.cpp:265-283 (off-diagonals carry -2/3 div u as coded)
+            T txx = miu0 * (T(2) * ux - T(2.0 / 3.0) * div);
+            T txy = miu0 * (uy + vx - T(2.0 / 3.0) * div);
+            T txz = miu0 * (uz + wx - T(2.0 / 3.0) * div);
+            T tyy = miu0 * (T(2) * vy - T(2.0 / 3.0) * div);
+            T tyz = miu0 * (vz + wy - T(2.0 / 3.0) * div);
+            T tzz = miu0 * (T(2) * wz - T(2.0 / 3.0) * div);
+            T u = qz[0][4], v = qz[1][4], w = qz[2][4];
+            const long id = i + sy * j + sz * k;
+            out[0][id] = txx;
+            out[1][id] = tyy;
+            out[2][id] = tzz;
+            out[3][id] = txy;
+            out[4][id] = txz;
+            out[5][id] = tyz;
+            out[6][id] = u * txx + v * txy + w * txz + coeff0 * tx_;
+            out[7][id] = u * txy + v * tyy + w * tyz + coeff0 * ty_;
+            out[8][id] = u * txz + v * tyz + w * tzz + coeff0 * tz_;
+        }
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -359,7 +421,9 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 
     // f^+ / f^- line buffers (skewed, f^- mirrored: Skew<R>::at / mir) and the
     // plain viscous-flux line
-    T *Pl = reinterpret_cast<T *>(sP) + ll * NP, *Ml = reinterpret_cast<T *>(sM) + ll * NP;
+    // a line's T view lives inside the bytes of its own double line, so that
+    // in the x sweep (wave-level sync only) no wave's writes reach another's line
+    T *Pl = reinterpret_cast<T *>(sP + ll * NP), *Ml = reinterpret_cast<T *>(sM + ll * NP);
     T *Vl = sV + ll * NV + HALO;
     double *Pw = sP + wave * NP, *Mw = sM + wave * NP;                 // interface values
     double *Pd = sP + ll * NP, *Md = sM + ll * NP;
@@ -370,7 +434,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     const bool neg = lane >= 32;
     const int chunk = neg ? 31 - h : h;
     const int m0 = chunk * R;
-    const T *Lw = reinterpret_cast<const T *>(neg ? sM : sP) + wave * NP + h * (R + 1);
+    const T *Lw = reinterpret_cast<const T *>((neg ? sM : sP) + wave * NP) + h * (R + 1);
     const int m0r = neg ? m0 + R - 1 : m0;
     const int mstep = neg ? -1 : 1;
     double res = 0.0;
@@ -392,7 +456,8 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 if (c < HALO) { Pl[Skew<R>::at(c + n)] = fp; Ml[Skew<R>::mir(c + n)] = fm; }
                 if (c >= n - HALO) { Pl[Skew<R>::at(c - n)] = fp; Ml[Skew<R>::mir(c - n)] = fm; }
                 if (A.vis != VIS_NONE && v > 0) {
-                    T x = static_cast<const T *>(A.fv[v - 1])[id];
+                    T x = A.fv_f32 ? T(static_cast<const float *>(A.fv[v - 1])[id])
+                                   : T(static_cast<const double *>(A.fv[v - 1])[id]);
                     Vl[c] = x;
                     if (c < HALO) Vl[c + n] = x;
                     if (c >= n - HALO) Vl[c - n] = x;
@@ -683,10 +748,17 @@ template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
-    if (A.mixed)
-        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
-    else
-        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+    // The 7-point schemes keep FP64 stencils in the mixed path too: their
+    // smoothness indicators cancel terms up to 17246 f^2, so FP32 inputs alone
+    // move the state by ~1e-4 in a few steps.
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS);
+    if constexpr (fp32_ok) {
+        if (A.mixed) {
+            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
 }
 
 template <int SCH, int RIE>
@@ -832,8 +904,9 @@ static int validate(const nuc_config *c, std::string &err)
 extern "C" long long nuc_footprint(const nuc_config *c)
 {
     long long N = (long long)c->nx * c->ny * c->nz;
-    long long doubles = 15 * N + 5 * N + (c->model == NUC_NAVIERSTOKES3D ? 13 * N : 0);
-    if (c->model != NUC_NAVIERSTOKES3D && c->forcing == NUC_FORCING_CONST) doubles += 4 * N;
+    // 3 states + rhs; 9 viscous flux fields (Navier-Stokes); u,v,w,T (forcing)
+    long long doubles = 15 * N + 5 * N + (c->model == NUC_NAVIERSTOKES3D ? 9 * N : 0);
+    if (c->forcing == NUC_FORCING_CONST) doubles += 4 * N;
     long long fb = (c->forcing == NUC_FORCING_CONST) ? nucforce::plan_bytes(*c) : 0;
     return doubles * 8 + (N / 32 + 4096) * 8 + 4096 + fb;
 }
@@ -887,7 +960,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     if (!dalloc(&x->rhs, 5 * N * sizeof(double))) return fail("device allocation (rhs)");
     x->forcing = (cfg->forcing == NUC_FORCING_CONST);
     x->mixed = (cfg->precision == NUC_MIXED);
-    if (cfg->model == NUC_NAVIERSTOKES3D || x->forcing)
+    if (x->forcing)
         if (!dalloc(&x->prim, 4 * N * sizeof(double))) return fail("device allocation (prim)");
     if (cfg->model == NUC_NAVIERSTOKES3D)
         if (!dalloc(&x->fv, 9 * N * sizeof(double))) return fail("device allocation (fv)");
@@ -961,7 +1034,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     {
         PrimArgs P{};
         for (int v = 0; v < 5; v++) P.q[v] = qi + v * N;
-        const bool wp = ns || x->forcing;
+        const bool wp = x->forcing;   // the gradient kernel reads Q itself
         if (wp) for (int f = 0; f < 4; f++) P.prim[f] = x->prim + f * N;
         P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
         P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2];
@@ -982,7 +1055,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     }
     if (ns) {
         GradArgs G{};
-        for (int f = 0; f < 4; f++) G.prim[f] = x->prim + f * N;
+        for (int v = 0; v < 5; v++) G.q[v] = qi + v * N;
         for (int o = 0; o < 9; o++) G.out[o] = x->fv + o * N;
         G.nx = x->n[0]; G.ny = x->n[1]; G.nz = x->n[2];
         for (int d = 0; d < 3; d++) {
@@ -992,10 +1065,10 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         }
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
         G.tref = c.t_ref; G.tinf = c.t_inf;
-        dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
+        dim3 grid((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + GKZ - 1) / GKZ);
         ProfScope ps(x, 2);
-        if (x->mixed) hipLaunchKernelGGL(k_grad<float>, grid, dim3(64, 4, 1), 0, x->stream, G);
-        else hipLaunchKernelGGL(k_grad<double>, grid, dim3(64, 4, 1), 0, x->stream, G);
+        if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+        else hipLaunchKernelGGL(k_gradq<double>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
     }
     if (x->forcing) {
         // Const forcing (src/physicsModel.cpp:1368-1677) initialises rhs; the
@@ -1021,6 +1094,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         S.stage = stage;
         S.accum = (d == 0 && x->forcing) ? 1 : 0;
         S.mixed = x->mixed ? 1 : 0;
+        S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
         S.fmean = (d == 2 && x->forcing) ? x->scal + 4 : nullptr;
         S.epilogue = (d == 2 && epilogue) ? 1 : 0;
         if (S.epilogue) {

@@ -87,7 +87,7 @@ def test_footprint_scales_with_cells_and_options():
     b = G.footprint(G.make_config(n=(128, 64, 64)))
     assert 1.9 < b / a < 2.1
     # 29 FP64 fields per cell for Navier-Stokes
-    assert abs(a / 64 ** 3 / 8 - 33) < 1.5
+    assert abs(a / 64 ** 3 / 8 - 29) < 1.5
     e = G.footprint(G.make_config(n=(64, 64, 64), model=G.EULER3D))
     assert e < a
     f = G.footprint(G.make_config(n=(64, 64, 64), forcing=G.FORCING_CONST, forcing_k=2.0))
