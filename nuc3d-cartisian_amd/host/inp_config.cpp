@@ -28,7 +28,8 @@ static bool read_pairs(const std::string &path, std::vector<std::pair<std::strin
 
 int scheme_from_name(const std::string &s, int *halo)
 {
-    // src/fieldOperator.cpp:130-176; an unknown name falls back to weno5js
+    // src/fieldOperator.cpp:130-176; an unknown name would select weno5js there but
+    // leave its halo width unset (:177-180), so it is rejected here (-1)
     if (s == "weno7js") { *halo = 4; return NUC_WENO7JS; }
     if (s == "hccs") { *halo = 4; return NUC_HCCS; }
     if (s == "weno5js") { *halo = 3; return NUC_WENO5JS; }
