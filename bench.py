@@ -75,6 +75,20 @@ def cpu_baseline(q, n, threads):
                       f"{t1 - t0:.1f} s"}
 
 
+def measured_traffic(kernel, n):
+    """HBM bytes per launch of `kernel` at n^3 from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE and WRITE_SIZE scaled by a device-copy calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_crweno5_256_fp64_traffic.md")
+    if n != 256 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("machine-readable: "):
+            rec = json.loads(line[len("machine-readable: "):])
+            if rec.get("kernel") == kernel:
+                return float(rec["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
 def stage_stats(G, cfg, q, steps, warmup):
     """(seconds for `steps` timed steps, stats_ex after warmup+steps, profile)"""
     with G.Context(cfg) as ctx:
@@ -201,9 +215,13 @@ def main():
     else:
         alg_bytes = B_ALG_STAGE * cells
     achieved = alg_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = measured_traffic(name, n)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": name,
-                "avg_launch_ms": round(avg_ms, 4)}
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": name,
+                "avg_launch_ms": round(avg_ms, 4), "alg_bytes": alg_bytes}
+    if traffic:
+        roofline["traffic_source"] = traffic_src
+        roofline["traffic_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
     stage_ms = ms_per_step / 3.0
     if args.profile_slots:
         for k, (t, c) in prof.items():

@@ -26,6 +26,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 using namespace nucdev;
@@ -466,7 +467,10 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
         }
         sync_lines(d);
         double out[R];
-        if (wv) line_half<SCH, R, T, double>(Lw, m0r, mstep, z, h, neg, out);
+        // weights and rows in T where FP32 keeps them in range; WENO5-Z squares
+        // tau/(is+1e-40) and stays FP64
+        using TW = typename std::conditional<SCH == WENO5Z, double, T>::type;
+        if (wv) line_half<SCH, R, T, TW, double>(Lw, m0r, mstep, z, h, neg, out);
         sync_lines(d);
         if (wv) {
             // interface m = m0r + mstep*r goes to slot at(m) of the f^+ buffer
@@ -625,14 +629,21 @@ __global__ __launch_bounds__(256) void k_stats(StatArgs A)
     }
 }
 
-__global__ void k_stats_reduce(const double *part, long nb, double *out)
+// one workgroup per statistic: strided partial sums in a fixed order, then a
+// fixed-shape tree (deterministic run to run)
+__global__ __launch_bounds__(256) void k_stats_reduce(const double *part, long nb, double *out)
 {
-    // one thread per statistic: fixed sequential order over blocks
-    int s = threadIdx.x;
-    if (s >= NSTAT) return;
+    __shared__ double sm[256];
+    const int s = blockIdx.x, t = threadIdx.x;
     double acc = 0.0;
-    for (long b = 0; b < nb; b++) acc += part[b * NSTAT + s];
-    out[s] = acc;
+    for (long b = t; b < nb; b += 256) acc += part[b * NSTAT + s];
+    sm[t] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) sm[t] += sm[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[s] = sm[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -672,7 +683,7 @@ __global__ __launch_bounds__(64) void k_op_recon(OpRecArgs A)
     const int mstep = neg ? -1 : 1;
     const Zone z{n + 1, n, A.ts};
     double out[R];
-    line_half<SCH, R, double, double>((neg ? sLm : L) + h * (R + 1), m0r, mstep, z, h, neg, out);
+    line_half<SCH, R, double, double, double>((neg ? sLm : L) + h * (R + 1), m0r, mstep, z, h, neg, out);
     if (neg != (A.upwind < 0)) return;
     for (int r = 0; r < R; r++) {
         int m = m0r + mstep * r;
@@ -1273,7 +1284,7 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
     }
     S.gamma = c.gamma; S.mach = c.mach; S.tref = c.t_ref; S.tinf = c.t_inf; S.part = x->stat_part;
     hipLaunchKernelGGL(k_stats, grid, dim3(64, 4, 1), 0, x->stream, S);
-    hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(64), 0, x->stream, x->stat_part, nb, x->stat_out);
+    hipLaunchKernelGGL(k_stats_reduce, dim3(NSTAT), dim3(256), 0, x->stream, x->stat_part, nb, x->stat_out);
     NUC_CHECK(x, hipGetLastError());
     double s[NSTAT];
     NUC_CHECK(x, hipMemcpyAsync(s, x->stat_out, sizeof(s), hipMemcpyDeviceToHost, x->stream));

@@ -37,6 +37,23 @@ __device__ __forceinline__ double frcp(double x)
     e = fma(-x, r, 1.0);
     return fma(r, e, r);
 }
+// FP32 (mixed path): the hardware reciprocal, 1 ulp
+__device__ __forceinline__ float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// Unnormalised WENO5 weights n_k ~ c_k / (eps + is_k)^2.  FP64 uses the
+// division-free products c_k * s_i * s_j (one reciprocal per row in total);
+// FP32 uses c_k * rcp(s_k), since the products can leave FP32's range.
+template <typename TW>
+__device__ __forceinline__ void w5_weights(TW b0, TW b1, TW b2, TW c0, TW c1, TW c2, TW &n0, TW &n1, TW &n2)
+{
+    const TW eps = TW(1.0e-6);
+    TW s0 = sq(eps + b0), s1 = sq(eps + b1), s2 = sq(eps + b2);
+    if constexpr (sizeof(TW) == 8) {
+        n0 = c0 * (s1 * s2); n1 = c1 * (s0 * s2); n2 = c2 * (s0 * s1);
+    } else {
+        n0 = c0 * frcp(s0); n1 = c1 * frcp(s1); n2 = c2 * frcp(s2);
+    }
+}
 
 template <int SCH> struct Radius { static constexpr int value = (SCH == WENO7JS || SCH == HCCS) ? 3 : 2; };
 
@@ -159,11 +176,11 @@ __device__ __forceinline__ void w5_parts(const T *f, T &b0, T &b1, T &b2, T &q0,
 // upwind coordinates.  m0r is the interface index of row 0 and mstep its step
 // (+1 for f^+, -1 for f^- whose rows run backwards).
 // T is the type of the stencils (smoothness indicators, candidate values);
-// TW the type of the nonlinear weights, the compact rows, their tridiagonal
-// solve and the result (FP64 in both precision paths).
-template <int SCH, int R, typename T, typename TW>
+// TW the type of the nonlinear weights and compact rows; TS the type of the
+// tridiagonal solve and of the result (FP64 in both precision paths).
+template <int SCH, int R, typename T, typename TW, typename TS>
 __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const Zone &z, int h, bool neg,
-                                          TW out[R])
+                                          TS out[R])
 {
     constexpr int RAD = Radius<SCH>::value;
     constexpr int W = R + 2 * RAD;
@@ -188,11 +205,10 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         for (int r = 0; r < R; r++) {
             T b0, b1, b2, q0, q1, q2;
             w5_parts<T>(win + r, b0, b1, b2, q0, q1, q2);
-            const TW eps = TW(1.0e-6);
-            TW s0 = sq(eps + TW(b0)), s1 = sq(eps + TW(b1)), s2 = sq(eps + TW(b2));
-            // alpha_k = c_k / s_k; normalised with a single reciprocal
-            TW n0 = TW(0.1) * (s1 * s2), n1 = TW(0.6) * (s0 * s2), n2 = TW(0.3) * (s0 * s1);
-            out[r] = (n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2);
+            // alpha_k = c_k / s_k, normalised with a single reciprocal
+            TW n0, n1, n2;
+            w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(0.1), TW(0.6), TW(0.3), n0, n1, n2);
+            out[r] = TS((n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2));
         }
     } else if constexpr (SCH == WENO5Z) {
 #pragma unroll
@@ -209,21 +225,20 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TW a0 = TW(0.1) * (TW(1) + sq(tau / (B0 + eps)));
             TW a1 = TW(0.6) * (TW(1) + sq(tau / (B1 + eps)));
             TW a2 = TW(0.3) * (TW(1) + sq(tau / (B2 + eps)));
-            out[r] = (a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2);
+            out[r] = TS((a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2));
         }
     } else if constexpr (SCH == WENO7JS) {
 #pragma unroll
-        for (int r = 0; r < R; r++) out[r] = weno7z<T, TW>(win + r, TW(1.0e-20));
+        for (int r = 0; r < R; r++) out[r] = TS(weno7z<T, TW>(win + r, TW(1.0e-20)));
     } else if constexpr (SCH == CRWENO5) {
-        Chunk<R, TW> ch;
+        Chunk<R, TS> ch;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const T *f = win + r;
             T b0, b1, b2, q0, q1, q2;
             w5_parts<T>(f, b0, b1, b2, q0, q1, q2);
-            const TW eps = TW(1.0e-6);
-            TW s0 = sq(eps + TW(b0)), s1 = sq(eps + TW(b1)), s2 = sq(eps + TW(b2));
-            TW P0 = s1 * s2, P1 = s0 * s2, P2 = s0 * s1;
+            TW P0, P1, P2;   // weights with unit ideal weights
+            w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(1), TW(1), TW(1), P0, P1, P2);
             // explicit WENO5-JS closure value (src/crweno5.cpp:87-129) from the same indicators
             TW n0 = TW(0.1) * P0, n1 = TW(0.6) * P1, n2 = TW(0.3) * P2;
             TW wexp = (n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2);
@@ -238,17 +253,17 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             int m = m0r + mstep * r;
             bool valid = m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
-            TW a = (valid && !expl) ? lo : TW(0);
-            TW b = (valid && !expl) ? di : TW(1);
-            TW c = (valid && !expl) ? hi : TW(0);
-            TW d = valid ? (expl ? wexp : rhs) : TW(0);
+            TS a = (valid && !expl) ? TS(lo) : TS(0);
+            TS b = (valid && !expl) ? TS(di) : TS(1);
+            TS c = (valid && !expl) ? TS(hi) : TS(0);
+            TS d = valid ? TS(expl ? wexp : rhs) : TS(0);
             ch.row(r, a, b, c, d);
         }
         ch.solve(h);
 #pragma unroll
         for (int r = 0; r < R; r++) out[r] = ch.Df[r];
     } else {  // HCCS
-        Chunk<R, TW> ch;
+        Chunk<R, TS> ch;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const T *f = win + r;   // 7-point window for the closures, 5-point from f+1
@@ -263,7 +278,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                     hccs_row<T, TW>(f[1], f[2], f[3], f[4], f[5], a, b, c, d);
                 }
             }
-            ch.row(r, a, b, c, d);
+            ch.row(r, TS(a), TS(b), TS(c), TS(d));
         }
         ch.solve(h);
 #pragma unroll
