@@ -374,7 +374,8 @@ __device__ __forceinline__ void sync_lines(int d)
 // T: type of the split fluxes, line buffers and stencils (double, or float
 // for the mixed path); weights, tridiagonal solves, interface values, the
 // flux difference and the RHS/RK3 accumulation are FP64 in both.
-template <int SCH, int RIE, int R, int LPB, typename T>
+// FULL: n = 32R and every workgroup's lines exist, so no bounds checks.
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
     constexpr int NP = Skew<R>::pitch(32 * R);
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     else { ll = t % LPB; g = t / LPB; }
     const long blk = xcd_block(blockIdx.x, gridDim.x);
     const long L = blk * LPB + ll;
-    const bool lv = L < A.nlines;
+    const bool lv = FULL || L < A.nlines;
     long base = 0, stride = 1;
     if (d == 0) { base = L * A.nx; stride = 1; }
     else if (d == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
         int c = g + 64 * k;
-        bool ok = lv && c < n;
+        bool ok = FULL || (lv && c < n);
         long id = base + (long)c * stride;
         T q[5];
 #pragma unroll
@@ -428,7 +429,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     T *Vl = sV + ll * NV + HALO;
     double *Pw = sP + wave * NP, *Mw = sM + wave * NP;                 // interface values
     double *Pd = sP + ll * NP, *Md = sM + ll * NP;
-    const bool wv = blk * LPB + wave < A.nlines;
+    const bool wv = FULL || blk * LPB + wave < A.nlines;
     const Zone z{n, A.nloc, A.ts};
     // compute-mapping geometry of this lane (nuc_lines.h)
     const int h = lane & 31;
@@ -441,36 +442,55 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     double res = 0.0;
     const double dt = A.epilogue ? *A.dtp : 0.0;
 
-#pragma unroll 1
-    for (int v = 0; v < 5; v++) {
-        double qv[CPL];
+    // global loads of variable v (state and viscous flux) are issued one
+    // variable ahead, so that they are in flight during the previous
+    // variable's reconstruction
+    double qv[CPL];
+    T xv[CPL];
+    auto fetch = [&](int v, double qo[CPL], T xo[CPL]) {
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
-            qv[k] = 0.0;
-            if (lv && c < n) {
+            qo[k] = 0.0;
+            xo[k] = T(0);
+            if (FULL || (lv && c < n)) {
                 long id = base + (long)c * stride;
-                qv[k] = A.q[v][id];
+                qo[k] = A.q[v][id];
+                if (A.vis != VIS_NONE && v > 0)
+                    xo[k] = A.fv_f32 ? T(static_cast<const float *>(A.fv[v - 1])[id])
+                                     : T(static_cast<const double *>(A.fv[v - 1])[id]);
+            }
+        }
+    };
+    fetch(0, qv, xv);
+#pragma unroll 1
+    for (int v = 0; v < 5; v++) {
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
+            if (FULL || (lv && c < n)) {
                 T fp, fm;
                 split_var<RIE, T>(sp[k], T(qv[k]), v, d, fp, fm);
+                // the cell and its periodic image: cells within HALO of an end
+                // are copied past the other end, all others to a dummy slot
+                const int cw = c < HALO ? c + n : (c >= n - HALO ? c - n : 32 * R + HALO);
                 Pl[Skew<R>::at(c)] = fp; Ml[Skew<R>::mir(c)] = fm;
-                if (c < HALO) { Pl[Skew<R>::at(c + n)] = fp; Ml[Skew<R>::mir(c + n)] = fm; }
-                if (c >= n - HALO) { Pl[Skew<R>::at(c - n)] = fp; Ml[Skew<R>::mir(c - n)] = fm; }
+                Pl[Skew<R>::at(cw)] = fp; Ml[Skew<R>::mir(cw)] = fm;
                 if (A.vis != VIS_NONE && v > 0) {
-                    T x = A.fv_f32 ? T(static_cast<const float *>(A.fv[v - 1])[id])
-                                   : T(static_cast<const double *>(A.fv[v - 1])[id]);
-                    Vl[c] = x;
-                    if (c < HALO) Vl[c + n] = x;
-                    if (c >= n - HALO) Vl[c - n] = x;
+                    Vl[c] = xv[k];
+                    Vl[cw] = xv[k];
                 }
             }
         }
+        double qnx[CPL];
+        T xnx[CPL];
+        if (v < 4) fetch(v + 1, qnx, xnx);
         sync_lines(d);
         double out[R];
         // weights and rows in T where FP32 keeps them in range; WENO5-Z squares
         // tau/(is+1e-40) and stays FP64
         using TW = typename std::conditional<SCH == WENO5Z, double, T>::type;
-        if (wv) line_half<SCH, R, T, TW, double>(Lw, m0r, mstep, z, h, neg, out);
+        if (wv) line_half<SCH, R, T, TW, double>(Lw, m0r, mstep, z, h, neg, out, FULL);
         sync_lines(d);
         if (wv) {
             // interface m = m0r + mstep*r goes to slot at(m) of the f^+ buffer
@@ -479,11 +499,11 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
             // for 0 < r < R
             double *Ob = (neg ? Mw : Pw) + h * (R + 1);
             double *Ow = Ob - (neg ? 1 : 0);
-            if (m0r < n) Ob[neg ? Skew<R>::at(-1) : Skew<R>::at(0)] = out[0];
+            if (FULL || m0r < n) Ob[neg ? Skew<R>::at(-1) : Skew<R>::at(0)] = out[0];
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 int m = m0r + mstep * r;
-                if (m < n) Ow[Skew<R>::at(r)] = out[r];
+                if (FULL || m < n) Ow[Skew<R>::at(r)] = out[r];
             }
             // periodic copy of interface 0 to interface n
             if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
@@ -493,7 +513,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
-            if (lv && c < n) {
+            if (FULL || (lv && c < n)) {
                 long id = base + (long)c * stride;
                 double r = (d > 0 || A.accum) ? A.rhs[v][id] : 0.0;
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
@@ -520,6 +540,10 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
             }
         }
         sync_lines(d);
+        if (v < 4) {
+#pragma unroll
+            for (int k = 0; k < CPL; k++) { qv[k] = qnx[k]; xv[k] = xnx[k]; }
+        }
     }
     if (A.epilogue && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
@@ -763,13 +787,17 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
     // smoothness indicators cancel terms up to 17246 f^2, so FP32 inputs alone
     // move the state by ~1e-4 in a few steps.
     constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS);
+    const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
+    const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
     if constexpr (fp32_ok) {
         if (A.mixed) {
-            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+            if (full) hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, A);
+            else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, A);
             return;
         }
     }
-    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double>), dim3((unsigned)nb), dim3(64 * SWEEP_LPB), 0, s, A);
+    if (full) hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false>), grid, block, 0, s, A);
 }
 
 template <int SCH, int RIE>

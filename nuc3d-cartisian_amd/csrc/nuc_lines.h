@@ -71,14 +71,17 @@ template <int R> struct Skew {
     static constexpr int B = ((HALO + 1 + R - 1) / R) * R;
     __host__ __device__ static constexpr int at(int y) { return (y + B) + (y + B) / R; }
     __device__ __forceinline__ static int mir(int x) { return at(32 * R - 2 - x); }
-    static constexpr int span(int ncap) { return at(ncap + HALO - 1) + 1; }
+    // slots up to cell ncap+HALO: the last one is a write-only dummy (halo
+    // writes of interior cells go there instead of branching)
+    static constexpr int span(int ncap) { return at(ncap + HALO) + 1; }
     static constexpr int pitch(int ncap) { return span(ncap) + ((2 - span(ncap) % 16) + 16) % 16; }
 };
 
-// pitch of an unskewed line of ncap cells with HALO ghosts on each side, = 2 (mod 16)
+// pitch of an unskewed line of ncap cells with HALO ghosts on each side and
+// one dummy slot, = 2 (mod 16)
 __host__ __device__ constexpr int plain_pitch(int ncap)
 {
-    return (ncap + 2 * HALO) + ((2 - (ncap + 2 * HALO) % 16) + 16) % 16;
+    return (ncap + 2 * HALO + 1) + ((2 - (ncap + 2 * HALO + 1) % 16) + 16) % 16;
 }
 
 // normalised parallel cyclic reduction over the 32 lanes of a half wave:
@@ -178,9 +181,10 @@ __device__ __forceinline__ void w5_parts(const T *f, T &b0, T &b1, T &b2, T &q0,
 // T is the type of the stencils (smoothness indicators, candidate values);
 // TW the type of the nonlinear weights and compact rows; TS the type of the
 // tridiagonal solve and of the result (FP64 in both precision paths).
+// full: every row of every lane is an interface of the line (n = 32R).
 template <int SCH, int R, typename T, typename TW, typename TS>
 __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const Zone &z, int h, bool neg,
-                                          TS out[R])
+                                          TS out[R], bool full = false)
 {
     constexpr int RAD = Radius<SCH>::value;
     constexpr int W = R + 2 * RAD;
@@ -251,7 +255,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TW rhs = (m0 * TW(f[1]) + (TW(5) * (m0 + m1) + m2) * TW(f[2]) + (m1 + TW(5) * m2) * TW(f[3]))
                      * TW(1.0 / 6.0);
             int m = m0r + mstep * r;
-            bool valid = m < z.nrows;
+            bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
             TS a = (valid && !expl) ? TS(lo) : TS(0);
             TS b = (valid && !expl) ? TS(di) : TS(1);
@@ -268,7 +272,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         for (int r = 0; r < R; r++) {
             const T *f = win + r;   // 7-point window for the closures, 5-point from f+1
             int m = m0r + mstep * r;
-            bool valid = m < z.nrows;
+            bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
             TW a = TW(0), b = TW(1), c = TW(0), d = TW(0);
             if (valid) {
