@@ -360,15 +360,17 @@ __device__ __forceinline__ long xcd_block(unsigned p, unsigned nb)
 // Line-buffer synchronisation: in the x sweep every wave loads and computes
 // its own line, so a wave-level barrier suffices (LDS operations of one wave
 // complete in order); the y/z load mapping spreads a line over all waves.
+__device__ __forceinline__ void sync_wave()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ __forceinline__ void sync_lines(int d)
 {
-    if (d == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
+    if (d == 0) sync_wave();
+    else __syncthreads();
 }
 
 // T: type of the split fluxes, line buffers and stencils (double, or float
@@ -491,7 +493,9 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
         // tau/(is+1e-40) and stays FP64
         using TW = typename std::conditional<SCH == WENO5Z, double, T>::type;
         if (wv) line_half<SCH, R, T, TW, double>(Lw, m0r, mstep, z, h, neg, out, FULL);
-        sync_lines(d);
+        // the interface values overwrite the wave's own line only: the
+        // reconstruction of the other waves is not waited for
+        sync_wave();
         if (wv) {
             // interface m = m0r + mstep*r goes to slot at(m) of the f^+ buffer
             // or mir(m) of the f^- buffer; relative to the lane base
