@@ -72,7 +72,7 @@ struct GradArgs {
     const double *q[5];     // conserved state (k_gradq)
     const double *prim[4];
     void *out[9];           // 9 viscous flux components, T of the kernel
-    int nx, ny, nz;
+    int nx, ny, nz, gkz;
     int nloc[3], tsv[3], vis[3];
     double inv_h[3];
     int law;
@@ -167,13 +167,16 @@ __device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >
 
 // Gradients and viscous fluxes straight from the conserved state, as a
 // plane-marching stencil: a workgroup owns a GTX x GTY column of cells and
-// walks GKZ planes along z.  x/y derivatives read a (GTX+8) x (GTY+8) tile of
+// walks A.gkz planes along z.  x/y derivatives read a (GTX+8) x (GTY+8) tile of
 // the current plane's primitives in LDS (halos wrapped periodically); the z
 // derivative reads a 9-plane register queue of the thread's own column.  Each
 // state value is fetched from HBM about once (plus the z overlap of
 // neighbouring workgroups), instead of the 27 stencil reads per field of a
 // gather kernel, and no primitive field round-trips through HBM.
-constexpr int GTX = 64, GTY = 8, GKZ = 32;
+constexpr int GTX = 32, GTY = 16;
+// planes per workgroup: long marches amortise the 8-plane z halo on large
+// boxes; short ones give small boxes enough workgroups
+inline int grad_planes(int nz) { return nz >= 256 ? 32 : (nz >= 128 ? 16 : (nz >= 64 ? 8 : 4)); }
 
 __device__ __forceinline__ void con2prim_uvwT(const double *const *q, long id, double gamma, double mach,
                                               double &u, double &v, double &w, double &T)
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
     constexpr int PX = GTX + 8, PY = GTY + 8;
     __shared__ T tile[4][PY][PX];
     const int tx = threadIdx.x, ty = threadIdx.y, t = tx + GTX * ty;
-    const int i0 = blockIdx.x * GTX, j0 = blockIdx.y * GTY, k0 = blockIdx.z * GKZ;
+    const int i0 = blockIdx.x * GTX, j0 = blockIdx.y * GTY, k0 = blockIdx.z * A.gkz;
     const int i = i0 + tx, j = j0 + ty;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const long sy = nx, sz = (long)nx * ny;
@@ -210,7 +213,7 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
         con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
         qz[0][o + 1] = T(u); qz[1][o + 1] = T(v); qz[2][o + 1] = T(w); qz[3][o + 1] = T(Tt);
     }
-    const int kend = min(k0 + GKZ, nz);
+    const int kend = min(k0 + A.gkz, nz);
     const T S = T(A.tref / A.tinf);
     const T coeff_den = T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
     const int cli = iw % A.nloc[0], clj = jw % A.nloc[1];
@@ -1108,7 +1111,8 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         }
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
         G.tref = c.t_ref; G.tinf = c.t_inf;
-        dim3 grid((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + GKZ - 1) / GKZ);
+        G.gkz = grad_planes(x->n[2]);
+        dim3 grid((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + G.gkz - 1) / G.gkz);
         ProfScope ps(x, 2);
         if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
         else hipLaunchKernelGGL(k_gradq<double>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
