@@ -516,13 +516,28 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
             if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
             if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
         }
+        // the final phase's own global reads (rhs of the previous sweeps, u^n of
+        // the RK update) are issued before the barrier, so that their latency
+        // overlaps the wait; the reconstruction's registers are free by now
+        double rin[CPL], unv[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
+            rin[k] = 0.0;
+            unv[k] = 0.0;
+            if (FULL || (lv && c < n)) {
+                long id = base + (long)c * stride;
+                if (d > 0 || A.accum) rin[k] = A.rhs[v][id];
+                if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
+            }
+        }
         sync_lines(d);
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
             if (FULL || (lv && c < n)) {
                 long id = base + (long)c * stride;
-                double r = (d > 0 || A.accum) ? A.rhs[v][id] : 0.0;
+                double r = rin[k];
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
                 r += ((Pd[Skew<R>::at(c + 1)] + Md[Skew<R>::mir(c + 1)]) - (Pd[Skew<R>::at(c)] + Md[Skew<R>::mir(c)]))
                      * A.inv_h;
@@ -537,7 +552,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                     if (A.stage == 0) {
                         nu = qv[k] + rr * dt;
                     } else {
-                        double un = A.qn[v][id];
+                        double un = unv[k];
                         if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + qv[k]);
                         else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qv[k]);
                         if (A.stage == 2) res += sq(nu - un);
