@@ -85,6 +85,24 @@ __host__ __device__ constexpr int plain_pitch(int ncap)
     return (ncap + 2 * HALO + 1) + ((4 - (ncap + 2 * HALO + 1) % 16) + 16) % 16;
 }
 
+// Distance-1 lane shifts as DPP moves (no LDS round trip): shr1 gives lane i
+// the value of lane i-1, shl1 the value of lane i+1; the lanes without a source
+// (0, resp. 63) read 0.  Across the boundary of the two half waves the values
+// do move (lane 32 reads lane 31), so callers mask h == 0 / h == 31 as they
+// do for the width-32 shuffles.
+__device__ __forceinline__ int dpp_shr1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x138, 0xF, 0xF, true); }
+__device__ __forceinline__ int dpp_shl1(int x) { return __builtin_amdgcn_update_dpp(0, x, 0x130, 0xF, 0xF, true); }
+__device__ __forceinline__ double shr1(double x)
+{
+    return __hiloint2double(dpp_shr1(__double2hiint(x)), dpp_shr1(__double2loint(x)));
+}
+__device__ __forceinline__ double shl1(double x)
+{
+    return __hiloint2double(dpp_shl1(__double2hiint(x)), dpp_shl1(__double2loint(x)));
+}
+__device__ __forceinline__ float shr1(float x) { return __int_as_float(dpp_shr1(__float_as_int(x))); }
+__device__ __forceinline__ float shl1(float x) { return __int_as_float(dpp_shl1(__float_as_int(x))); }
+
 // normalised parallel cyclic reduction over the 32 lanes of a half wave:
 // row k: x_k + al*x_{k-1} + ga*x_{k+1} = de  ->  x_k = de on return
 template <typename T>
@@ -92,8 +110,14 @@ __device__ __forceinline__ void pcr32(T &al, T &ga, T &de, int h)
 {
 #pragma unroll
     for (int s = 1; s < 32; s <<= 1) {
-        T alm = __shfl_up(al, s, 32), gam = __shfl_up(ga, s, 32), dem = __shfl_up(de, s, 32);
-        T alp = __shfl_down(al, s, 32), gap = __shfl_down(ga, s, 32), dep = __shfl_down(de, s, 32);
+        T alm, gam, dem, alp, gap, dep;
+        if (s == 1) {
+            alm = shr1(al); gam = shr1(ga); dem = shr1(de);
+            alp = shl1(al); gap = shl1(ga); dep = shl1(de);
+        } else {
+            alm = __shfl_up(al, s, 32); gam = __shfl_up(ga, s, 32); dem = __shfl_up(de, s, 32);
+            alp = __shfl_down(al, s, 32); gap = __shfl_down(ga, s, 32); dep = __shfl_down(de, s, 32);
+        }
         if (h < s) { alm = T(0); gam = T(0); dem = T(0); }
         if (h + s >= 32) { alp = T(0); gap = T(0); dep = T(0); }
         T r = frcp(T(1) - al * gam - ga * alp);
@@ -143,7 +167,7 @@ template <int R, typename T> struct Chunk {
             D0 = r0 * (Df[0] - Cf[0] * Df[1]);
         }
         T A1 = Af[R - 1], C1 = Cf[R - 1], D1 = Df[R - 1];
-        T A0n = __shfl_down(A0, 1, 32), C0n = __shfl_down(C0, 1, 32), D0n = __shfl_down(D0, 1, 32);
+        T A0n = shl1(A0), C0n = shl1(C0), D0n = shl1(D0);
         if (h == 31) { A0n = T(0); C0n = T(0); D0n = T(0); }
         T rb = frcp(T(1) - A1 * C0 - C1 * A0n);
         T al = -rb * A1 * A0;
@@ -151,7 +175,7 @@ template <int R, typename T> struct Chunk {
         T de = rb * (D1 - A1 * D0 - C1 * D0n);
         pcr32(al, ga, de, h);
         T X1 = de;
-        T X1m = __shfl_up(X1, 1, 32);
+        T X1m = shr1(X1);
         if (h == 0) X1m = T(0);
         T X0 = D0 - A0 * X1m - C0 * X1;
 #pragma unroll
