@@ -103,6 +103,32 @@ __device__ __forceinline__ double shl1(double x)
 __device__ __forceinline__ float shr1(float x) { return __int_as_float(dpp_shr1(__float_as_int(x))); }
 __device__ __forceinline__ float shl1(float x) { return __int_as_float(dpp_shl1(__float_as_int(x))); }
 
+// Distance-16 exchange inside each half wave with one v_permlane16_swap per
+// dword: swapping a value with itself leaves, in the first result, the lanes of
+// the odd 16-lane rows holding the value of the lane 16 below (lane i-16), and,
+// in the second, the lanes of the even rows holding the value of the lane 16
+// above (lane i+16).  up16 is meaningful for h >= 16, dn16 for h < 16 (h = lane
+// index within the half wave); the other lanes are masked by the callers.
+__device__ __forceinline__ void swap16(unsigned x, unsigned &up, unsigned &dn)
+{
+    auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    up = r[0]; dn = r[1];
+}
+__device__ __forceinline__ void swap16(double x, double &up, double &dn)
+{
+    unsigned hu, hd, lu, ld;
+    swap16((unsigned)__double2hiint(x), hu, hd);
+    swap16((unsigned)__double2loint(x), lu, ld);
+    up = __hiloint2double((int)hu, (int)lu);
+    dn = __hiloint2double((int)hd, (int)ld);
+}
+__device__ __forceinline__ void swap16(float x, float &up, float &dn)
+{
+    unsigned u, d;
+    swap16((unsigned)__float_as_int(x), u, d);
+    up = __int_as_float((int)u); dn = __int_as_float((int)d);
+}
+
 // normalised parallel cyclic reduction over the 32 lanes of a half wave:
 // row k: x_k + al*x_{k-1} + ga*x_{k+1} = de  ->  x_k = de on return
 template <typename T>
@@ -114,6 +140,8 @@ __device__ __forceinline__ void pcr32(T &al, T &ga, T &de, int h)
         if (s == 1) {
             alm = shr1(al); gam = shr1(ga); dem = shr1(de);
             alp = shl1(al); gap = shl1(ga); dep = shl1(de);
+        } else if (s == 16) {
+            swap16(al, alm, alp); swap16(ga, gam, gap); swap16(de, dem, dep);
         } else {
             alm = __shfl_up(al, s, 32); gam = __shfl_up(ga, s, 32); dem = __shfl_up(de, s, 32);
             alp = __shfl_down(al, s, 32); gap = __shfl_down(ga, s, 32); dep = __shfl_down(de, s, 32);
