@@ -153,6 +153,16 @@ int nuc_config_from_inp(const char *dir, nuc_config *cfg, int *start_step, int *
  *   rescaled to the given u_rms; rho = 1, p = 1/(gamma Mach^2). */
 int nuc_init_tgv(const nuc_config *cfg, double *q);
 int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsigned long long seed, double *q);
+/* nuc_init_shu_osher: Mach-3 shock / entropy-wave profile along x, measured
+ *   from the box centre: (rho, u, p) = (3.857143, 2.629369, 10.33333) for
+ *   x < -4, (1 + 0.2 sin 5x, 0, 1) elsewhere, plus a solenoidal velocity
+ *   perturbation of rms pert * 2.629369 (nuc_init_hit's generator, k0 = 4,
+ *   seed).  No reference counterpart: the reference's initial conditions
+ *   (src/block.cpp:285-551) have no shock case, and its boundaries are periodic,
+ *   so the seam at x = +-lx/2 is a second discontinuity.  With Mach =
+ *   gamma^-1/2 the temperature T = gamma M^2 p / rho of src/physicsModel.cpp:342
+ *   equals p / rho. */
+int nuc_init_shu_osher(const nuc_config *cfg, double pert, unsigned long long seed, double *q);
 
 #ifdef __cplusplus
 }

@@ -202,8 +202,9 @@ void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vecto
 typedef std::complex<double> cplx;
 
 // Seeded random-phase solenoidal field.  The spectrum
-// E(k) = 16 sqrt(2/pi) (k/k0)^4 / k0 exp(-2 (k/k0)^2) (src/block.cpp:458) is
-// populated for |k_x|,|k_y|,|k_z| <= K = min(n/2-1, 4 k0) -- beyond 4 k0 it is
+// E(k) = 16 sqrt(2/pi) (k/k0)^4 / k0 exp(-2 (k/k0)^2) (src/block.cpp:458), k the
+// physical wave number, is populated for mode numbers
+// |k_x|,|k_y|,|k_z| <= K = min(n/2-1, 4 k0) -- beyond 4 k0 it is
 // below 1e-13 of its peak -- and synthesised by a separable inverse DFT with
 // FFTW's backward sign (e^{+i}) and no normalisation, keeping the real part
 // (src/block.cpp:504-510).  Phases come from mt19937_64(seed), three draws per
@@ -224,7 +225,9 @@ void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long lon
     for (int kz = -K[2]; kz <= K[2]; kz++)
         for (int ky = -K[1]; ky <= K[1]; ky++)
             for (int kx = -K[0]; kx <= K[0]; kx++) {
-                double kx0 = kx, ky0 = ky, kz0 = kz;
+                // physical wave numbers 2 pi k / L, so that the field is
+                // solenoidal on any box (the factor is exactly 1 on [0, 2 pi)^3)
+                double kx0 = kx * (2.0 * pie / c.lx), ky0 = ky * (2.0 * pie / c.ly), kz0 = kz * (2.0 * pie / c.lz);
                 double kmag = std::sqrt(kx0 * kx0 + ky0 * ky0 + kz0 * kz0);
                 double k12 = std::sqrt(kx0 * kx0 + ky0 * ky0);
                 if (kmag < 1e-6) kmag = 1.0;
@@ -308,6 +311,41 @@ void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long lon
     }
 }
 
+// Shu-Osher shock / entropy-wave profile along x (the reference has no such
+// initial condition; its boundaries are periodic only, so the box also holds
+// the seam at x = +-lx/2 as a second discontinuity).  x is measured from the
+// box centre: (rho, u, p) = (3.857143, 2.629369, 10.33333) for x < -4, else
+// (1 + 0.2 sin 5x, 0, 1).  A solenoidal velocity perturbation from initial_hit
+// (k0 = 4, seeded) with rms pert * 2.629369 is added to all three components.
+// With Mach = gamma^-1/2 the solver's temperature T = gamma M^2 p / rho equals
+// p / rho.
+void initial_shu_osher(const nuc_config &c, double pert, unsigned long long seed, std::vector<double> &q)
+{
+    const long N = (long)c.nx * c.ny * c.nz;
+    const double uL = 2.629369, rhoL = 3.857143, pL = 10.33333;
+    std::vector<double> v;
+    initial_hit(c, 4.0, pert * uL, seed, v);
+    q.assign(5 * N, 0.0);
+    const int nl = c.nx / c.npx;
+    const double dx = c.lx / c.nx, g = c.gamma;
+    for (int k = 0; k < c.nz; k++)
+        for (int j = 0; j < c.ny; j++)
+            for (int i = 0; i < c.nx; i++) {
+                // cell centres per rank as initial_taylorgreen, shifted to the box centre
+                const double x = c.lx / c.npx * (i / nl) + dx * (0.5 + i % nl) - 0.5 * c.lx;
+                const long id = i + (long)c.nx * (j + (long)c.ny * k);
+                double rho, u, p;
+                if (x < -4.0) { rho = rhoL; u = uL; p = pL; }
+                else { rho = 1.0 + 0.2 * std::sin(5.0 * x); u = 0.0; p = 1.0; }
+                const double uu = u + v[N + id], vv = v[2 * N + id], ww = v[3 * N + id];
+                q[id] = rho;
+                q[N + id] = rho * uu;
+                q[2 * N + id] = rho * vv;
+                q[3 * N + id] = rho * ww;
+                q[4 * N + id] = p / (g - 1.0) + 0.5 * rho * (uu * uu + vv * vv + ww * ww);
+            }
+}
+
 bool write_restart(const std::string &path, double time, const std::vector<double> &q)
 {
     std::ofstream f(path, std::ios::binary);
@@ -344,6 +382,16 @@ extern "C" int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsi
     if (!cfg || !q || !(k0 > 0) || !(u_rms >= 0)) return NUC_EINVAL;
     std::vector<double> v;
     nuc3d::initial_hit(*cfg, k0, u_rms, seed, v);
+    std::memcpy(q, v.data(), v.size() * sizeof(double));
+    return NUC_OK;
+}
+
+extern "C" int nuc_init_shu_osher(const nuc_config *cfg, double pert, unsigned long long seed, double *q)
+{
+    if (!cfg || !q || !(pert >= 0)) return NUC_EINVAL;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1 || cfg->npx < 1 || cfg->nx % cfg->npx != 0) return NUC_EINVAL;
+    std::vector<double> v;
+    nuc3d::initial_shu_osher(*cfg, pert, seed, v);
     std::memcpy(q, v.data(), v.size() * sizeof(double));
     return NUC_OK;
 }

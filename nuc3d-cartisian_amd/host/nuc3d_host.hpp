@@ -175,6 +175,8 @@ void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vecto
 // Seeded random-phase solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2)
 // (src/block.cpp:397-551 with a fixed seed); rho = 1, p = 1/(gamma M^2)
 void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long long seed, std::vector<double> &q);
+// Shu-Osher profile along x with a seeded solenoidal velocity perturbation
+void initial_shu_osher(const nuc_config &c, double pert, unsigned long long seed, std::vector<double> &q);
 // restart file: [double time][5 fields x-fastest] (src/block.cpp:609-671)
 bool write_restart(const std::string &path, double time, const std::vector<double> &q);
 bool read_restart(const std::string &path, double &time, std::vector<double> &q);

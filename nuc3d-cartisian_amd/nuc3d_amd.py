@@ -32,7 +32,7 @@ JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
            "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_set_graph", "nuc_query", "nuc_sync",
            "nuc_stats", "nuc_stats_ex", "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp",
-           "nuc_init_tgv", "nuc_init_hit", "nuc_profile", "nuc_profile_read",
+           "nuc_init_tgv", "nuc_init_hit", "nuc_init_shu_osher", "nuc_profile", "nuc_profile_read",
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
            "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error"]
 
@@ -115,6 +115,7 @@ def lib():
         L.nuc_init_tgv.argtypes = [ctypes.POINTER(NucConfig), _dp]
         L.nuc_init_hit.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_double,
                                    ctypes.c_ulonglong, _dp]
+        L.nuc_init_shu_osher.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_ulonglong, _dp]
         L.nuc_profile.argtypes = [vp, ctypes.c_int]
         L.nuc_profile_read.argtypes = [vp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_long),
                                        ctypes.POINTER(ctypes.c_char_p)]
@@ -189,6 +190,14 @@ def init_tgv(cfg):
 def init_hit(cfg, k0=4.0, u_rms=1.0, seed=20261202):
     q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
     _check(lib().nuc_init_hit(ctypes.byref(cfg), float(k0), float(u_rms), int(seed), _ptr(q)), "nuc_init_hit")
+    return q
+
+
+def init_shu_osher(cfg, pert=0.05, seed=20261602):
+    """Shu-Osher profile along x (box centred on x = 0) with a seeded solenoidal
+    velocity perturbation of rms pert * 2.629369 (nuc_init_shu_osher)."""
+    q = np.empty(5 * cfg.nx * cfg.ny * cfg.nz)
+    _check(lib().nuc_init_shu_osher(ctypes.byref(cfg), float(pert), int(seed), _ptr(q)), "nuc_init_shu_osher")
     return q
 
 

@@ -239,3 +239,62 @@ def test_forced_rhs_and_ten_steps_match_oracle(case):
     err = _rel_linf(q_gpu, q_ref, N)
     print("forced 10-step rel Linf", err, flush=True)
     assert np.all(err <= 1e-12), err
+
+
+# Long lines: every chunk length R (lane h holds R consecutive interfaces,
+# R in {2, 4, 6, 8, 16} from n up to 64, 128, 192, 256, 512) in every sweep
+# direction, both with n = 32 R (the kernels without tail checks) and with a
+# ragged tail, with closure rows inside the line (rank grids).  Thin boxes keep
+# the oracle's work small; the short sides are the minimum the halos allow.
+LONG_CASES = [
+    # n, rank grid, scheme, vis, riemann, model
+    ((256, 12, 12), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 256, 12), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 12, 256), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((128, 12, 10), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((10, 12, 160), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((12, 192, 12), (1, 1, 1), G.WENO5Z, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((512, 10, 12), (1, 1, 1), G.CRWENO5, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 10, 480), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((100, 12, 12), (2, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 12, 384), (1, 1, 3), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((256, 12, 12), (1, 1, 1), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 160, 12), (1, 1, 1), G.WENO7JS, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 12, 512), (1, 1, 1), G.WENO5Z, G.CD6, G.LF, G.EULER3D),
+]
+
+
+@pytest.mark.parametrize("case", LONG_CASES, ids=lambda c: f"{c[0]}-{c[1]}-s{c[2]}-v{c[3]}-r{c[4]}-m{c[5]}")
+def test_long_lines_ten_steps_match_oracle(case):
+    n, npr, scheme, vis, rie, model = case
+    print(f"long-line case {case}", flush=True)
+    cfl = 0.5
+    if rie == G.LF and n[2] > min(n[0], n[1]):
+        # LF's dt omits the z wave speed (the z eigenvalue lands in the eta
+        # slot, src/physicsModel.cpp:653-654), so a z line much finer than x/y
+        # needs a smaller CFL number to stay stable
+        cfl = 0.4 * min(n[0], n[1]) / n[2]
+    gc, oc = _cfgs(n, npr, scheme, vis, rie, model, cfl=cfl)
+    q = _perturbed_tgv(oc, 5)
+    N = oc.nx * oc.ny * oc.nz
+    s = O.Solver(oc)
+    s.set_state(q)
+    dt_ref, res_ref, t_ref = s.step(10)
+    q_ref = s.get_state()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        dt_gpu, res_gpu, t_gpu = ctx.step(10)
+        q_gpu = ctx.download()
+    err = _rel_linf(q_gpu, q_ref, N)
+    print("10-step rel Linf per variable", err, flush=True)
+    # Thin boxes leave the short directions under-resolved (12 cells per
+    # period), where the nonlinear weights sit far from their ideal values and
+    # the compact rows lose diagonal dominance: the reference algorithm itself
+    # then moves by up to ~2e-11 after 10 steps when its input changes by one
+    # ulp (measured here with the oracle).  Every case is held to
+    # max(1e-12, 3 x that measured floor); dt and the residual follow the state.
+    tol = np.maximum(1e-12, 3.0 * _ulp_floor(oc, q, 10))
+    print("max(1e-12, 1-ulp floor x3)", tol, flush=True)
+    assert np.all(err <= tol), (err, tol)
+    assert abs(dt_gpu - dt_ref) <= max(1e-13, tol.max()) * dt_ref
+    assert abs(res_gpu - res_ref) <= max(1e-9, 100 * tol.max()) * abs(res_ref)

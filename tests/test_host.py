@@ -118,3 +118,34 @@ def test_server_rejects_job_above_budget():
     with G.Server(hbm_budget=G.footprint(cfg) - 1, max_active=2) as srv:
         with pytest.raises(G.NucError):
             srv.submit(cfg, np.ones(5 * 64 ** 3), 1)
+
+
+def test_shu_osher_initial_condition():
+    """nuc_init_shu_osher: the two states on either side of x = -4 (box centred
+    on 0), a divergence-free perturbation of the requested rms, and the seed
+    fixing the field."""
+    nx, ny, nz = 64, 16, 16
+    cfg = G.make_config(n=(nx, ny, nz), L=(10.0, 2.5, 2.5), mach=1.0 / np.sqrt(1.4))
+    q0 = G.init_shu_osher(cfg, pert=0.0).reshape(5, nz, ny, nx)
+    x = (np.arange(nx) + 0.5) * 10.0 / nx - 5.0
+    left = x < -4.0
+    assert np.allclose(q0[0][..., left], 3.857143) and np.allclose(q0[1][..., left], 3.857143 * 2.629369)
+    assert np.allclose(q0[0][..., ~left], 1.0 + 0.2 * np.sin(5.0 * x[~left]))
+    assert np.all(q0[1][..., ~left] == 0) and np.all(q0[2] == 0) and np.all(q0[3] == 0)
+    assert np.allclose(q0[4][..., ~left], 1.0 / 0.4)
+    assert np.allclose(q0[4][..., left], 10.33333 / 0.4 + 0.5 * 3.857143 * 2.629369 ** 2)
+    q = G.init_shu_osher(cfg, pert=0.05, seed=7).reshape(5, nz, ny, nx)
+    assert np.array_equal(q, G.init_shu_osher(cfg, pert=0.05, seed=7).reshape(5, nz, ny, nx))
+    assert not np.array_equal(q, G.init_shu_osher(cfg, pert=0.05, seed=8).reshape(5, nz, ny, nx))
+    assert np.array_equal(q[0], q0[0])
+    du = [q[1 + c] / q[0] - q0[1 + c] / q0[0] for c in range(3)]
+    rms = np.sqrt(np.mean(du[0] ** 2 + du[1] ** 2 + du[2] ** 2) / 3.0)
+    assert abs(rms - 0.05 * 2.629369) < 1e-12
+    # spectral divergence of the perturbation vanishes (arrays stored z, y, x)
+    L = (2.5, 2.5, 10.0)
+    k = np.meshgrid(*[2 * np.pi * np.fft.fftfreq(n, d=Ld / n) for n, Ld in zip((nz, ny, nx), L)], indexing="ij")
+    div = k[2] * np.fft.fftn(du[0]) + k[1] * np.fft.fftn(du[1]) + k[0] * np.fft.fftn(du[2])
+    mag = np.sqrt(sum(np.abs(k[a] * np.fft.fftn(du[2 - a])) ** 2 for a in range(3)))
+    assert np.abs(div).max() <= 1e-10 * mag.max()
+    with pytest.raises(G.NucError):
+        G.init_shu_osher(cfg, pert=-1.0)
