@@ -114,6 +114,12 @@ STAT_NAMES = ["ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "om_x", "om_
               "tau_t", "skewness", "eps_s", "eps_d"]
 
 
+# compatible jobs packed into one launch up to 65536 cells per pack (pairs of
+# 32^3 boxes): packing beats concurrent streams only for such tiny boxes
+SERVE_PACK_CELLS = 1 << 16
+SERVE_MAX_ACTIVE = 16
+
+
 def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
     import numpy as np
     rng = np.random.default_rng(seed)
@@ -135,10 +141,10 @@ def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
     return jobs, arrivals
 
 
-def run_serving(G, jobs, arrivals=None):
+def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=SERVE_MAX_ACTIVE, stats=None):
     """All jobs through one server; arrivals None = all at once. Returns (wall s, latencies s)."""
     import numpy as np
-    with G.Server(hbm_budget=96 << 30, max_active=16) as srv:
+    with G.Server(hbm_budget=96 << 30, max_active=max_active, pack_cells=pack_cells) as srv:
         ids = []
         t0 = time.perf_counter()
         for k, (cfg, q, _) in enumerate(jobs):
@@ -158,6 +164,8 @@ def run_serving(G, jobs, arrivals=None):
                 raise RuntimeError(f"serving job {jid} failed: state {r[1].state} error {r[1].error}")
             lat.append(r[1].done_s - r[1].submit_s)
         wall = time.perf_counter() - t0
+        if stats is not None:
+            stats.update(srv.stats())
     return wall, np.array(lat)
 
 
@@ -248,7 +256,8 @@ def main():
         print("bench: serving", file=sys.stderr, flush=True)
         jobs, arrivals = serving_stream(G, args.serve_jobs, 32, 96, SERVE_STEPS, 20.0, 20261202 + 500)
         run_serving(G, jobs[:4])                      # warm the kernels and the allocator
-        wall_sat, _ = run_serving(G, jobs)
+        sst = {}
+        wall_sat, _ = run_serving(G, jobs, stats=sst)
         wall_seq = run_sequential(G, jobs)
         wall_poi, lat = run_serving(G, jobs, arrivals)
         serving = {"jobs": len(jobs), "steps_per_job": SERVE_STEPS, "grid_n": [32, 96],
@@ -258,7 +267,9 @@ def main():
                    "poisson_rate_per_s": 20.0,
                    "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
                    "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-                   "poisson_wall_s": round(wall_poi, 3)}
+                   "poisson_wall_s": round(wall_poi, 3),
+                   "max_active": SERVE_MAX_ACTIVE, "pack_cells": SERVE_PACK_CELLS,
+                   "single_slices": int(sst.get("single_slices", 0)), "pack_slices": int(sst.get("pack_slices", 0))}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -102,6 +102,28 @@ int nuc_set_graph(nuc_ctx *ctx, int enable);
 /* NUC_OK when all enqueued work of ctx has finished, NUC_EBUSY otherwise; never blocks. */
 int nuc_query(nuc_ctx *ctx);
 int nuc_sync(nuc_ctx *ctx, nuc_step_info *info);
+
+/* ---- packs: several small boxes in one launch per kernel -----------------
+ * No counterpart in the reference (one block per MPI job, src/block.cpp:191-215);
+ * for serving many small independent boxes.  A pack advances every member
+ * context by whole RK3 steps with one launch of each kernel for all members:
+ * workgroup b of a packed grid runs member j's work for its own workgroup
+ * b - off[j], so each member's result is bit-identical to nuc_step on that
+ * member alone.  Members must be compatible (nuc_pack_compatible: same
+ * schemes, splitting, model, viscous schemes, precision and line-length
+ * class per direction, no forcing, lines of at most 128 cells); they keep
+ * their own state, dt, residual and time.  The pack works on its own stream,
+ * ordered after each member's earlier work and before its later work, so
+ * nuc_sync/nuc_query/nuc_download_state on a member see the pack's steps.
+ * use_graph: replay one captured hipGraph of a packed step.  Destroy a pack
+ * before any of its members; 1 to 64 members. */
+typedef struct nuc_pack nuc_pack;
+int nuc_pack_compatible(const nuc_ctx *a, const nuc_ctx *b);
+int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nuc_pack **out);
+int nuc_pack_step_async(nuc_pack *pack, int nsteps);
+int nuc_pack_destroy(nuc_pack *pack);
+/* the message of the last failure of pack (or of nuc_pack_create when pack is NULL) */
+const char *nuc_pack_last_error(const nuc_pack *pack);
 /* Global statistics of postproc::solveQ (src/postproc.cpp:736-929), FP64:
  * out[0..15] means of ux,uy,uz,vx,vy,vz,wx,wy,wz, omega_x,y,z, enstrophy, Q,
  * helicity, kinetic energy; out[16..24] u_rms, M_t, sqrt(<(du_i/dx_i)^2>/3),

@@ -14,6 +14,9 @@
  * HBM budget (nuc_footprint of the job) are free, keeps every admitted job's
  * work in flight on its own stream as hipGraph replays of whole RK3 steps, and
  * retires finished jobs, so jobs that arrive while others run join at once.
+ * Small compatible jobs that are idle together are advanced as one pack (one
+ * launch per kernel for all of them, nuc_pack_*), which leaves each result
+ * unchanged.
  * Calls are thread-safe.  Errors of one job never stop the loop.
  */
 #ifndef NUC3D_SERVE_H
@@ -28,6 +31,8 @@ typedef struct nuc_server_config {
     int max_active;         /* jobs in flight at once (each has its own stream) */
     int steps_per_slice;    /* RK3 steps enqueued per job per scheduling turn (0: tuned per job) */
     int use_graphs;         /* replay captured hipGraph steps (1) or launch kernels (0) */
+    long long pack_cells;   /* idle compatible jobs (nuc_pack_compatible) of together at most this
+                               many cells are advanced as one pack (nuc_pack_*); 0: no packing */
 } nuc_server_config;
 
 enum nuc_job_state { NUC_JOB_QUEUED = 0, NUC_JOB_RUNNING = 1, NUC_JOB_DONE = 2, NUC_JOB_FAILED = 3 };
@@ -44,6 +49,8 @@ typedef struct nuc_job_result {
 typedef struct nuc_server_stats {
     long long completed, failed, queued, active, peak_active;
     long long bytes_in_use, peak_bytes;
+    long long single_slices;   /* slices enqueued for one job on its own stream */
+    long long pack_slices;     /* slices enqueued for a pack of jobs */
 } nuc_server_stats;
 
 typedef struct nuc_server nuc_server;

@@ -38,40 +38,61 @@ constexpr int NUC_PROF_SLOTS = 8;
 
 // ---------------------------------------------------------------------------
 // kernel arguments
+//
+// Buffer pointers of the hot-path kernels are declared in the global address
+// space for the device (GMEM): the same bytes reach a kernel either as its
+// argument or read from a device table (packed launches), and a pointer read
+// from memory is otherwise a generic address to the compiler, accessed with
+// flat instructions that also hold the LDS wait counter.  The host pass sees
+// plain pointers of the same size.
 // ---------------------------------------------------------------------------
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GMEM __attribute__((address_space(1)))
+#else
+#define GMEM
+#endif
+
+// a buffer pointer as stored in the argument structs (an explicit cast for
+// the device pass, the identity for the host pass)
+template <typename T>
+__host__ __device__ inline T GMEM *gp(T *p)
+{
+    return (T GMEM *)p;
+}
+
 struct SweepArgs {
     int dir, n, nloc, ts, tsv, vis, riemann;
     int nx, ny, nz;
     long nlines;
     double inv_h, gamma;
-    const double *q[5];
-    const void *fv[4];      // viscous flux components (float when fv_f32, else double)
+    const double GMEM *q[5];
+    const void GMEM *fv[4];   // viscous flux components (float when fv_f32, else double)
     int fv_f32;
-    double *rhs[5];
+    double GMEM *rhs[5];
     int stage, epilogue;
-    int accum;              // x sweep adds to rhs (forcing initialised it)
-    int mixed;              // FP32 stencils (host-side dispatch only)
-    const double *fmean;    // z sweep: <f.u> added back to the energy RHS
-    const double *qn[5];
-    double *qout[5];
-    const double *dtp;
-    double *res_part;
+    int accum;                // x sweep adds to rhs (forcing initialised it)
+    int mixed;                // FP32 stencils (host-side dispatch only)
+    const double GMEM *fmean; // z sweep: <f.u> added back to the energy RHS
+    const double GMEM *qn[5];
+    double GMEM *qout[5];
+    const double GMEM *dtp;
+    double GMEM *res_part;
 };
 
 struct PrimArgs {
-    const double *q[5];
-    double *prim[4];
+    const double GMEM *q[5];
+    double GMEM *prim[4];
     long N;
     int nx, ny, nlx, nly, nlz, npx, npy, P;
     int riemann, do_lam, write_prim;
     double gamma, mach;
-    unsigned long long *lam;
+    unsigned long long GMEM *lam;
 };
 
 struct GradArgs {
-    const double *q[5];     // conserved state (k_gradq)
-    const double *prim[4];
-    void *out[9];           // 9 viscous flux components, T of the kernel
+    const double GMEM *q[5];  // conserved state (k_gradq)
+    const double GMEM *prim[4];
+    void GMEM *out[9];        // 9 viscous flux components, T of the kernel
     int nx, ny, nz, gkz;
     int nloc[3], tsv[3], vis[3];
     double inv_h[3];
@@ -85,7 +106,9 @@ __device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigne
 // ---------------------------------------------------------------------------
 // con2prim (src/physicsModel.cpp:331-344) and eigenvalue maxima
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_prim(PrimArgs A)
+// bid/nbk: this workgroup's index and the workgroup count of the job's
+// grid-stride loop
+__device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
 {
     __shared__ unsigned long long sl[3 * MAXRANK];
     const int t = threadIdx.x;
@@ -96,7 +119,7 @@ __global__ __launch_bounds__(256) void k_prim(PrimArgs A)
     int cur = -1;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     const double gm1 = A.gamma - 1.0;
-    for (long id = (long)blockIdx.x * 256 + t; id < A.N; id += (long)gridDim.x * 256) {
+    for (long id = bid * 256 + t; id < A.N; id += nbk * 256) {
         double rho = A.q[0][id];
         double ir = 1.0 / rho;
         double u = A.q[1][id] * ir, v = A.q[2][id] * ir, w = A.q[3][id] * ir;
@@ -136,29 +159,39 @@ __global__ __launch_bounds__(256) void k_prim(PrimArgs A)
         }
         __syncthreads();
         for (int i = t; i < 3 * A.P; i += 256)
-            if (sl[i]) atomicMax(&A.lam[i], sl[i]);
+            if (sl[i]) atomicMax((unsigned long long *)&A.lam[i], sl[i]);
     }
 }
 
+__global__ __launch_bounds__(256) void k_prim(PrimArgs A) { prim_body(A, blockIdx.x, gridDim.x); }
+
 // dt per rank and min over ranks; resets the eigenvalue slots for the next step
-__global__ void k_dt(unsigned long long *lam, int P, double hx, double hy, double hz, double cfl, int set_global,
-                     double *scal)
+struct DtArgs {
+    unsigned long long GMEM *lam;
+    int P, set_global;
+    double hx, hy, hz, cfl;
+    double GMEM *scal;
+};
+
+__device__ __forceinline__ void dt_body(const DtArgs &A)
 {
     if (threadIdx.x == 0) {
         double dtmin = 0.0;
-        for (int r = 0; r < P; r++) {
-            double dt_xi = hx / bits2d(lam[3 * r + 0]);
-            double dt_eta = hy / bits2d(lam[3 * r + 1]);
-            double dt_zeta = hz / bits2d(lam[3 * r + 2]);
+        for (int r = 0; r < A.P; r++) {
+            double dt_xi = A.hx / bits2d(A.lam[3 * r + 0]);
+            double dt_eta = A.hy / bits2d(A.lam[3 * r + 1]);
+            double dt_zeta = A.hz / bits2d(A.lam[3 * r + 2]);
             double dt = 1.0 / (1.0 / dt_xi + 1.0 / dt_eta + 1.0 / dt_zeta);
             if (r == 0 || dt < dtmin) dtmin = dt;
         }
-        scal[3] = dtmin;
-        if (set_global) scal[0] = dtmin * cfl;
+        A.scal[3] = dtmin;
+        if (A.set_global) A.scal[0] = dtmin * A.cfl;
     }
     __syncthreads();
-    for (int i = threadIdx.x; i < 3 * P; i += blockDim.x) lam[i] = 0ull;
+    for (int i = threadIdx.x; i < 3 * A.P; i += blockDim.x) A.lam[i] = 0ull;
 }
+
+__global__ void k_dt(DtArgs A) { dt_body(A); }
 
 // ---------------------------------------------------------------------------
 // gradients + viscous fluxes
@@ -178,7 +211,7 @@ constexpr int GTX = 32, GTY = 16;
 // boxes; short ones give small boxes enough workgroups
 inline int grad_planes(int nz) { return nz >= 256 ? 32 : (nz >= 128 ? 16 : (nz >= 64 ? 8 : 4)); }
 
-__device__ __forceinline__ void con2prim_uvwT(const double *const *q, long id, double gamma, double mach,
+__device__ __forceinline__ void con2prim_uvwT(const double GMEM *const *q, long id, double gamma, double mach,
                                               double &u, double &v, double &w, double &T)
 {
     // src/physicsModel.cpp:331-344, the same expression as k_prim
@@ -192,12 +225,12 @@ __device__ __forceinline__ void con2prim_uvwT(const double *const *q, long id, d
 }
 
 template <typename T>
-__global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
+__device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, int bz)
 {
     constexpr int PX = GTX + 8, PY = GTY + 8;
     __shared__ T tile[4][PY][PX];
     const int tx = threadIdx.x, ty = threadIdx.y, t = tx + GTX * ty;
-    const int i0 = blockIdx.x * GTX, j0 = blockIdx.y * GTY, k0 = blockIdx.z * A.gkz;
+    const int i0 = bx * GTX, j0 = by * GTY, k0 = bz * A.gkz;
     const int i = i0 + tx, j = j0 + ty;
     const int nx = A.nx, ny = A.ny, nz = A.nz;
     const long sy = nx, sz = (long)nx * ny;
@@ -219,7 +252,7 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
     const int cli = iw % A.nloc[0], clj = jw % A.nloc[1];
     const bool inx = cli >= A.tsv[0] && cli < A.nloc[0] - A.tsv[0];
     const bool iny = clj >= A.tsv[1] && clj < A.nloc[1] - A.tsv[1];
-    T *const *out = reinterpret_cast<T *const *>(A.out);
+    T GMEM *const *out = reinterpret_cast<T GMEM *const *>(A.out);
     for (int k = k0; k < kend; k++) {
         // advance the queue by one plane
 #pragma unroll
@@ -289,6 +322,9 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A)
         __syncthreads();
     }
 }
+
+template <typename T>
+__global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A) { gradq_body<T>(A, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ---------------------------------------------------------------------------
 // the direction sweep
@@ -380,8 +416,9 @@ __device__ __forceinline__ void sync_lines(int d)
 // for the mixed path); weights, tridiagonal solves, interface values, the
 // flux difference and the RHS/RK3 accumulation are FP64 in both.
 // FULL: n = 32R and every workgroup's lines exist, so no bounds checks.
+// blk: the workgroup's logical index within its job (line group blk*LPB ..)
 template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
-__global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
+__device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 {
     constexpr int NP = Skew<R>::pitch(32 * R);
     constexpr int NV = plain_pitch(32 * R);
@@ -399,7 +436,6 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     int ll, g;
     if (d == 0) { ll = wave; g = lane; }
     else { ll = t % LPB; g = t / LPB; }
-    const long blk = xcd_block(blockIdx.x, gridDim.x);
     const long L = blk * LPB + ll;
     const bool lv = FULL || L < A.nlines;
     long base = 0, stride = 1;
@@ -462,8 +498,8 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
                 long id = base + (long)c * stride;
                 qo[k] = A.q[v][id];
                 if (A.vis != VIS_NONE && v > 0)
-                    xo[k] = A.fv_f32 ? T(static_cast<const float *>(A.fv[v - 1])[id])
-                                     : T(static_cast<const double *>(A.fv[v - 1])[id]);
+                    xo[k] = A.fv_f32 ? T(static_cast<const float GMEM *>(A.fv[v - 1])[id])
+                                     : T(static_cast<const double GMEM *>(A.fv[v - 1])[id]);
             }
         }
     };
@@ -581,8 +617,25 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
     }
 }
 
-__global__ void k_res(const double *part, int nb, double inv_count, double *scal)
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
+__global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
+    sweep_body<SCH, RIE, R, LPB, T, FULL>(A, xcd_block(blockIdx.x, gridDim.x));
+}
+
+struct ResArgs {
+    const double GMEM *part;
+    int nb;
+    double inv_count;
+    double GMEM *scal;
+};
+
+__device__ __forceinline__ void res_body(const ResArgs &A)
+{
+    const double GMEM *part = A.part;
+    const int nb = A.nb;
+    double GMEM *scal = A.scal;
+    const double inv_count = A.inv_count;
     __shared__ double sm[256];
     double acc = 0.0;
     for (int i = threadIdx.x; i < nb; i += 256) acc += part[i];
@@ -597,6 +650,8 @@ __global__ void k_res(const double *part, int nb, double inv_count, double *scal
         scal[2] += scal[0];  // time += dt (src/block.cpp:208-209)
     }
 }
+
+__global__ void k_res(ResArgs A) { res_body(A); }
 
 // ---------------------------------------------------------------------------
 // statistics (postproc::solveQ set), per-block partial sums then a fixed-order reduce
@@ -852,6 +907,102 @@ void launch_sweep(int scheme, int R, const SweepArgs &A, hipStream_t s)
     }
 }
 
+// ---------------------------------------------------------------------------
+// packed launches: one grid covers the workgroups of several jobs (contexts
+// with the same kernel configuration); workgroup b belongs to job j with
+// off[j] <= b < off[j+1] and runs that job's body with that job's arguments
+// at job-relative index b - off[j], i.e. exactly the work of the same
+// workgroup in the job's own launch.  Only short lines (R <= PACK_RMAX) are
+// packed: longer ones fill the device alone.  The tables are const
+// __restrict__, so their entries are read with scalar loads.
+// ---------------------------------------------------------------------------
+constexpr int MAXPACK = 64;
+constexpr int PACK_RMAX = 4;
+
+__device__ __forceinline__ int pack_job(const int *off, int nj, long b)
+{
+    int j = 0;
+    while (j + 1 < nj && b >= off[j + 1]) j++;
+    return j;
+}
+
+__global__ __launch_bounds__(256) void k_prim_pack(const PrimArgs *__restrict__ tab, const int *__restrict__ off, int nj)
+{
+    const int j = pack_job(off, nj, blockIdx.x);
+    prim_body(tab[j], (long)blockIdx.x - off[j], (long)off[j + 1] - off[j]);
+}
+
+__global__ void k_dt_pack(const DtArgs *__restrict__ tab) { dt_body(tab[blockIdx.x]); }
+
+__global__ void k_res_pack(const ResArgs *__restrict__ tab) { res_body(tab[blockIdx.x]); }
+
+template <typename T>
+__global__ __launch_bounds__(GTX * GTY) void k_gradq_pack(const GradArgs *__restrict__ tab, const int *__restrict__ off, int nj)
+{
+    const int j = pack_job(off, nj, blockIdx.x);
+    const GradArgs &A = tab[j];
+    const int rel = (int)blockIdx.x - off[j];
+    const int gx = (A.nx + GTX - 1) / GTX, gy = (A.ny + GTY - 1) / GTY;
+    gradq_body<T>(A, rel % gx, (rel / gx) % gy, rel / (gx * gy));
+}
+
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
+__global__ __launch_bounds__(64 * LPB) void k_sweep_pack(const SweepArgs *__restrict__ tab, const int *__restrict__ off, int nj)
+{
+    const long b = xcd_block(blockIdx.x, gridDim.x);
+    const int j = pack_job(off, nj, b);
+    sweep_body<SCH, RIE, R, LPB, T, FULL>(tab[j], b - off[j]);
+}
+
+struct PackLaunch {
+    const SweepArgs *tab;
+    const int *off;
+    int nj;
+    long nblocks;
+    bool mixed, full;
+};
+
+template <int SCH, int RIE, int R>
+void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
+{
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS);
+    const dim3 grid((unsigned)P.nblocks), block(64 * SWEEP_LPB);
+    if constexpr (fp32_ok) {
+        if (P.mixed) {
+            if (P.full) hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, P.tab, P.off, P.nj);
+            else hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, P.tab, P.off, P.nj);
+            return;
+        }
+    }
+    if (P.full) hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, double, true>), grid, block, 0, s, P.tab, P.off, P.nj);
+    else hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, double, false>), grid, block, 0, s, P.tab, P.off, P.nj);
+}
+
+template <int SCH, int RIE>
+void launch_sweep_pack_sr(int R, const PackLaunch &P, hipStream_t s)
+{
+    if (R == 2) launch_sweep_pack_r<SCH, RIE, 2>(P, s);
+    else launch_sweep_pack_r<SCH, RIE, 4>(P, s);
+}
+
+template <int SCH>
+void launch_sweep_pack_s(int riemann, int R, const PackLaunch &P, hipStream_t s)
+{
+    if (riemann == RIE_LF) launch_sweep_pack_sr<SCH, RIE_LF>(R, P, s);
+    else launch_sweep_pack_sr<SCH, RIE_AUSM>(R, P, s);
+}
+
+void launch_sweep_pack(int scheme, int riemann, int R, const PackLaunch &P, hipStream_t s)
+{
+    switch (scheme) {
+    case WENO5JS: launch_sweep_pack_s<WENO5JS>(riemann, R, P, s); break;
+    case WENO5Z: launch_sweep_pack_s<WENO5Z>(riemann, R, P, s); break;
+    case WENO7JS: launch_sweep_pack_s<WENO7JS>(riemann, R, P, s); break;
+    case CRWENO5: launch_sweep_pack_s<CRWENO5>(riemann, R, P, s); break;
+    default: launch_sweep_pack_s<HCCS>(riemann, R, P, s); break;
+    }
+}
+
 template <int SCH, int R>
 void launch_op_r(const OpRecArgs &A, hipStream_t s)
 {
@@ -1083,9 +1234,26 @@ extern "C" int nuc_download_state(nuc_ctx *x, double *q)
     return NUC_OK;
 }
 
-// one RHS evaluation of state buffer `in`; with epilogue the z sweep performs
-// RK stage `stage` from `in` (and q[0] = u^n) into `outb`
-static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
+// Arguments of every launch of one RHS evaluation of state buffer `in`; with
+// epilogue the z sweep performs RK stage `stage` from `in` (and q[0] = u^n)
+// into `outb`.  The single-context launches and the packed launches are built
+// from the same arguments.
+struct StageArgs {
+    PrimArgs prim{};
+    long prim_nb = 0;
+    bool do_prim = false;
+    DtArgs dt{};
+    bool do_dt = false;
+    GradArgs grad{};
+    dim3 ggrid;
+    bool do_grad = false;
+    SweepArgs sw[3]{};
+    long sw_nb[3] = {0, 0, 0};
+    ResArgs res{};
+    bool do_res = false;
+};
+
+static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, StageArgs &sa)
 {
     const nuc_config &c = x->c;
     const long N = x->N;
@@ -1093,31 +1261,24 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     const bool ns = (c.model == NUC_NAVIERSTOKES3D);
     const bool lam_needed = (stage == 0);
     {
-        PrimArgs P{};
-        for (int v = 0; v < 5; v++) P.q[v] = qi + v * N;
+        PrimArgs &P = sa.prim;
+        for (int v = 0; v < 5; v++) P.q[v] = gp(qi + v * N);
         const bool wp = x->forcing;   // the gradient kernel reads Q itself
-        if (wp) for (int f = 0; f < 4; f++) P.prim[f] = x->prim + f * N;
+        if (wp) for (int f = 0; f < 4; f++) P.prim[f] = gp(x->prim + f * N);
         P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
         P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2];
         P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
         P.riemann = c.riemann; P.do_lam = lam_needed ? 1 : 0; P.write_prim = wp ? 1 : 0;
-        P.gamma = c.gamma; P.mach = c.mach; P.lam = x->lam;
-        if (P.do_lam || P.write_prim) {
-            long nb = (N + 4095) / 4096;
-            if (nb > 65536) nb = 65536;
-            ProfScope ps(x, 0);
-            hipLaunchKernelGGL(k_prim, dim3((unsigned)nb), dim3(256), 0, x->stream, P);
-        }
-        if (lam_needed) {
-            ProfScope ps(x, 1);
-            hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, x->stream, x->lam, x->P, x->h[0], x->h[1], x->h[2], c.cfl,
-                               epilogue ? 1 : 0, x->scal);
-        }
+        P.gamma = c.gamma; P.mach = c.mach; P.lam = gp(x->lam);
+        sa.do_prim = P.do_lam || P.write_prim;
+        sa.prim_nb = std::min<long>((N + 4095) / 4096, 65536);
+        sa.do_dt = lam_needed;
+        sa.dt = DtArgs{gp(x->lam), x->P, epilogue ? 1 : 0, x->h[0], x->h[1], x->h[2], c.cfl, gp(x->scal)};
     }
     if (ns) {
-        GradArgs G{};
-        for (int v = 0; v < 5; v++) G.q[v] = qi + v * N;
-        for (int o = 0; o < 9; o++) G.out[o] = x->fv + o * N;
+        GradArgs &G = sa.grad;
+        for (int v = 0; v < 5; v++) G.q[v] = gp(qi + v * N);
+        for (int o = 0; o < 9; o++) G.out[o] = gp(x->fv + o * N);
         G.nx = x->n[0]; G.ny = x->n[1]; G.nz = x->n[2];
         for (int d = 0; d < 3; d++) {
             G.nloc[d] = x->nloc[d]; G.tsv[d] = x->tsv[d];
@@ -1127,10 +1288,56 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
         G.tref = c.t_ref; G.tinf = c.t_inf;
         G.gkz = grad_planes(x->n[2]);
-        dim3 grid((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + G.gkz - 1) / G.gkz);
+        sa.ggrid = dim3((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + G.gkz - 1) / G.gkz);
+        sa.do_grad = true;
+    }
+    // fv component order per direction: x (txx,txy,txz,ex) y (txy,tyy,tyz,ey) z (txz,tyz,tzz,ez)
+    static const int fvmap[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
+    for (int d = 0; d < 3; d++) {
+        SweepArgs &S = sa.sw[d];
+        S.dir = d; S.n = x->n[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
+        S.vis = ns ? ((c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6) : VIS_NONE;
+        S.riemann = c.riemann;
+        S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
+        S.nlines = N / x->n[d];
+        S.inv_h = 1.0 / x->h[d];
+        S.gamma = c.gamma;
+        for (int v = 0; v < 5; v++) { S.q[v] = gp(qi + v * N); S.rhs[v] = gp(x->rhs + v * N); }
+        if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(x->fv + (long)fvmap[d][o] * N);
+        S.stage = stage;
+        S.accum = (d == 0 && x->forcing) ? 1 : 0;
+        S.mixed = x->mixed ? 1 : 0;
+        S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
+        S.fmean = (d == 2 && x->forcing) ? gp(x->scal + 4) : nullptr;
+        S.epilogue = (d == 2 && epilogue) ? 1 : 0;
+        if (S.epilogue) {
+            for (int v = 0; v < 5; v++) { S.qn[v] = gp(x->q[0] + v * N); S.qout[v] = gp(x->q[outb] + v * N); }
+            S.dtp = gp(x->scal);
+            S.res_part = gp(x->res_part);
+        }
+        sa.sw_nb[d] = (S.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
+    }
+    sa.do_res = epilogue && stage == 2;
+    sa.res = ResArgs{gp(x->res_part), (int)x->res_nb, 1.0 / ((double)N * 5.0), gp(x->scal)};
+}
+
+static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
+{
+    StageArgs sa;
+    stage_args(x, in, stage, outb, epilogue, sa);
+    const long N = x->N;
+    if (sa.do_prim) {
+        ProfScope ps(x, 0);
+        hipLaunchKernelGGL(k_prim, dim3((unsigned)sa.prim_nb), dim3(256), 0, x->stream, sa.prim);
+    }
+    if (sa.do_dt) {
+        ProfScope ps(x, 1);
+        hipLaunchKernelGGL(k_dt, dim3(1), dim3(64), 0, x->stream, sa.dt);
+    }
+    if (sa.do_grad) {
         ProfScope ps(x, 2);
-        if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
-        else hipLaunchKernelGGL(k_gradq<double>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+        if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, sa.ggrid, dim3(GTX, GTY, 1), 0, x->stream, sa.grad);
+        else hipLaunchKernelGGL(k_gradq<double>, sa.ggrid, dim3(GTX, GTY, 1), 0, x->stream, sa.grad);
     }
     if (x->forcing) {
         // Const forcing (src/physicsModel.cpp:1368-1677) initialises rhs; the
@@ -1140,37 +1347,13 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         ProfScope ps(x, 7);
         nucforce::enqueue(x->fplan, x->prim, x->prim + N, x->prim + 2 * N, r, x->scal + 4, x->stream);
     }
-    // fv component order per direction: x (txx,txy,txz,ex) y (txy,tyy,tyz,ey) z (txz,tyz,tzz,ez)
-    static const int fvmap[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
     for (int d = 0; d < 3; d++) {
-        SweepArgs S{};
-        S.dir = d; S.n = x->n[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
-        S.vis = ns ? ((c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6) : VIS_NONE;
-        S.riemann = c.riemann;
-        S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
-        S.nlines = N / x->n[d];
-        S.inv_h = 1.0 / x->h[d];
-        S.gamma = c.gamma;
-        for (int v = 0; v < 5; v++) { S.q[v] = qi + v * N; S.rhs[v] = x->rhs + v * N; }
-        if (ns) for (int o = 0; o < 4; o++) S.fv[o] = x->fv + (long)fvmap[d][o] * N;
-        S.stage = stage;
-        S.accum = (d == 0 && x->forcing) ? 1 : 0;
-        S.mixed = x->mixed ? 1 : 0;
-        S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
-        S.fmean = (d == 2 && x->forcing) ? x->scal + 4 : nullptr;
-        S.epilogue = (d == 2 && epilogue) ? 1 : 0;
-        if (S.epilogue) {
-            for (int v = 0; v < 5; v++) { S.qn[v] = x->q[0] + v * N; S.qout[v] = x->q[outb] + v * N; }
-            S.dtp = x->scal;
-            S.res_part = x->res_part;
-        }
         ProfScope ps(x, 3 + d);
-        launch_sweep(c.scheme[d], x->cpt[d], S, x->stream);
+        launch_sweep(x->c.scheme[d], x->cpt[d], sa.sw[d], x->stream);
     }
-    if (epilogue && stage == 2) {
+    if (sa.do_res) {
         ProfScope ps(x, 6);
-        hipLaunchKernelGGL(k_res, dim3(1), dim3(256), 0, x->stream, x->res_part, (int)x->res_nb,
-                           1.0 / ((double)N * 5.0), x->scal);
+        hipLaunchKernelGGL(k_res, dim3(1), dim3(256), 0, x->stream, sa.res);
     }
     return NUC_OK;
 }
@@ -1306,11 +1489,11 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
         own_prim = true;
     }
     PrimArgs P{};
-    for (int v = 0; v < 5; v++) P.q[v] = x->q[0] + v * N;
-    for (int f = 0; f < 4; f++) P.prim[f] = prim + f * N;
+    for (int v = 0; v < 5; v++) P.q[v] = gp(x->q[0] + v * N);
+    for (int f = 0; f < 4; f++) P.prim[f] = gp(prim + f * N);
     P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
     P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2]; P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
-    P.riemann = c.riemann; P.do_lam = 0; P.write_prim = 1; P.gamma = c.gamma; P.mach = c.mach; P.lam = x->lam;
+    P.riemann = c.riemann; P.do_lam = 0; P.write_prim = 1; P.gamma = c.gamma; P.mach = c.mach; P.lam = gp(x->lam);
     long nbp = (N + 4095) / 4096;
     if (nbp > 65536) nbp = 65536;
     hipLaunchKernelGGL(k_prim, dim3((unsigned)nbp), dim3(256), 0, x->stream, P);
@@ -1470,5 +1653,240 @@ extern "C" int nuc_profile_read(const nuc_ctx *x, int slot, double *total_ms, lo
     if (total_ms) *total_ms = x->prof_ms[slot];
     if (launches) *launches = x->prof_n[slot];
     if (name) *name = kProfNames[slot];
+    return NUC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// packs: several contexts advanced by one launch per kernel (include/nuc3d_gpu.h)
+// ---------------------------------------------------------------------------
+static bool packable(const nuc_ctx *x)
+{
+    if (x->forcing) return false;
+    for (int d = 0; d < 3; d++)
+        if (x->cpt[d] > PACK_RMAX) return false;
+    return true;
+}
+
+static bool sweep_full(const nuc_ctx *x, int d)
+{
+    return x->n[d] == 32 * x->cpt[d] && (x->N / x->n[d]) % SWEEP_LPB == 0;
+}
+
+// same kernels for every launch of a step: scheme, splitting, model,
+// viscous schemes, precision, chunk length and tail-free variant per direction
+static bool pack_key_equal(const nuc_ctx *a, const nuc_ctx *b)
+{
+    if (a->c.riemann != b->c.riemann || a->c.model != b->c.model || a->mixed != b->mixed) return false;
+    for (int d = 0; d < 3; d++) {
+        if (a->c.scheme[d] != b->c.scheme[d] || a->c.vis_scheme[d] != b->c.vis_scheme[d]) return false;
+        if (a->cpt[d] != b->cpt[d] || sweep_full(a, d) != sweep_full(b, d)) return false;
+    }
+    return true;
+}
+
+struct nuc_pack {
+    std::vector<nuc_ctx *> m;
+    hipStream_t stream = nullptr;
+    std::vector<hipEvent_t> ev_in;
+    hipEvent_t ev_out = nullptr;
+    char *dbuf = nullptr;
+    // device tables (pointers into dbuf)
+    const PrimArgs *prim = nullptr;
+    const DtArgs *dt = nullptr;
+    const GradArgs *grad[3] = {nullptr, nullptr, nullptr};
+    const SweepArgs *sw[3][3] = {};
+    const ResArgs *res = nullptr;
+    const int *prim_off = nullptr, *grad_off = nullptr, *sw_off[3] = {nullptr, nullptr, nullptr};
+    long prim_total = 0, grad_total = 0, sw_total[3] = {0, 0, 0};
+    bool ns = false, mixed = false;
+    int scheme[3] = {0, 0, 0}, riemann = 0, cpt[3] = {0, 0, 0};
+    bool full[3] = {false, false, false};
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::string err;
+};
+
+extern "C" int nuc_pack_compatible(const nuc_ctx *a, const nuc_ctx *b)
+{
+    if (!a || !b) return 0;
+    return (packable(a) && packable(b) && pack_key_equal(a, b)) ? 1 : 0;
+}
+
+static std::string g_pack_err;
+
+extern "C" const char *nuc_pack_last_error(const nuc_pack *p) { return p ? p->err.c_str() : g_pack_err.c_str(); }
+
+extern "C" int nuc_pack_destroy(nuc_pack *p)
+{
+    if (!p) return NUC_OK;
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->dbuf) (void)hipFree(p->dbuf);
+    for (hipEvent_t e : p->ev_in) (void)hipEventDestroy(e);
+    if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return NUC_OK;
+}
+
+extern "C" int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nuc_pack **out)
+{
+    if (!members || !out || n < 1 || n > MAXPACK) { g_pack_err = "pack needs 1..64 contexts"; return NUC_EINVAL; }
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        if (!members[i]) { g_pack_err = "null context in pack"; return NUC_EINVAL; }
+        for (int k = 0; k < i; k++)
+            if (members[k] == members[i]) { g_pack_err = "context listed twice in pack"; return NUC_EINVAL; }
+        if (!packable(members[i])) {
+            g_pack_err = "context not packable (forcing on, or a line longer than 128 cells)";
+            return NUC_EINVAL;
+        }
+        if (!pack_key_equal(members[0], members[i])) {
+            g_pack_err = "contexts in a pack must share scheme, splitting, model, viscous schemes, precision "
+                         "and line-length class per direction";
+            return NUC_EINVAL;
+        }
+    }
+    nuc_pack *p = new nuc_pack();
+    p->m.assign(members, members + n);
+    const nuc_ctx *x0 = members[0];
+    p->ns = (x0->c.model == NUC_NAVIERSTOKES3D);
+    p->mixed = x0->mixed;
+    p->riemann = x0->c.riemann;
+    for (int d = 0; d < 3; d++) { p->scheme[d] = x0->c.scheme[d]; p->cpt[d] = x0->cpt[d]; p->full[d] = sweep_full(x0, d); }
+    p->use_graph = use_graph != 0;
+    auto fail = [&](const char *what, int rc) {
+        g_pack_err = what;
+        nuc_pack_destroy(p);
+        return rc;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess) return fail("pack stream", NUC_EHIP);
+    p->ev_in.resize(n, nullptr);
+    for (int i = 0; i < n; i++)
+        if (hipEventCreateWithFlags(&p->ev_in[i], hipEventDisableTiming) != hipSuccess) return fail("pack event", NUC_EHIP);
+    if (hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming) != hipSuccess) return fail("pack event", NUC_EHIP);
+
+    // the three stages of one RK3 step, per member: stage s reads q[s] and
+    // writes q[(s+1)%3] (as enqueue_one_step)
+    std::vector<StageArgs> sa((size_t)3 * n);
+    for (int i = 0; i < n; i++)
+        for (int st = 0; st < 3; st++) stage_args(members[i], st, st, (st + 1) % 3, true, sa[(size_t)st * n + i]);
+    // host image of the tables: every block 256-byte aligned
+    std::vector<char> img;
+    auto put = [&](const void *src, size_t bytes) {
+        size_t o = (img.size() + 255) & ~(size_t)255;
+        img.resize(o + bytes);
+        memcpy(img.data() + o, src, bytes);
+        return o;
+    };
+    std::vector<PrimArgs> vp(n);
+    std::vector<DtArgs> vd(n);
+    std::vector<ResArgs> vr(n);
+    std::vector<int> poff(n + 1), goff(n + 1), soff[3];
+    for (int d = 0; d < 3; d++) soff[d].resize(n + 1);
+    poff[0] = goff[0] = soff[0][0] = soff[1][0] = soff[2][0] = 0;
+    for (int i = 0; i < n; i++) {
+        const StageArgs &a0 = sa[i];
+        vp[i] = a0.prim;
+        vd[i] = a0.dt;
+        vr[i] = sa[(size_t)2 * n + i].res;
+        poff[i + 1] = poff[i] + (int)a0.prim_nb;
+        goff[i + 1] = goff[i] + (int)(a0.ggrid.x * a0.ggrid.y * a0.ggrid.z);
+        for (int d = 0; d < 3; d++) soff[d][i + 1] = soff[d][i] + (int)a0.sw_nb[d];
+    }
+    size_t o_prim = put(vp.data(), sizeof(PrimArgs) * n);
+    size_t o_dt = put(vd.data(), sizeof(DtArgs) * n);
+    size_t o_res = put(vr.data(), sizeof(ResArgs) * n);
+    size_t o_poff = put(poff.data(), sizeof(int) * (n + 1));
+    size_t o_goff = put(goff.data(), sizeof(int) * (n + 1));
+    size_t o_soff[3], o_grad[3], o_sw[3][3];
+    for (int d = 0; d < 3; d++) o_soff[d] = put(soff[d].data(), sizeof(int) * (n + 1));
+    for (int st = 0; st < 3; st++) {
+        std::vector<GradArgs> vg(n);
+        for (int i = 0; i < n; i++) vg[i] = sa[(size_t)st * n + i].grad;
+        o_grad[st] = put(vg.data(), sizeof(GradArgs) * n);
+        for (int d = 0; d < 3; d++) {
+            std::vector<SweepArgs> vs(n);
+            for (int i = 0; i < n; i++) vs[i] = sa[(size_t)st * n + i].sw[d];
+            o_sw[st][d] = put(vs.data(), sizeof(SweepArgs) * n);
+        }
+    }
+    if (hipMalloc(&p->dbuf, img.size()) != hipSuccess) return fail("pack table allocation", NUC_ENOMEM);
+    if (hipMemcpyAsync(p->dbuf, img.data(), img.size(), hipMemcpyHostToDevice, p->stream) != hipSuccess ||
+        hipStreamSynchronize(p->stream) != hipSuccess)
+        return fail("pack table upload", NUC_EHIP);
+    p->prim = reinterpret_cast<const PrimArgs *>(p->dbuf + o_prim);
+    p->dt = reinterpret_cast<const DtArgs *>(p->dbuf + o_dt);
+    p->res = reinterpret_cast<const ResArgs *>(p->dbuf + o_res);
+    p->prim_off = reinterpret_cast<const int *>(p->dbuf + o_poff);
+    p->grad_off = reinterpret_cast<const int *>(p->dbuf + o_goff);
+    for (int d = 0; d < 3; d++) p->sw_off[d] = reinterpret_cast<const int *>(p->dbuf + o_soff[d]);
+    for (int st = 0; st < 3; st++) {
+        p->grad[st] = reinterpret_cast<const GradArgs *>(p->dbuf + o_grad[st]);
+        for (int d = 0; d < 3; d++) p->sw[st][d] = reinterpret_cast<const SweepArgs *>(p->dbuf + o_sw[st][d]);
+    }
+    p->prim_total = poff[n];
+    p->grad_total = goff[n];
+    for (int d = 0; d < 3; d++) p->sw_total[d] = soff[d][n];
+    *out = p;
+    return NUC_OK;
+}
+
+// one RK3 step of every member, the launch order of enqueue_one_step
+static void pack_enqueue_step(nuc_pack *p)
+{
+    const int nj = (int)p->m.size();
+    for (int st = 0; st < 3; st++) {
+        if (st == 0) {
+            hipLaunchKernelGGL(k_prim_pack, dim3((unsigned)p->prim_total), dim3(256), 0, p->stream, p->prim,
+                               p->prim_off, nj);
+            hipLaunchKernelGGL(k_dt_pack, dim3(nj), dim3(64), 0, p->stream, p->dt);
+        }
+        if (p->ns) {
+            if (p->mixed)
+                hipLaunchKernelGGL(k_gradq_pack<float>, dim3((unsigned)p->grad_total), dim3(GTX, GTY, 1), 0, p->stream,
+                                   p->grad[st], p->grad_off, nj);
+            else
+                hipLaunchKernelGGL(k_gradq_pack<double>, dim3((unsigned)p->grad_total), dim3(GTX, GTY, 1), 0, p->stream,
+                                   p->grad[st], p->grad_off, nj);
+        }
+        for (int d = 0; d < 3; d++) {
+            PackLaunch L{p->sw[st][d], p->sw_off[d], nj, p->sw_total[d], p->mixed, p->full[d]};
+            launch_sweep_pack(p->scheme[d], p->riemann, p->cpt[d], L, p->stream);
+        }
+        if (st == 2) hipLaunchKernelGGL(k_res_pack, dim3(nj), dim3(256), 0, p->stream, p->res);
+    }
+}
+
+extern "C" int nuc_pack_step_async(nuc_pack *p, int nsteps)
+{
+    if (!p || nsteps < 0) return NUC_EINVAL;
+    if (nsteps == 0) return NUC_OK;
+    // the pack's stream waits for each member's earlier work ...
+    for (size_t i = 0; i < p->m.size(); i++) {
+        NUC_CHECK(p, hipEventRecord(p->ev_in[i], p->m[i]->stream));
+        NUC_CHECK(p, hipStreamWaitEvent(p->stream, p->ev_in[i], 0));
+    }
+    if (p->use_graph) {
+        if (!p->gexec) {
+            NUC_CHECK(p, hipStreamBeginCapture(p->stream, hipStreamCaptureModeRelaxed));
+            pack_enqueue_step(p);
+            hipError_t e = hipStreamEndCapture(p->stream, &p->graph);
+            if (e != hipSuccess) { p->err = std::string("pack graph capture: ") + hipGetErrorString(e); return NUC_EHIP; }
+            NUC_CHECK(p, hipGraphInstantiate(&p->gexec, p->graph, nullptr, nullptr, 0));
+        }
+        for (int it = 0; it < nsteps; it++) NUC_CHECK(p, hipGraphLaunch(p->gexec, p->stream));
+    } else {
+        for (int it = 0; it < nsteps; it++) pack_enqueue_step(p);
+        NUC_CHECK(p, hipGetLastError());
+    }
+    // ... and each member's later work waits for the pack's steps
+    NUC_CHECK(p, hipEventRecord(p->ev_out, p->stream));
+    for (nuc_ctx *x : p->m) {
+        NUC_CHECK(p, hipStreamWaitEvent(x->stream, p->ev_out, 0));
+        x->pending += nsteps;
+    }
     return NUC_OK;
 }

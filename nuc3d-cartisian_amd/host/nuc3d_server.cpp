@@ -12,10 +12,21 @@
 //   retire  : a job whose last slice is done is checked (finite dt and
 //             residual), downloaded, its context destroyed and its bytes
 //             returned to the budget.
-// Slice length is tuned per job from the observed duration of its own slices,
-// aiming at about SLICE_TARGET_MS of device work per turn: long enough to keep
-// the host off the critical path for small boxes, short enough that a large
-// box does not hold its slot's turn for long.
+//   pack    : idle jobs that can share kernels (nuc_pack_compatible) are
+//             grouped, up to pack_cells cells per group, and a group of two or
+//             more advances as one pack: one launch per kernel for all of its
+//             members (nuc_pack_step_async), each member's result unchanged.
+//             Packs are kept while their membership recurs (the members of a
+//             pack finish their slices together, so the same group usually
+//             forms again) and dropped when a member retires.
+// Slice length (steps per turn) is sized per job from a model of its own step
+// time -- a fixed launch cost plus its cells at the device's small-box
+// throughput -- aiming at about SLICE_TARGET_MS per turn, with at least
+// SLICE_MIN steps: long enough that the host's per-turn work (query, sync,
+// enqueue) stays off the critical path, short enough that a large box does
+// not hold its slot for long.  Wall-clock slice times are not used: with many
+// jobs in flight they measure the queueing behind other jobs, not the job.
+// A pack takes the shortest slice of its members.
 #include "nuc3d_serve.h"
 
 #include <algorithm>
@@ -32,8 +43,13 @@
 
 namespace {
 
-constexpr double SLICE_TARGET_MS = 4.0;
-constexpr int SLICE_MAX = 64;
+constexpr double SLICE_TARGET_MS = 8.0;
+constexpr int SLICE_MIN = 4;
+constexpr int SLICE_MAX = 32;
+// modelled step time of a box alone: launches of one RK3 step, plus three
+// stages of its cells at a small-box throughput of 2.5e9 cell-updates/s
+inline double step_ms_model(long long cells) { return 0.25 + 3.0 * (double)cells / 2.5e9 * 1e3; }
+constexpr size_t MAX_PACKS = 16;
 
 struct Job {
     long long id = 0;
@@ -44,7 +60,13 @@ struct Job {
     nuc_ctx *ctx = nullptr;
     nuc_job_result r{};
     std::string err;
-    double slice_t0 = 0.0;
+    long long cells = 0;
+    bool packable = false;
+};
+
+struct PackRec {
+    std::vector<long long> ids;   // member job ids, in pack order
+    nuc_pack *pack = nullptr;
 };
 
 }  // namespace
@@ -59,6 +81,8 @@ struct nuc_server {
     long long next_id = 1;
     long long bytes_in_use = 0, peak_bytes = 0, completed = 0, failed = 0, peak_active = 0;
     long long n_active = 0;
+    long long single_slices = 0, pack_slices = 0;
+    std::vector<PackRec> packs;                   // owned by the worker
     bool stop = false;
     std::thread worker;
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -69,10 +93,26 @@ struct nuc_server {
     }
     void run();
     void finish(const std::shared_ptr<Job> &j, int state, int error, const std::string &msg);
+    void drop_packs_of(long long id);
+    void launch_idle(std::vector<std::shared_ptr<Job>> &idle);
+    bool start_slice(const std::shared_ptr<Job> &j);
 };
+
+void nuc_server::drop_packs_of(long long id)
+{
+    for (size_t i = 0; i < packs.size();) {
+        if (std::find(packs[i].ids.begin(), packs[i].ids.end(), id) != packs[i].ids.end()) {
+            nuc_pack_destroy(packs[i].pack);
+            packs.erase(packs.begin() + i);
+        } else {
+            ++i;
+        }
+    }
+}
 
 void nuc_server::finish(const std::shared_ptr<Job> &j, int state, int error, const std::string &msg)
 {
+    drop_packs_of(j->id);   // a pack must go before its members
     if (j->ctx) {
         nuc_destroy(j->ctx);
         j->ctx = nullptr;
@@ -124,14 +164,22 @@ void nuc_server::run()
                 continue;
             }
             if (cfg.use_graphs) nuc_set_graph(j->ctx, 1);
+            j->cells = (long long)j->cfg.nx * j->cfg.ny * j->cfg.nz;
+            j->packable = nuc_pack_compatible(j->ctx, j->ctx) != 0;
             rc = nuc_upload_state(j->ctx, j->q.data());
             if (rc != NUC_OK) {
                 finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
                 continue;
             }
-            j->slice = cfg.steps_per_slice > 0 ? cfg.steps_per_slice : 1;
+            if (cfg.steps_per_slice > 0) {
+                j->slice = cfg.steps_per_slice;
+            } else {
+                int k = (int)(SLICE_TARGET_MS / step_ms_model((long long)j->cfg.nx * j->cfg.ny * j->cfg.nz));
+                j->slice = std::min(SLICE_MAX, std::max(SLICE_MIN, k));
+            }
             active.push_back(j);
         }
+        std::vector<std::shared_ptr<Job>> idle;
         for (size_t a = 0; a < active.size();) {
             auto j = active[a];
             int q = nuc_query(j->ctx);
@@ -149,11 +197,6 @@ void nuc_server::run()
                         retire = true;
                     } else {
                         j->r.info = info;
-                        if (cfg.steps_per_slice <= 0) {
-                            double per_step_ms = (now() - j->slice_t0) * 1e3 / j->inflight;
-                            int s = (int)(SLICE_TARGET_MS / std::max(per_step_ms, 1e-3));
-                            j->slice = std::min(SLICE_MAX, std::max(1, s));
-                        }
                         j->inflight = 0;
                     }
                 }
@@ -163,28 +206,102 @@ void nuc_server::run()
                     else finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
                     retire = true;
                 } else if (!retire) {
-                    int k = std::min(j->slice, j->nsteps - j->done);
-                    j->slice_t0 = now();
-                    int rc = nuc_step_async(j->ctx, k);
-                    if (rc != NUC_OK) {
-                        finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
-                        retire = true;
-                    } else {
-                        j->done += k;
-                        j->inflight = k;
-                    }
+                    idle.push_back(j);
                 }
             }
             progressed = true;
             if (retire) active.erase(active.begin() + a);
             else ++a;
         }
+        launch_idle(idle);
         if (!progressed) {
             std::unique_lock<std::mutex> lk(mu);
             if (stop) return;
             // idle: every admitted job is busy on the device, or nothing to do
             cv_work.wait_for(lk, std::chrono::microseconds(active.empty() ? 2000 : 50));
         }
+    }
+}
+
+// one slice of a job on its own stream; false when the job failed (retired)
+bool nuc_server::start_slice(const std::shared_ptr<Job> &j)
+{
+    int k = std::min(j->slice, j->nsteps - j->done);
+    int rc = nuc_step_async(j->ctx, k);
+    if (rc != NUC_OK) {
+        finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
+        active.erase(std::find(active.begin(), active.end(), j));
+        return false;
+    }
+    j->done += k;
+    j->inflight = k;
+    single_slices++;
+    return true;
+}
+
+void nuc_server::launch_idle(std::vector<std::shared_ptr<Job>> &idle)
+{
+    std::vector<bool> used(idle.size(), false);
+    for (size_t a = 0; a < idle.size(); a++) {
+        if (used[a]) continue;
+        used[a] = true;
+        std::vector<std::shared_ptr<Job>> grp{idle[a]};
+        if (cfg.pack_cells > 0 && idle[a]->packable) {
+            long long cells = idle[a]->cells;
+            for (size_t b = a + 1; b < idle.size() && grp.size() < 64; b++) {
+                if (used[b] || !idle[b]->packable || cells + idle[b]->cells > cfg.pack_cells) continue;
+                if (!nuc_pack_compatible(idle[a]->ctx, idle[b]->ctx)) continue;
+                used[b] = true;
+                cells += idle[b]->cells;
+                grp.push_back(idle[b]);
+            }
+        }
+        if (grp.size() == 1) {
+            start_slice(grp[0]);
+            continue;
+        }
+        std::vector<long long> ids;
+        for (auto &j : grp) ids.push_back(j->id);
+        PackRec *rec = nullptr;
+        for (auto &p : packs)
+            if (p.ids == ids) { rec = &p; break; }
+        if (!rec) {
+            // bounded cache: drop a pack none of whose members has work in flight
+            if (packs.size() >= MAX_PACKS) {
+                for (size_t i = 0; i < packs.size(); i++) {
+                    bool busy = false;
+                    for (long long id : packs[i].ids)
+                        for (auto &j : active)
+                            if (j->id == id && j->inflight > 0) busy = true;
+                    if (!busy) {
+                        nuc_pack_destroy(packs[i].pack);
+                        packs.erase(packs.begin() + i);
+                        break;
+                    }
+                }
+            }
+            std::vector<nuc_ctx *> m;
+            for (auto &j : grp) m.push_back(j->ctx);
+            nuc_pack *pk = nullptr;
+            if (nuc_pack_create(m.data(), (int)m.size(), cfg.use_graphs, &pk) != NUC_OK) {
+                for (auto &j : grp) start_slice(j);   // run them one by one instead
+                continue;
+            }
+            packs.push_back(PackRec{ids, pk});
+            rec = &packs.back();
+        }
+        int k = SLICE_MAX;
+        for (auto &j : grp) k = std::min(k, std::min(j->slice, j->nsteps - j->done));
+        if (nuc_pack_step_async(rec->pack, k) != NUC_OK) {
+            drop_packs_of(grp[0]->id);
+            for (auto &j : grp) start_slice(j);
+            continue;
+        }
+        for (auto &j : grp) {
+            j->done += k;
+            j->inflight = k;
+        }
+        pack_slices++;
     }
 }
 
@@ -207,6 +324,8 @@ extern "C" int nuc_server_destroy(nuc_server *s)
     }
     s->cv_work.notify_all();
     if (s->worker.joinable()) s->worker.join();
+    for (auto &p : s->packs) nuc_pack_destroy(p.pack);
+    s->packs.clear();
     for (auto &j : s->active)
         if (j->ctx) nuc_destroy(j->ctx);
     delete s;
@@ -278,6 +397,8 @@ extern "C" int nuc_server_stats_get(nuc_server *s, nuc_server_stats *st)
     st->peak_active = s->peak_active;
     st->bytes_in_use = s->bytes_in_use;
     st->peak_bytes = s->peak_bytes;
+    st->single_slices = s->single_slices;
+    st->pack_slices = s->pack_slices;
     return NUC_OK;
 }
 

@@ -34,7 +34,9 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_stats", "nuc_stats_ex", "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp",
            "nuc_init_tgv", "nuc_init_hit", "nuc_init_shu_osher", "nuc_profile", "nuc_profile_read",
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
-           "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error"]
+           "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error",
+           "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
+           "nuc_pack_last_error"]
 
 
 class NucConfig(ctypes.Structure):
@@ -59,7 +61,7 @@ class StepInfo(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_longlong), ("max_active", ctypes.c_int),
-                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int), ("pack_cells", ctypes.c_longlong)]
 
 
 class JobResult(ctypes.Structure):
@@ -69,7 +71,8 @@ class JobResult(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_longlong) for k in ("completed", "failed", "queued", "active", "peak_active",
-                                                  "bytes_in_use", "peak_bytes")]
+                                                  "bytes_in_use", "peak_bytes", "single_slices",
+                                                  "pack_slices")]
 
 
 class NucError(RuntimeError):
@@ -130,6 +133,12 @@ def lib():
         L.nuc_server_now.restype = ctypes.c_double
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
+        L.nuc_pack_compatible.argtypes = [vp, vp]
+        L.nuc_pack_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
+        L.nuc_pack_step_async.argtypes = [vp, ctypes.c_int]
+        L.nuc_pack_destroy.argtypes = [vp]
+        L.nuc_pack_last_error.argtypes = [vp]
+        L.nuc_pack_last_error.restype = ctypes.c_char_p
         _lib = L
     return _lib
 
@@ -332,11 +341,51 @@ def op_rk3(stage, rhs, un, ui, dt):
     return ui
 
 
+def pack_compatible(a, b):
+    """True when two contexts can share packed launches (nuc_pack_compatible)."""
+    return bool(lib().nuc_pack_compatible(a.h, b.h))
+
+
+class Pack:
+    """Several contexts advanced by one launch per kernel (nuc_pack_*).  Each
+    member's result equals nuc_step on that member alone; close the pack before
+    its members."""
+
+    def __init__(self, contexts, use_graph=False):
+        self.members = list(contexts)
+        arr = (ctypes.c_void_p * len(self.members))(*[c.h for c in self.members])
+        h = ctypes.c_void_p()
+        rc = lib().nuc_pack_create(arr, len(self.members), 1 if use_graph else 0, ctypes.byref(h))
+        if rc != 0:
+            raise NucError(rc, "nuc_pack_create: " + lib().nuc_pack_last_error(None).decode())
+        self.h = h
+
+    def step_async(self, n=1):
+        rc = lib().nuc_pack_step_async(self.h, n)
+        if rc != 0:
+            raise NucError(rc, "nuc_pack_step_async: " + lib().nuc_pack_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nuc_pack_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
 class Server:
     """The serving loop (include/nuc3d_serve.h): submit jobs, fetch results."""
 
-    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_slice=0, use_graphs=True):
-        c = ServerConfig(int(hbm_budget), int(max_active), int(steps_per_slice), 1 if use_graphs else 0)
+    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_slice=0, use_graphs=True, pack_cells=0):
+        c = ServerConfig(int(hbm_budget), int(max_active), int(steps_per_slice), 1 if use_graphs else 0,
+                         int(pack_cells))
         h = ctypes.c_void_p()
         _check(lib().nuc_server_create(ctypes.byref(c), ctypes.byref(h)), "nuc_server_create")
         self.h = h

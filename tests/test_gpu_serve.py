@@ -121,3 +121,32 @@ def test_failed_job_does_not_disturb_others():
     assert r2[1].state == G.JOB_FAILED
     assert r1[1].state == G.JOB_DONE and r3[1].state == G.JOB_DONE
     assert np.array_equal(r1[0], r3[0])
+
+
+def test_packed_serving_results_identical_to_alone():
+    """With packing on, small compatible jobs that are idle together advance as
+    one pack; each result still equals the job run alone."""
+    print("serving with packs", flush=True)
+    mix = [((32, 32, 32), G.CRWENO5, 0.1, 1600.0, 11, True, 6),
+           ((40, 40, 40), G.CRWENO5, 0.3, 900.0, 12, False, 6),
+           ((36, 36, 36), G.CRWENO5, 0.2, 2000.0, 13, False, 6),
+           ((48, 48, 48), G.CRWENO5, 0.45, 500.0, 14, True, 6),
+           ((32, 32, 32), G.WENO5Z, 0.25, 700.0, 15, False, 6),
+           ((40, 40, 40), G.WENO5Z, 0.35, 1500.0, 16, True, 6),
+           ((160, 160, 160), G.WENO5JS, 0.2, 1000.0, 17, False, 2)]
+    ids, ref = {}, {}
+    for i, (n, sch, mach, re, seed, tgv, steps) in enumerate(mix):
+        cfg, q = _job(n, sch, mach, re, seed, tgv)
+        ref[i] = (cfg, q, steps)
+    with G.Server(hbm_budget=16 << 30, max_active=8, steps_per_slice=2, pack_cells=1 << 20) as srv:
+        # submitted back to back, so that compatible jobs are idle together
+        for i, (cfg, q, steps) in ref.items():
+            ids[i] = srv.submit(cfg, q, steps)
+        out = {i: _wait(srv, jid) for i, jid in ids.items()}
+        st = srv.stats()
+    print("server stats", st, flush=True)
+    assert st["completed"] == len(mix) and st["pack_slices"] > 0
+    for i, (state, r) in out.items():
+        cfg, q, steps = ref[i]
+        assert r.state == G.JOB_DONE and r.info.step == steps
+        assert np.array_equal(state, _alone(cfg, q, steps)), i
