@@ -99,6 +99,15 @@ int nuc_step_async(nuc_ctx *ctx, int nsteps);
  * dominates small boxes.  No counterpart in the reference (its step loop is
  * src/block.cpp:191-215).  Profiling (nuc_profile) falls back to launches. */
 int nuc_set_graph(nuc_ctx *ctx, int enable);
+/* Autotuner: time each candidate tile shape of the gradient kernel (planes per
+ * workgroup of its z march: 4, 8, 16, 32) on ctx's current state and keep the
+ * fastest; the choice is cached per (nx, ny, nz, precision) for the process.
+ * Results do not depend on the tile shape.  No counterpart in the reference.
+ * nuc_tile_shape reports the shape in use; nuc_set_tile_shape sets it
+ * (1 <= planes <= nz). */
+int nuc_autotune(nuc_ctx *ctx);
+int nuc_tile_shape(const nuc_ctx *ctx, int *grad_planes);
+int nuc_set_tile_shape(nuc_ctx *ctx, int grad_planes);
 /* NUC_OK when all enqueued work of ctx has finished, NUC_EBUSY otherwise; never blocks. */
 int nuc_query(nuc_ctx *ctx);
 int nuc_sync(nuc_ctx *ctx, nuc_step_info *info);

@@ -33,6 +33,7 @@ typedef struct nuc_server_config {
     int use_graphs;         /* replay captured hipGraph steps (1) or launch kernels (0) */
     long long pack_cells;   /* idle compatible jobs (nuc_pack_compatible) of together at most this
                                many cells are advanced as one pack (nuc_pack_*); 0: no packing */
+    int autotune;           /* pick each job's tile shapes with nuc_autotune (cached per shape) */
 } nuc_server_config;
 
 enum nuc_job_state { NUC_JOB_QUEUED = 0, NUC_JOB_RUNNING = 1, NUC_JOB_DONE = 2, NUC_JOB_FAILED = 3 };

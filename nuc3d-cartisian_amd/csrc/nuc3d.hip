@@ -27,6 +27,9 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 using namespace nucdev;
@@ -1044,6 +1047,8 @@ struct nuc_ctx {
     double *stat_out = nullptr;
     bool forcing = false;
     bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
+    // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
+    int gkz = 0;
     // one captured RK3 step, replayed by nuc_step/nuc_step_async (nuc_set_graph)
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -1287,7 +1292,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         }
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
         G.tref = c.t_ref; G.tinf = c.t_inf;
-        G.gkz = grad_planes(x->n[2]);
+        G.gkz = x->gkz > 0 ? x->gkz : grad_planes(x->n[2]);
         sa.ggrid = dim3((x->n[0] + GTX - 1) / GTX, (x->n[1] + GTY - 1) / GTY, (x->n[2] + G.gkz - 1) / G.gkz);
         sa.do_grad = true;
     }
@@ -1383,6 +1388,84 @@ static int enqueue_steps(nuc_ctx *x, int nsteps)
     }
     for (int it = 0; it < nsteps; it++) enqueue_one_step(x);
     NUC_CHECK(x, hipGetLastError());
+    return NUC_OK;
+}
+
+// ---- autotuning of tile shapes ----------------------------------------------
+// The gradient kernel's z march (planes per workgroup) trades the 8-plane halo
+// re-read against the number of workgroups; its best value depends on the box
+// shape.  nuc_autotune times every candidate on the context's own state and
+// keeps the fastest; results are shared by all contexts of the same shape and
+// precision in the process.  The tile shape does not change any result.
+static std::mutex g_tune_mu;
+static std::map<std::tuple<int, int, int, int>, int> g_tune_gkz;
+
+extern "C" int nuc_autotune(nuc_ctx *x)
+{
+    if (!x) return NUC_EINVAL;
+    if (x->c.model != NUC_NAVIERSTOKES3D) return NUC_OK;   // no gradient kernel
+    const auto key = std::make_tuple(x->n[0], x->n[1], x->n[2], x->mixed ? 1 : 0);
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_gkz.find(key);
+        if (it != g_tune_gkz.end()) {
+            x->gkz = it->second;
+            if (x->gexec) { (void)hipGraphExecDestroy(x->gexec); x->gexec = nullptr; }
+            if (x->graph) { (void)hipGraphDestroy(x->graph); x->graph = nullptr; }
+            return NUC_OK;
+        }
+    }
+    StageArgs sa;
+    stage_args(x, 0, 0, 0, false, sa);
+    hipEvent_t e0, e1;
+    NUC_CHECK(x, hipEventCreate(&e0));
+    NUC_CHECK(x, hipEventCreate(&e1));
+    const int cand[4] = {4, 8, 16, 32};
+    int best = grad_planes(x->n[2]);
+    float best_ms = 1e30f;
+    for (int gkz : cand) {
+        if (gkz > x->n[2]) continue;
+        GradArgs G = sa.grad;
+        G.gkz = gkz;
+        const dim3 grid(sa.ggrid.x, sa.ggrid.y, (x->n[2] + gkz - 1) / gkz);
+        float tot = 0.f;
+        for (int rep = 0; rep < 4; rep++) {
+            (void)hipEventRecord(e0, x->stream);
+            if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+            else hipLaunchKernelGGL(k_gradq<double>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+            (void)hipEventRecord(e1, x->stream);
+            NUC_CHECK(x, hipEventSynchronize(e1));
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0) tot += ms;   // the first launch warms caches
+        }
+        if (tot < best_ms) { best_ms = tot; best = gkz; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    NUC_CHECK(x, hipGetLastError());
+    x->gkz = best;
+    // a captured step holds the old launch shape
+    if (x->gexec) { (void)hipGraphExecDestroy(x->gexec); x->gexec = nullptr; }
+    if (x->graph) { (void)hipGraphDestroy(x->graph); x->graph = nullptr; }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_gkz[key] = best;
+    return NUC_OK;
+}
+
+extern "C" int nuc_set_tile_shape(nuc_ctx *x, int planes)
+{
+    if (!x || planes < 1 || planes > x->n[2]) return NUC_EINVAL;
+    x->gkz = planes;
+    if (x->gexec) { (void)hipGraphExecDestroy(x->gexec); x->gexec = nullptr; }
+    if (x->graph) { (void)hipGraphDestroy(x->graph); x->graph = nullptr; }
+    return NUC_OK;
+}
+
+extern "C" int nuc_tile_shape(const nuc_ctx *x, int *grad_planes_out)
+{
+    if (!x || !grad_planes_out) return NUC_EINVAL;
+    *grad_planes_out = x->gkz > 0 ? x->gkz : grad_planes(x->n[2]);
     return NUC_OK;
 }
 

@@ -167,6 +167,7 @@ void nuc_server::run()
             j->cells = (long long)j->cfg.nx * j->cfg.ny * j->cfg.nz;
             j->packable = nuc_pack_compatible(j->ctx, j->ctx) != 0;
             rc = nuc_upload_state(j->ctx, j->q.data());
+            if (rc == NUC_OK && cfg.autotune) rc = nuc_autotune(j->ctx);
             if (rc != NUC_OK) {
                 finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
                 continue;

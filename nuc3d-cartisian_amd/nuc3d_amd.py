@@ -36,7 +36,7 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
            "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error",
            "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
-           "nuc_pack_last_error"]
+           "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape"]
 
 
 class NucConfig(ctypes.Structure):
@@ -61,7 +61,8 @@ class StepInfo(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_longlong), ("max_active", ctypes.c_int),
-                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int), ("pack_cells", ctypes.c_longlong)]
+                ("steps_per_slice", ctypes.c_int), ("use_graphs", ctypes.c_int), ("pack_cells", ctypes.c_longlong),
+                ("autotune", ctypes.c_int)]
 
 
 class JobResult(ctypes.Structure):
@@ -133,6 +134,9 @@ def lib():
         L.nuc_server_now.restype = ctypes.c_double
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
+        L.nuc_autotune.argtypes = [vp]
+        L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
         L.nuc_pack_compatible.argtypes = [vp, vp]
         L.nuc_pack_create.argtypes = [ctypes.POINTER(vp), ctypes.c_int, ctypes.c_int, ctypes.POINTER(vp)]
         L.nuc_pack_step_async.argtypes = [vp, ctypes.c_int]
@@ -262,6 +266,19 @@ class Context:
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
 
+    def autotune(self):
+        """Pick the gradient kernel's tile shape by timing (nuc_autotune); returns it."""
+        _check(lib().nuc_autotune(self.h), "nuc_autotune", self.h)
+        return self.tile_shape()
+
+    def set_tile_shape(self, grad_planes):
+        _check(lib().nuc_set_tile_shape(self.h, int(grad_planes)), "nuc_set_tile_shape", self.h)
+
+    def tile_shape(self):
+        g = ctypes.c_int()
+        _check(lib().nuc_tile_shape(self.h, ctypes.byref(g)), "nuc_tile_shape", self.h)
+        return g.value
+
     def set_graph(self, enable=True):
         """Replay one captured hipGraph per RK3 step instead of per-kernel launches."""
         _check(lib().nuc_set_graph(self.h, 1 if enable else 0), "nuc_set_graph", self.h)
@@ -383,9 +400,10 @@ class Pack:
 class Server:
     """The serving loop (include/nuc3d_serve.h): submit jobs, fetch results."""
 
-    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_slice=0, use_graphs=True, pack_cells=0):
+    def __init__(self, hbm_budget=64 << 30, max_active=16, steps_per_slice=0, use_graphs=True, pack_cells=0,
+                 autotune=False):
         c = ServerConfig(int(hbm_budget), int(max_active), int(steps_per_slice), 1 if use_graphs else 0,
-                         int(pack_cells))
+                         int(pack_cells), 1 if autotune else 0)
         h = ctypes.c_void_p()
         _check(lib().nuc_server_create(ctypes.byref(c), ctypes.byref(h)), "nuc_server_create")
         self.h = h

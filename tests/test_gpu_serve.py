@@ -150,3 +150,30 @@ def test_packed_serving_results_identical_to_alone():
         cfg, q, steps = ref[i]
         assert r.state == G.JOB_DONE and r.info.step == steps
         assert np.array_equal(state, _alone(cfg, q, steps)), i
+
+
+def test_autotuned_tile_shape_keeps_results():
+    """nuc_autotune picks the gradient kernel's tile shape by timing; the state
+    after several steps is unchanged by the choice, and the choice is shared by
+    contexts of the same shape."""
+    cfg, q = _job((48, 40, 64), G.CRWENO5, 0.3, 900.0, 21, False)
+    ref = _alone(cfg, q, 3)
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        g = ctx.autotune()
+        assert g in (4, 8, 16, 32)
+        ctx.step(3)
+        assert np.array_equal(ctx.download(), ref)
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        assert ctx.autotune() == g
+        ctx.set_graph(True)
+        ctx.step(3)
+        assert np.array_equal(ctx.download(), ref)
+    for planes in (1, 4, 8, 16, 32, 64):
+        with G.Context(cfg) as ctx:
+            ctx.upload(q)
+            ctx.set_tile_shape(planes)
+            assert ctx.tile_shape() == planes
+            ctx.step(3)
+            assert np.array_equal(ctx.download(), ref), planes
