@@ -90,12 +90,16 @@ def measured_traffic(kernel, n):
 
 
 TILE_SHAPES = {}
+PCIE = {}
 
 
 def stage_stats(G, cfg, q, steps, warmup):
     """(seconds for `steps` timed steps, stats_ex after warmup+steps, profile)"""
     with G.Context(cfg) as ctx:
         ctx.upload(q)
+        t_up = time.perf_counter()
+        ctx.upload(q)                  # host -> HBM over PCIe (pageable numpy buffer)
+        PCIE["upload_ms"] = (time.perf_counter() - t_up) * 1e3
         # tile shapes from the library's autotuner, before the untimed warmup
         TILE_SHAPES[cfg.precision] = ctx.autotune()
         if warmup:
@@ -110,7 +114,9 @@ def stage_stats(G, cfg, q, steps, warmup):
         prof = ctx.profile_read()
         ctx.profile(False)
         st = ctx.stats_ex()
+        t_dn = time.perf_counter()
         q_end = ctx.download()
+        PCIE["download_ms"] = (time.perf_counter() - t_dn) * 1e3
     return t1 - t0, st, prof, q_end, dt
 
 
@@ -307,7 +313,9 @@ def main():
                                "cd8 viscous + TVD-RK3, CFL 0.5, FP64",
                    "grid": [n, n, n], "rank_grid": [1, 1, 1], "mach_t": 0.3, "reynolds": 200.0,
                    "stage_ms": round(stage_ms, 4), "dt_last": dt,
-                   "grad_planes_autotuned": TILE_SHAPES.get(G.FP64)},
+                   "grad_planes_autotuned": TILE_SHAPES.get(G.FP64),
+                   "state_upload_ms": round(PCIE.get("upload_ms", 0.0), 2),
+                   "state_download_ms": round(PCIE.get("download_ms", 0.0), 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
