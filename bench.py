@@ -152,10 +152,12 @@ def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
     return jobs, arrivals
 
 
-def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=SERVE_MAX_ACTIVE, stats=None):
+def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=SERVE_MAX_ACTIVE, stats=None,
+                steps=None, hbm_budget=96 << 30, progress=0):
     """All jobs through one server; arrivals None = all at once. Returns (wall s, latencies s)."""
     import numpy as np
-    with G.Server(hbm_budget=96 << 30, max_active=max_active, pack_cells=pack_cells, autotune=True) as srv:
+    steps = SERVE_STEPS if steps is None else steps
+    with G.Server(hbm_budget=hbm_budget, max_active=max_active, pack_cells=pack_cells, autotune=True) as srv:
         ids = []
         t0 = time.perf_counter()
         for k, (cfg, q, _) in enumerate(jobs):
@@ -163,9 +165,12 @@ def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=
                 wait = arrivals[k] - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
-            ids.append(srv.submit(cfg, q, SERVE_STEPS))
+            ids.append(srv.submit(cfg, q, steps))
         lat = []
-        for jid in ids:
+        for k, jid in enumerate(ids):
+            if progress and k % progress == 0:
+                print(f"bench: serving, {k}/{len(ids)} jobs done, {time.perf_counter() - t0:.1f} s",
+                      file=sys.stderr, flush=True)
             while True:
                 r = srv.result(jid, want_state=False)
                 if r is not None:
@@ -180,14 +185,62 @@ def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=
     return wall, np.array(lat)
 
 
-def run_sequential(G, jobs):
+def run_sequential(G, jobs, steps=None, progress=0):
+    steps = SERVE_STEPS if steps is None else steps
     t0 = time.perf_counter()
-    for cfg, q, _ in jobs:
+    for k, (cfg, q, _) in enumerate(jobs):
+        if progress and k % progress == 0:
+            print(f"bench: sequential, {k}/{len(jobs)} jobs done, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
         with G.Context(cfg) as ctx:
             ctx.upload(q)
-            ctx.step(SERVE_STEPS)
+            ctx.step(steps)
             ctx.download()
     return time.perf_counter() - t0
+
+
+def serve_full(G):
+    """The full serving stream: 256 jobs (TGV or decaying turbulence, N
+    log-uniform in [32, 192], Mach U(0.1, 0.5), Re U(400, 3200), scheme uniform
+    over WENO5-JS/WENO5-Z/CRWENO5, 300 RK3 steps each), Poisson arrivals at 20
+    jobs/s, admitted against a 64 GiB HBM budget, as an FP64 stream and a mixed
+    stream; each against the same jobs run one at a time on plain contexts."""
+    import copy
+    import numpy as np
+    njobs, steps = 256, 300
+    print("bench: generating the 256-job stream", file=sys.stderr, flush=True)
+    jobs, arrivals = serving_stream(G, njobs, 32, 192, steps, 20.0, 20261202 + 500)
+    out = {"metric": "serving stream: jobs/s and p50/p99 job latency (arrival to result)",
+           "jobs": njobs, "steps_per_job": steps, "grid_n": [32, 192], "poisson_rate_per_s": 20.0,
+           "hbm_budget_gib": 64, "max_active": SERVE_MAX_ACTIVE, "pack_cells": SERVE_PACK_CELLS,
+           "cells_per_job_mean": float(np.mean([nj ** 3 for _, _, nj in jobs]))}
+    for label, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
+        stream = []
+        for cfg, q, nj in jobs:
+            c = copy.copy(cfg)
+            c.precision = prec
+            stream.append((c, q, nj))
+        print(f"bench: {label} stream, warm-up and tuner cache", file=sys.stderr, flush=True)
+        run_serving(G, stream[:4], steps=2)
+        seen = set()
+        for cfg, q, nj in stream:
+            if nj not in seen:
+                seen.add(nj)
+                with G.Context(cfg) as tctx:
+                    tctx.upload(q)
+                    tctx.autotune()
+        st = {}
+        wall, lat = run_serving(G, stream, arrivals, steps=steps, hbm_budget=64 << 30, progress=32, stats=st)
+        seq = run_sequential(G, stream, steps=steps, progress=32)
+        out[label] = {"jobs_per_s": round(njobs / wall, 3), "wall_s": round(wall, 2),
+                      "sequential_jobs_per_s": round(njobs / seq, 3), "sequential_wall_s": round(seq, 2),
+                      "speedup_vs_sequential": round(seq / wall, 3),
+                      "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
+                      "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+                      "peak_active": int(st.get("peak_active", 0)), "peak_bytes": int(st.get("peak_bytes", 0)),
+                      "pack_slices": int(st.get("pack_slices", 0))}
+        print(f"bench: {label} stream {out[label]}", file=sys.stderr, flush=True)
+    print(json.dumps(out))
 
 
 SERVE_STEPS = 100
@@ -205,6 +258,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=64)
+    ap.add_argument("--serve-full", action="store_true",
+                    help="only the full 256-job serving stream (N 32-192, 300 steps; several minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box per GPU and measures one MI355X; "
@@ -213,6 +268,10 @@ def main():
 
     import numpy as np
     import nuc3d_amd as G
+
+    if args.serve_full:
+        serve_full(G)
+        return
 
     n = args.n
     cfg = workload_config(G, n)
