@@ -92,6 +92,13 @@ int nuc_rhs(nuc_ctx *ctx, double *rhs_host, double *dt_local);
 /* nsteps TVD-RK3 steps: block::solve (src/block.cpp:191-215), i.e. three
  * (RHS + PDEData3d::solve) stages each.  info may be NULL. */
 int nuc_step(nuc_ctx *ctx, int nsteps, nuc_step_info *info);
+/* One RK stage: the RHS of the current stage state and PDEData3d::solve's
+ * update (src/PDEData3d.cpp:70-91, src/TVD-RK3.cpp:16-157), i.e. one pass of
+ * block::solve's stage loop (src/block.cpp:198-202).  Stages run in order
+ * 0, 1, 2; dt is set at stage 0.  Between stages nuc_download_state,
+ * nuc_stats and nuc_rhs see the current stage state; nuc_step and packs refuse
+ * a context in the middle of a step, and nuc_upload_state starts a new step. */
+int nuc_rk3_stage(nuc_ctx *ctx, int stage);
 /* enqueue nsteps steps without waiting (timing); nuc_sync waits */
 int nuc_step_async(nuc_ctx *ctx, int nsteps);
 /* Replay steps from one captured hipGraph of a whole RK3 step (kernel nodes

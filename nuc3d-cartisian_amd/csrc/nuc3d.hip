@@ -1049,6 +1049,8 @@ struct nuc_ctx {
     bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
     // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
     int gkz = 0;
+    // next RK stage of nuc_rk3_stage; the current state is q[next_stage]
+    int next_stage = 0;
     // one captured RK3 step, replayed by nuc_step/nuc_step_async (nuc_set_graph)
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -1226,6 +1228,7 @@ extern "C" const char *nuc_last_error(const nuc_ctx *x) { return x ? x->err.c_st
 extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
 {
     if (!x || !q) return NUC_EINVAL;
+    x->next_stage = 0;   // an upload starts a new step
     NUC_CHECK(x, hipMemcpyAsync(x->q[0], q, 5 * x->N * sizeof(double), hipMemcpyHostToDevice, x->stream));
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
@@ -1234,7 +1237,7 @@ extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
 extern "C" int nuc_download_state(nuc_ctx *x, double *q)
 {
     if (!x || !q) return NUC_EINVAL;
-    NUC_CHECK(x, hipMemcpyAsync(q, x->q[0], 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipMemcpyAsync(q, x->q[x->next_stage], 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }
@@ -1486,9 +1489,17 @@ extern "C" int nuc_query(nuc_ctx *x)
     return NUC_EHIP;
 }
 
+static bool mid_step(nuc_ctx *x)
+{
+    if (x->next_stage == 0) return false;
+    x->err = "a step is in progress: run its remaining stages with nuc_rk3_stage";
+    return true;
+}
+
 extern "C" int nuc_step_async(nuc_ctx *x, int nsteps)
 {
     if (!x || nsteps < 0) return NUC_EINVAL;
+    if (mid_step(x)) return NUC_EINVAL;
     int rc = enqueue_steps(x, nsteps);
     if (rc == NUC_OK) x->pending += nsteps;
     return rc;
@@ -1525,6 +1536,7 @@ extern "C" int nuc_sync(nuc_ctx *x, nuc_step_info *info)
 extern "C" int nuc_step(nuc_ctx *x, int nsteps, nuc_step_info *info)
 {
     if (!x || nsteps < 0) return NUC_EINVAL;
+    if (nsteps > 0 && mid_step(x)) return NUC_EINVAL;
     for (int it = 0; it < nsteps; it++) {
         int rc = enqueue_steps(x, 1);
         if (rc != NUC_OK) return rc;
@@ -1536,10 +1548,32 @@ extern "C" int nuc_step(nuc_ctx *x, int nsteps, nuc_step_info *info)
     return NUC_OK;
 }
 
+// one RK stage of block::solve (src/block.cpp:198-202): the RHS of the current
+// stage state (NaiverStokesData3d::solve) and PDEData3d::solve's update
+// (src/PDEData3d.cpp:70-91) into the next stage state
+extern "C" int nuc_rk3_stage(nuc_ctx *x, int stage)
+{
+    if (!x) return NUC_EINVAL;
+    if (stage != x->next_stage) {
+        x->err = "RK stages run in order 0, 1, 2 (src/block.cpp:198-202); next is stage " +
+                 std::to_string(x->next_stage);
+        return NUC_EINVAL;
+    }
+    enqueue_stage(x, stage, stage, (stage + 1) % 3, true);
+    NUC_CHECK(x, hipGetLastError());
+    x->next_stage = (stage + 1) % 3;
+    if (stage < 2) {
+        NUC_CHECK(x, hipStreamSynchronize(x->stream));
+        return NUC_OK;
+    }
+    x->pending += 1;   // the step is complete: time, step count, dt and residual
+    return nuc_sync(x, nullptr);
+}
+
 extern "C" int nuc_rhs(nuc_ctx *x, double *rhs_host, double *dt_local)
 {
     if (!x) return NUC_EINVAL;
-    enqueue_stage(x, 0, 0, 0, false);
+    enqueue_stage(x, x->next_stage, 0, 0, false);
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     if (rhs_host)
@@ -1572,7 +1606,7 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
         own_prim = true;
     }
     PrimArgs P{};
-    for (int v = 0; v < 5; v++) P.q[v] = gp(x->q[0] + v * N);
+    for (int v = 0; v < 5; v++) P.q[v] = gp(x->q[x->next_stage] + v * N);
     for (int f = 0; f < 4; f++) P.prim[f] = gp(prim + f * N);
     P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
     P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2]; P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
@@ -1592,7 +1626,7 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
         NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&x->stat_out), NSTAT * sizeof(double), x->stream));
     StatArgs S{};
     for (int f = 0; f < 4; f++) S.prim[f] = prim + f * N;
-    S.rho = x->q[0];
+    S.rho = x->q[x->next_stage];
     S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
     for (int d = 0; d < 3; d++) {
         S.nloc[d] = x->nloc[d]; S.tsv[d] = x->tsv[d];
@@ -1826,6 +1860,10 @@ extern "C" int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nu
             g_pack_err = "context not packable (forcing on, or a line longer than 128 cells)";
             return NUC_EINVAL;
         }
+        if (members[i]->next_stage != 0) {
+            g_pack_err = "context is in the middle of a step (nuc_rk3_stage)";
+            return NUC_EINVAL;
+        }
         if (!pack_key_equal(members[0], members[i])) {
             g_pack_err = "contexts in a pack must share scheme, splitting, model, viscous schemes, precision "
                          "and line-length class per direction";
@@ -1947,6 +1985,8 @@ extern "C" int nuc_pack_step_async(nuc_pack *p, int nsteps)
 {
     if (!p || nsteps < 0) return NUC_EINVAL;
     if (nsteps == 0) return NUC_OK;
+    for (nuc_ctx *x : p->m)
+        if (x->next_stage != 0) { p->err = "a member is in the middle of a step (nuc_rk3_stage)"; return NUC_EINVAL; }
     // the pack's stream waits for each member's earlier work ...
     for (size_t i = 0; i < p->m.size(); i++) {
         NUC_CHECK(p, hipEventRecord(p->ev_in[i], p->m[i]->stream));

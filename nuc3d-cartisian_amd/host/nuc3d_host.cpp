@@ -171,6 +171,8 @@ nuc_step_info PDEData3d::step(int n)
     return info;
 }
 
+void PDEData3d::solve(int stage) { check(nuc_rk3_stage(ctx_, stage), "nuc_rk3_stage"); }
+
 // ---- initial conditions -------------------------------------------------------
 void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vector<double> &q)
 {

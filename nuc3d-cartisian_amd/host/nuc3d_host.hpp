@@ -168,6 +168,9 @@ public:
     double solveRHS(VectorField *rhs);
     // one RK3 step (block::solve)
     nuc_step_info step(int n = 1);
+    // one RK stage: RHS of the current stage state and the stage update
+    // (NaiverStokesData3d::solve + PDEData3d::solve, src/block.cpp:198-202)
+    void solve(int stage);
 };
 
 // Taylor-Green initial condition of src/block.cpp:359-395 on the rank grid

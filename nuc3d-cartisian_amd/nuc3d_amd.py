@@ -36,7 +36,7 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
            "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error",
            "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
-           "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape"]
+           "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage"]
 
 
 class NucConfig(ctypes.Structure):
@@ -135,6 +135,7 @@ def lib():
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
         L.nuc_autotune.argtypes = [vp]
+        L.nuc_rk3_stage.argtypes = [vp, ctypes.c_int]
         L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
         L.nuc_pack_compatible.argtypes = [vp, vp]
@@ -265,6 +266,10 @@ class Context:
 
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    def rk3_stage(self, stage):
+        """One RK stage (nuc_rk3_stage); stages run in order 0, 1, 2."""
+        _check(lib().nuc_rk3_stage(self.h, int(stage)), "nuc_rk3_stage", self.h)
 
     def autotune(self):
         """Pick the gradient kernel's tile shape by timing (nuc_autotune); returns it."""

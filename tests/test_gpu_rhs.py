@@ -298,3 +298,33 @@ def test_long_lines_ten_steps_match_oracle(case):
     assert np.all(err <= tol), (err, tol)
     assert abs(dt_gpu - dt_ref) <= max(1e-13, tol.max()) * dt_ref
     assert abs(res_gpu - res_ref) <= max(1e-9, 100 * tol.max()) * abs(res_ref)
+
+
+def test_rk3_stages_one_at_a_time():
+    """nuc_rk3_stage: stage 0 gives u1 = u^n - dt RHS(u^n) with dt = cfl *
+    dt_local (src/PDEData3d.cpp:70-91, src/TVD-RK3.cpp:59-64); three stages equal
+    one nuc_step bit for bit; stages out of order and whole steps in the middle of
+    a step are refused."""
+    gc, oc = _cfgs((24, 20, 18), (1, 1, 1), G.CRWENO5, G.CD6, G.LF, G.NAVIERSTOKES3D)
+    q = _perturbed_tgv(oc, 8)
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        ctx.step(1)
+        ref = ctx.download()
+        ref_info = ctx.sync()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        rhs, dt_local = ctx.rhs()
+        with pytest.raises(G.NucError):
+            ctx.rk3_stage(1)
+        ctx.rk3_stage(0)
+        u1 = ctx.download()
+        expect = q - gc.cfl * dt_local * rhs
+        assert np.abs(u1 - expect).max() <= 1e-14 * np.abs(expect).max()
+        with pytest.raises(G.NucError):
+            ctx.step(1)
+        ctx.rk3_stage(1)
+        ctx.rk3_stage(2)
+        assert np.array_equal(ctx.download(), ref)
+        assert ctx.sync() == ref_info
+        ctx.step(1)   # a whole step after a completed one is fine
