@@ -1830,17 +1830,19 @@ extern "C" int nuc_pack_compatible(const nuc_ctx *a, const nuc_ctx *b)
     return (packable(a) && packable(b) && pack_key_equal(a, b)) ? 1 : 0;
 }
 
-static std::string g_pack_err;
+static thread_local std::string g_pack_err;
 
 extern "C" const char *nuc_pack_last_error(const nuc_pack *p) { return p ? p->err.c_str() : g_pack_err.c_str(); }
 
 extern "C" int nuc_pack_destroy(nuc_pack *p)
 {
     if (!p) return NUC_OK;
+    // stream-ordered free: a device-wide hipFree would stall every other
+    // context's work (the serving loop drops packs while jobs run)
+    if (p->dbuf && p->stream) (void)hipFreeAsync(p->dbuf, p->stream);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->gexec) (void)hipGraphExecDestroy(p->gexec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    if (p->dbuf) (void)hipFree(p->dbuf);
     for (hipEvent_t e : p->ev_in) (void)hipEventDestroy(e);
     if (p->ev_out) (void)hipEventDestroy(p->ev_out);
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -1934,7 +1936,8 @@ extern "C" int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nu
             o_sw[st][d] = put(vs.data(), sizeof(SweepArgs) * n);
         }
     }
-    if (hipMalloc(&p->dbuf, img.size()) != hipSuccess) return fail("pack table allocation", NUC_ENOMEM);
+    if (hipMallocAsync(reinterpret_cast<void **>(&p->dbuf), img.size(), p->stream) != hipSuccess)
+        return fail("pack table allocation", NUC_ENOMEM);
     if (hipMemcpyAsync(p->dbuf, img.data(), img.size(), hipMemcpyHostToDevice, p->stream) != hipSuccess ||
         hipStreamSynchronize(p->stream) != hipSuccess)
         return fail("pack table upload", NUC_EHIP);
