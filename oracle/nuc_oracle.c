@@ -973,3 +973,271 @@ void orc_stats(orc_solver *s, double *out)
     out[24] = skew;
     free(g);
 }
+
+/* ---------------------------------------------------------------------------
+ * Time-averaged statistics and TKE budget: postproc::solveAveraged0/1/2
+ * (src/postproc.cpp:1186-2110) with the field lists of src/postproc.hpp:49-121.
+ * type 1: primaries + gradients, 2: + derived, 3: + TKE fields and budget
+ * (TimeAverageType, src/postproc.cpp:37-60).  Every derivative is solveGrad
+ * (src/postproc.cpp:2111-2124,2284-2298): the viscous scheme per direction with
+ * periodic rank ghosts, as diff_dir above.
+ * ------------------------------------------------------------------------- */
+struct orc_post {
+    int type;
+    long N;
+    long step;                 /* averStep */
+    double *tp, *ap;           /* primary: rho u v w p T e alpha miu coeff */
+    double *tg, *ag;           /* grads (21) */
+    double *td, *ad;           /* derived (28) */
+    double *tt, *at;           /* TKE (49) */
+    double *bud;               /* budget (45) */
+};
+
+static double *post_alloc(long n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+orc_post *orc_post_create(const orc_solver *s, int type)
+{
+    if (type < 1 || type > 3) return NULL;
+    orc_post *p = (orc_post *)calloc(1, sizeof(orc_post));
+    long N = s->N;
+    p->type = type;
+    p->N = N;
+    p->tp = post_alloc(ORC_NPRIM * N); p->ap = post_alloc(ORC_NPRIM * N);
+    p->tg = post_alloc(ORC_NGRAD * N); p->ag = post_alloc(ORC_NGRAD * N);
+    if (type >= 2) { p->td = post_alloc(ORC_NDER * N); p->ad = post_alloc(ORC_NDER * N); }
+    if (type >= 3) { p->tt = post_alloc(ORC_NTKE * N); p->at = post_alloc(ORC_NTKE * N); p->bud = post_alloc(ORC_NBUD * N); }
+    return p;
+}
+
+void orc_post_destroy(orc_post *p)
+{
+    if (!p) return;
+    free(p->tp); free(p->ap); free(p->tg); free(p->ag); free(p->td); free(p->ad);
+    free(p->tt); free(p->at); free(p->bud);
+    free(p);
+}
+
+/* f_aver = (f_aver*averStep + f_temp)/(averStep + 1), src/postproc.cpp:1318 */
+static void post_average(double *a, const double *t, long n, long step)
+{
+    for (long i = 0; i < n; i++) a[i] = (a[i] * step + t[i]) / (step + 1.0);
+}
+
+/* solveGrad of one field, one or all three directions */
+static void post_grad(orc_solver *s, const double *f, double *dx, double *dy, double *dz)
+{
+    if (dx) diff_dir(s, 0, f, dx);
+    if (dy) diff_dir(s, 1, f, dy);
+    if (dz) diff_dir(s, 2, f, dz);
+}
+
+#define F(base, k) ((base) + (size_t)(k) * N)
+
+/* solveAveraged0, src/postproc.cpp:1207-1349 */
+static void post_level0(orc_post *p, orc_solver *s)
+{
+    const long N = s->N;
+    compute_prims_and_eigen(s);
+    double *tp = p->tp, *tg = p->tg;
+    for (long id = 0; id < N; id++) {
+        for (int v = 0; v < 8; v++) tp[v * N + id] = s->prim[v * N + id];   /* rho u v w p T e alpha */
+        orc_viscosity(s->c.viscosity, s->prim[5 * N + id], s->c.reynolds, s->c.mach, s->c.prandtl, s->c.gamma,
+                      s->c.t_ref, s->c.t_inf, &tp[8 * N + id], &tp[9 * N + id]);   /* getMiu: miu, coeff */
+    }
+    /* d(u, v, w, p, T)/d(x, y, z) */
+    const int src[5] = {1, 2, 3, 4, 5};
+    for (int f = 0; f < 5; f++) post_grad(s, F(tp, src[f]), F(tg, 3 * f), F(tg, 3 * f + 1), F(tg, 3 * f + 2));
+    for (long id = 0; id < N; id++) {
+        double uy = tg[1 * N + id], uz = tg[2 * N + id], vx = tg[3 * N + id], vz = tg[5 * N + id];
+        double wx = tg[6 * N + id], wy = tg[7 * N + id];
+        double o0 = wy - vz, o1 = uz - wx, o2 = vx - uy;
+        tg[15 * N + id] = o0; tg[16 * N + id] = o1; tg[17 * N + id] = o2;
+        tg[18 * N + id] = pow(o0, 2.0); tg[19 * N + id] = pow(o1, 2.0); tg[20 * N + id] = pow(o2, 2.0);
+    }
+    post_average(p->ap, tp, ORC_NPRIM * N, p->step);
+    post_average(p->ag, tg, ORC_NGRAD * N, p->step);
+}
+
+/* solveAveraged1, src/postproc.cpp:1351-1483 */
+static void post_level1(orc_post *p, orc_solver *s)
+{
+    const long N = s->N;
+    const double *tp = p->tp, *tg = p->tg;
+    double *td = p->td;
+    for (long id = 0; id < N; id++) {
+        double rho = tp[id], u = tp[N + id], v = tp[2 * N + id], w = tp[3 * N + id], e = tp[6 * N + id];
+        double miu = tp[8 * N + id];
+        double ux = tg[id], uy = tg[N + id], uz = tg[2 * N + id], vx = tg[3 * N + id], vy = tg[4 * N + id];
+        double vz = tg[5 * N + id], wx = tg[6 * N + id], wy = tg[7 * N + id], wz = tg[8 * N + id];
+        double o[28];
+        o[0] = rho * u; o[1] = rho * v; o[2] = rho * w; o[3] = rho * e;
+        o[4] = u * u; o[5] = u * v; o[6] = u * w; o[7] = v * v; o[8] = v * w; o[9] = w * w;
+        o[10] = rho * u * u; o[11] = rho * u * v; o[12] = rho * u * w;
+        o[13] = rho * v * v; o[14] = rho * v * w; o[15] = rho * w * w;
+        double velgrad = ux + vy + wz;
+        o[16] = miu * (2.0 * ux - 2.0 / 3.0 * velgrad);
+        o[17] = miu * (uy + vx - 2.0 / 3.0 * velgrad);
+        o[18] = miu * (uz + wx - 2.0 / 3.0 * velgrad);
+        o[19] = miu * (2.0 * vy - 2.0 / 3.0 * velgrad);
+        o[20] = miu * (vz + wy - 2.0 / 3.0 * velgrad);
+        o[21] = miu * (2.0 * wz - 2.0 / 3.0 * velgrad);
+        for (int t = 0; t < 6; t++) o[22 + t] = rho * o[16 + t];
+        for (int t = 0; t < 28; t++) td[t * N + id] = o[t];
+    }
+    post_average(p->ad, td, ORC_NDER * N, p->step);
+}
+
+/* solveAveraged2, src/postproc.cpp:1485-2110 */
+static void post_level2(orc_post *p, orc_solver *s)
+{
+    const long N = s->N;
+    const double *tp = p->tp, *tg = p->tg, *td = p->td;
+    double *tt = p->tt;
+    /* derivatives of the viscous stress, src/postproc.cpp:1566-1592 */
+    diff_dir(s, 0, F(td, 16), F(tt, 6));
+    diff_dir(s, 0, F(td, 17), F(tt, 7));
+    diff_dir(s, 0, F(td, 18), F(tt, 8));
+    diff_dir(s, 1, F(td, 17), F(tt, 9));
+    diff_dir(s, 1, F(td, 19), F(tt, 10));
+    diff_dir(s, 1, F(td, 20), F(tt, 11));
+    diff_dir(s, 2, F(td, 18), F(tt, 12));
+    diff_dir(s, 2, F(td, 20), F(tt, 13));
+    diff_dir(s, 2, F(td, 21), F(tt, 14));
+    for (long id = 0; id < N; id++) {
+        double rho = tp[id], u = tp[N + id], v = tp[2 * N + id], w = tp[3 * N + id], pr = tp[4 * N + id];
+        double ux = tg[id], uy = tg[N + id], uz = tg[2 * N + id], vx = tg[3 * N + id], vy = tg[4 * N + id];
+        double vz = tg[5 * N + id], wx = tg[6 * N + id], wy = tg[7 * N + id], wz = tg[8 * N + id];
+        double txx = td[16 * N + id], txy = td[17 * N + id], txz = td[18 * N + id];
+        double tyy = td[19 * N + id], tyz = td[20 * N + id], tzz = td[21 * N + id];
+        tt[0 * N + id] = u * pr; tt[1 * N + id] = v * pr; tt[2 * N + id] = w * pr;
+        tt[3 * N + id] = pr * ux; tt[4 * N + id] = pr * vy; tt[5 * N + id] = pr * wz;
+        double a, b, c;
+        a = u * txx; b = v * txy; c = w * txz;
+        tt[15 * N + id] = a; tt[16 * N + id] = b; tt[17 * N + id] = c; tt[18 * N + id] = a + b + c;
+        a = u * txy; b = v * tyy; c = w * tyz;
+        tt[20 * N + id] = a; tt[21 * N + id] = b; tt[22 * N + id] = c; tt[23 * N + id] = a + b + c;
+        a = u * txz; b = v * tyz; c = w * tzz;
+        tt[25 * N + id] = a; tt[26 * N + id] = b; tt[27 * N + id] = c; tt[28 * N + id] = a + b + c;
+        tt[30 * N + id] = txx * ux; tt[31 * N + id] = txy * vx; tt[32 * N + id] = txz * wx;
+        tt[33 * N + id] = txy * uy; tt[34 * N + id] = tyy * vy; tt[35 * N + id] = tyz * wy;
+        tt[36 * N + id] = txz * uz; tt[37 * N + id] = tyz * vz; tt[38 * N + id] = tzz * wz;
+        tt[39 * N + id] = rho * u * u * u; tt[40 * N + id] = rho * u * u * v; tt[41 * N + id] = rho * u * u * w;
+        tt[42 * N + id] = rho * u * v * v; tt[43 * N + id] = rho * u * v * w; tt[44 * N + id] = rho * u * w * w;
+        tt[45 * N + id] = rho * v * v * v; tt[46 * N + id] = rho * v * v * w; tt[47 * N + id] = rho * v * w * w;
+        tt[48 * N + id] = rho * w * w * w;
+    }
+    diff_dir(s, 0, F(tt, 18), F(tt, 19));
+    diff_dir(s, 1, F(tt, 23), F(tt, 24));
+    diff_dir(s, 2, F(tt, 28), F(tt, 29));
+    post_average(p->at, tt, ORC_NTKE * N, p->step);
+
+    /* the budget from the averaged fields, src/postproc.cpp:1720-2110 */
+    const double *ap = p->ap, *ag = p->ag, *ad = p->ad, *at = p->at;
+    double *bu = p->bud;
+    post_grad(s, F(ad, 0), F(bu, 12), F(bu, 13), F(bu, 14));
+    post_grad(s, F(ad, 1), F(bu, 15), F(bu, 16), F(bu, 17));
+    post_grad(s, F(ad, 2), F(bu, 18), F(bu, 19), F(bu, 20));
+    for (long id = 0; id < N; id++) {
+        double pa = ap[4 * N + id];
+        bu[21 * N + id] = at[0 * N + id] - ap[1 * N + id] * pa;
+        bu[22 * N + id] = at[1 * N + id] - ap[2 * N + id] * pa;
+        bu[23 * N + id] = at[2 * N + id] - ap[3 * N + id] * pa;
+    }
+    diff_dir(s, 0, F(bu, 21), F(bu, 24));
+    diff_dir(s, 1, F(bu, 22), F(bu, 25));
+    diff_dir(s, 2, F(bu, 23), F(bu, 26));
+    for (long id = 0; id < N; id++) {
+        double u = ap[1 * N + id], v = ap[2 * N + id], w = ap[3 * N + id];
+        double txx = ad[16 * N + id], txy = ad[17 * N + id], txz = ad[18 * N + id];
+        double tyy = ad[19 * N + id], tyz = ad[20 * N + id], tzz = ad[21 * N + id];
+        bu[27 * N + id] = u * txx + v * txy + w * txz;
+        bu[28 * N + id] = u * txy + v * tyy + w * tyz;
+        bu[29 * N + id] = u * txz + v * tyz + w * tzz;
+    }
+    diff_dir(s, 0, F(bu, 27), F(bu, 30));
+    diff_dir(s, 1, F(bu, 28), F(bu, 31));
+    diff_dir(s, 2, F(bu, 29), F(bu, 32));
+    for (long id = 0; id < N; id++) {
+        double rho = ap[id], u = ap[1 * N + id], v = ap[2 * N + id], w = ap[3 * N + id], pa = ap[4 * N + id];
+        double ux = ag[id], uy = ag[N + id], uz = ag[2 * N + id], vx = ag[3 * N + id], vy = ag[4 * N + id];
+        double vz = ag[5 * N + id], wx = ag[6 * N + id], wy = ag[7 * N + id], wz = ag[8 * N + id];
+        double px = ag[9 * N + id], py = ag[10 * N + id], pz = ag[11 * N + id];
+        double ru = ad[0 * N + id], rv = ad[1 * N + id], rw = ad[2 * N + id];
+        double ruu = ad[10 * N + id], ruv = ad[11 * N + id], ruw = ad[12 * N + id];
+        double rvv = ad[13 * N + id], rvw = ad[14 * N + id], rww = ad[15 * N + id];
+        double txx = ad[16 * N + id], txy = ad[17 * N + id], txz = ad[18 * N + id];
+        double tyy = ad[19 * N + id], tyz = ad[20 * N + id], tzz = ad[21 * N + id];
+        const double *d = bu;
+        double tke = 0.5 * (ruu / rho - pow(ru / rho, 2.0)) + 0.5 * (rvv / rho - pow(rv / rho, 2.0)) +
+                     0.5 * (rww / rho - pow(rw / rho, 2.0));
+        double prod = (ruu - ru * ru / rho) * d[12 * N + id] + (ruv - ru * rv / rho) * d[13 * N + id] +
+                      (ruw - ru * rw / rho) * d[14 * N + id] + (ruv - ru * rv / rho) * d[15 * N + id] +
+                      (rvv - rv * rv / rho) * d[16 * N + id] + (rvw - rv * rw / rho) * d[17 * N + id] +
+                      (ruw - ru * rw / rho) * d[18 * N + id] + (rvw - rv * rw / rho) * d[19 * N + id] +
+                      (rww - rw * rw / rho) * d[20 * N + id];
+        double p1 = -(u - ru / rho) * px - (v - rv / rho) * py - (w - rw / rho) * pz;
+        double p2 = -d[24 * N + id] - d[25 * N + id] - d[26 * N + id];
+        double p3 = (at[3 * N + id] - ux * pa) + (at[4 * N + id] - vy * pa) + (at[5 * N + id] - wz * pa);
+        double v1 = (u - ru / rho) * at[6 * N + id] + (v - rv / rho) * at[7 * N + id] + (w - rw / rho) * at[8 * N + id] +
+                    (u - ru / rho) * at[9 * N + id] + (v - rv / rho) * at[10 * N + id] + (w - rw / rho) * at[11 * N + id] +
+                    (u - ru / rho) * at[12 * N + id] + (v - rv / rho) * at[13 * N + id] + (w - rw / rho) * at[14 * N + id];
+        double v2 = (at[19 * N + id] - d[30 * N + id]) + (at[24 * N + id] - d[31 * N + id]) + (at[29 * N + id] - d[32 * N + id]);
+        double v3 = -(at[30 * N + id] - ux * txx) - (at[31 * N + id] - vx * txy) - (at[32 * N + id] - wx * txz) -
+                    (at[33 * N + id] - uy * txy) - (at[34 * N + id] - vy * tyy) - (at[35 * N + id] - wy * tyz) -
+                    (at[36 * N + id] - uz * txz) - (at[37 * N + id] - vz * tyz) - (at[38 * N + id] - wz * tzz);
+        bu[0 * N + id] = tke;
+        bu[1 * N + id] = prod;
+        bu[4 * N + id] = p1; bu[5 * N + id] = p2; bu[6 * N + id] = p3;
+        bu[3 * N + id] = p1 + p2 + p3;
+        bu[8 * N + id] = v1; bu[9 * N + id] = v2; bu[10 * N + id] = v3;
+        bu[7 * N + id] = v1 + v2 + v3;
+        bu[33 * N + id] = ru * tke; bu[34 * N + id] = rv * tke; bu[35 * N + id] = rw * tke;
+    }
+    diff_dir(s, 0, F(bu, 33), F(bu, 36));
+    diff_dir(s, 1, F(bu, 34), F(bu, 37));
+    diff_dir(s, 2, F(bu, 35), F(bu, 38));
+    for (long id = 0; id < N; id++) {
+        double rho = ap[id];
+        double ru = ad[0 * N + id], rv = ad[1 * N + id], rw = ad[2 * N + id];
+        double ruu = ad[10 * N + id], ruv = ad[11 * N + id], ruw = ad[12 * N + id];
+        double rvv = ad[13 * N + id], rvw = ad[14 * N + id], rww = ad[15 * N + id];
+        double r3[10];
+        for (int t = 0; t < 10; t++) r3[t] = at[(39 + t) * N + id];   /* uuu uuv uuw uvv uvw uww vvv vvw vww www */
+        bu[11 * N + id] = -bu[36 * N + id] - bu[37 * N + id] - bu[38 * N + id];
+        /* the bracket of the last term, evaluated left to right as written */
+        double kk = ruu - ru * ru / rho + rvv - rv * rv / rho + rww - rw * rw / rho;
+        bu[39 * N + id] = 0.5 * r3[0] - ruu * ru / rho + 0.5 * ru * ru * ru / rho / rho +
+                          0.5 * r3[3] - ruv * rv / rho + 0.5 * ru * rv * rv / rho / rho +
+                          0.5 * r3[5] - ruw * rw / rho + 0.5 * ru * rw * rw / rho / rho - ru / rho * 0.5 * kk;
+        bu[40 * N + id] = 0.5 * r3[1] - ruv * ru / rho + 0.5 * rv * ru * ru / rho / rho +
+                          0.5 * r3[6] - rvv * rv / rho + 0.5 * rv * rv * rv / rho / rho +
+                          0.5 * r3[8] - rvw * rw / rho + 0.5 * rv * rw * rw / rho / rho - rv / rho * 0.5 * kk;
+        bu[41 * N + id] = 0.5 * r3[2] - ruw * ru / rho + 0.5 * rw * ru * ru / rho / rho +
+                          0.5 * r3[7] - rvw * rv / rho + 0.5 * rw * rv * rv / rho / rho +
+                          0.5 * r3[9] - rww * rw / rho + 0.5 * rw * rw * rw / rho / rho - rw / rho * 0.5 * kk;
+    }
+    diff_dir(s, 0, F(bu, 39), F(bu, 42));
+    diff_dir(s, 1, F(bu, 40), F(bu, 43));
+    diff_dir(s, 2, F(bu, 41), F(bu, 44));
+    for (long id = 0; id < N; id++) bu[2 * N + id] = -bu[42 * N + id] - bu[43 * N + id] - bu[44 * N + id];
+}
+#undef F
+
+void orc_post_update(orc_post *p, orc_solver *s)
+{
+    post_level0(p, s);
+    if (p->type >= 2) post_level1(p, s);
+    if (p->type >= 3) post_level2(p, s);
+    p->step++;
+}
+
+const double *orc_post_field(const orc_post *p, int group, int index)
+{
+    static const int count[5] = {ORC_NPRIM, ORC_NGRAD, ORC_NDER, ORC_NTKE, ORC_NBUD};
+    if (group < 0 || group > 4 || index < 0 || index >= count[group]) return NULL;
+    const double *base[5] = {p->ap, p->ag, p->ad, p->at, p->bud};
+    if (!base[group]) return NULL;
+    return base[group] + (size_t)index * p->N;
+}
+
+long orc_post_steps(const orc_post *p) { return p->step; }

@@ -81,6 +81,20 @@ void orc_init_tgv(const orc_config *cfg, double *q);
  * out[16..24] = u_rms, M_t, sqrt(div_aver), miu_aver, Re_lambda, lambda, rho_bar, tau_t, skew. */
 void orc_stats(orc_solver *s, double *out);
 
+/* --- time-averaged statistics and TKE budget ------------------------------
+ * postproc::solveAveraged0/1/2 (src/postproc.cpp:1186-2110).  type 1: primaries
+ * (10: rho u v w p T e alpha miu coeff) and gradients (21); 2: + derived (28);
+ * 3: + TKE fields (49) and the budget (45); field order of src/postproc.hpp:49-121.
+ * orc_post_update averages the solver's current state (averStep += 1).
+ * orc_post_field: group 0 primary, 1 grads, 2 derived, 3 TKE (averaged), 4 budget. */
+enum { ORC_NPRIM = 10, ORC_NGRAD = 21, ORC_NDER = 28, ORC_NTKE = 49, ORC_NBUD = 45 };
+typedef struct orc_post orc_post;
+orc_post *orc_post_create(const orc_solver *s, int type);
+void orc_post_destroy(orc_post *p);
+void orc_post_update(orc_post *p, orc_solver *s);
+const double *orc_post_field(const orc_post *p, int group, int index);
+long orc_post_steps(const orc_post *p);
+
 #ifdef __cplusplus
 }
 #endif

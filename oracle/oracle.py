@@ -72,6 +72,14 @@ def lib():
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.orc_init_tgv.argtypes = [ctypes.POINTER(OrcConfig), _dp]
         L.orc_stats.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_post_create.restype = ctypes.c_void_p
+        L.orc_post_create.argtypes = [ctypes.c_void_p, ctypes.c_int]
+        L.orc_post_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_post_update.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_post_field.restype = ctypes.POINTER(ctypes.c_double)
+        L.orc_post_field.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int]
+        L.orc_post_steps.restype = ctypes.c_long
+        L.orc_post_steps.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -247,3 +255,38 @@ class Solver:
         out = np.zeros(25)
         lib().orc_stats(self.h, _ptr(out))
         return out
+
+
+POST_COUNTS = (10, 21, 28, 49, 45)   # primary, grads, derived, TKE, budget (src/postproc.hpp:49-121)
+
+
+class Post:
+    """Time-averaged fields and TKE budget of postproc::solveAveraged0/1/2
+    (src/postproc.cpp:1186-2110) over a Solver's successive states."""
+
+    def __init__(self, solver, avg_type=3):
+        self.solver = solver
+        self.N = solver.N
+        self.h = lib().orc_post_create(solver.h, int(avg_type))
+        if not self.h:
+            raise ValueError("averaging type must be 1, 2 or 3")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_post_destroy(self.h)
+            self.h = None
+
+    def update(self):
+        lib().orc_post_update(self.h, self.solver.h)
+
+    def steps(self):
+        return lib().orc_post_steps(self.h)
+
+    def field(self, group, index):
+        ptr = lib().orc_post_field(self.h, int(group), int(index))
+        if not ptr:
+            raise ValueError(f"no field {group}/{index} at this averaging type")
+        return np.ctypeslib.as_array(ptr, shape=(self.N,)).copy()
+
+    def group(self, group):
+        return np.stack([self.field(group, i) for i in range(POST_COUNTS[group])])
