@@ -125,6 +125,34 @@ STAT_NAMES = ["ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "om_x", "om_
               "tau_t", "skewness", "eps_s", "eps_d"]
 
 
+BUDGET_TERMS = ["tke", "production", "transport", "pressure_diffusion", "p1", "p2", "p3", "viscous_diffusion",
+                "v1", "v2", "v3", "convection"]
+
+
+def budget_error(G, n, q, steps=3):
+    """Volume means of the TKE budget terms (nuc_average_*, time averages over
+    `steps` steps after each step, untimed) in FP64 and in the mixed path, and
+    the mixed path's error; terms whose FP64 mean is a divergence (zero on the
+    periodic box) are measured against the budget's scale."""
+    import numpy as np
+    means = {}
+    for prec in (G.FP64, G.MIXED):
+        with G.Context(workload_config(G, n, prec)) as ctx:
+            ctx.upload(q)
+            ctx.average_begin(3)
+            for _ in range(steps):
+                ctx.step(1)
+                ctx.average_update()
+            means[prec] = ctx.average_means(4)[:len(BUDGET_TERMS)]
+            ctx.average_end()
+    b64, bmx = means[G.FP64], means[G.MIXED]
+    scale = float(np.abs(b64[[0, 1, 3, 7]]).max())
+    return {"averaged_steps": steps,
+            "fp64_means": {t: float(b64[i]) for i, t in enumerate(BUDGET_TERMS)},
+            "rel_err_vs_fp64": {t: float(abs(bmx[i] - b64[i]) / max(abs(b64[i]), scale))
+                                for i, t in enumerate(BUDGET_TERMS)}}
+
+
 # compatible jobs packed into one launch up to 65536 cells per pack (pairs of
 # 32^3 boxes): packing beats concurrent streams only for such tiny boxes
 SERVE_PACK_CELLS = 1 << 16
@@ -317,6 +345,7 @@ def main():
                  "rel_err_vs_fp64": {"KE": float(rel[15]), "dissipation": float(abs(epsmx - eps64) / eps64),
                                      **{STAT_NAMES[i]: float(rel[i]) for i in (12, 16, 17, 18, 19, 20, 21, 24, 25, 26)}},
                  "after_steps": args.warmup + args.steps}
+        mixed["tke_budget"] = budget_error(G, n, q)
         if args.profile_slots:
             for k, (t, c) in mprof.items():
                 print(f"  mixed {k:10s} avg {t / max(c, 1):8.4f} ms", file=sys.stderr)

@@ -152,6 +152,28 @@ int nuc_stats(nuc_ctx *ctx, double *out25);
 #define NUC_NSTATS_EX 27
 int nuc_stats_ex(nuc_ctx *ctx, double *out, int nout);
 
+/* ---- time-averaged statistics and TKE budget ------------------------------
+ * postproc::solveAveraged0/1/2 (src/postproc.cpp:1186-2110) on the device, FP64
+ * in both precision paths, field order of src/postproc.hpp:49-121.
+ * nuc_average_begin(type): TimeAverageType 1 = primaries (10: rho u v w p T e
+ *   alpha miu coeff) and gradients (21: d(u,v,w,p,T)/d(x,y,z), vorticity and its
+ *   squares); 2 = + derived (28); 3 = + TKE fields (49) and the TKE budget (45:
+ *   tke, production, transport, pressure diffusion p1 p2 p3, viscous diffusion v1
+ *   v2 v3, convection, then the intermediate fields).  A new average starts from
+ *   zero fields (TimeAverageStartStep 0).
+ * nuc_average_update: one averaging step on the current state:
+ *   f_aver = (f_aver*n + f)/(n + 1); with type 3 also the budget.
+ * nuc_average_field: one field of group 0 primary, 1 grads, 2 derived, 3 TKE
+ *   (all time-averaged) or 4 budget, N doubles.
+ * nuc_average_means: volume means of every field of a group (fixed-order FP64
+ *   reductions); out holds at least the group's field count. */
+int nuc_average_begin(nuc_ctx *ctx, int type);
+int nuc_average_update(nuc_ctx *ctx);
+long long nuc_average_steps(const nuc_ctx *ctx);
+int nuc_average_field(nuc_ctx *ctx, int group, int index, double *out);
+int nuc_average_means(nuc_ctx *ctx, int group, double *out, int nout);
+int nuc_average_end(nuc_ctx *ctx);
+
 /* ---- operator hooks (reference plug-in API, one Field per call) -----------
  * These mirror interoplation::interpolationInner/BoundaryL/BoundaryR
  * (src/fieldOperator.h:13-37), differential::differential* (:58-81) and

@@ -1045,6 +1045,11 @@ struct nuc_ctx {
     double *stat_part = nullptr;
     long stat_nb = 0;
     double *stat_out = nullptr;
+    // time-averaged statistics (nuc_average_*): temporal and averaged fields of
+    // each group, budget fields, reduction partials; type 0 = not running
+    int post_type = 0;
+    long post_step = 0;
+    double *post_buf = nullptr;
     bool forcing = false;
     bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
     // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
@@ -1213,7 +1218,7 @@ extern "C" int nuc_destroy(nuc_ctx *x)
     if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
     if (x->graph) (void)hipGraphDestroy(x->graph);
     void *bufs[] = {x->q[0], x->q[1], x->q[2], x->rhs, x->prim, x->fv, x->lam, x->scal, x->res_part,
-                    x->stat_part, x->stat_out};
+                    x->stat_part, x->stat_out, x->post_buf};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, x->stream);
     if (x->forcing) nucforce::plan_destroy(x->fplan, x->stream);
@@ -1770,6 +1775,453 @@ extern "C" int nuc_profile_read(const nuc_ctx *x, int slot, double *total_ms, lo
     if (total_ms) *total_ms = x->prof_ms[slot];
     if (launches) *launches = x->prof_n[slot];
     if (name) *name = kProfNames[slot];
+    return NUC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// time-averaged statistics and TKE budget: postproc::solveAveraged0/1/2
+// (src/postproc.cpp:1186-2110), field lists of src/postproc.hpp:49-121.
+// All in FP64 on the FP64 state of either precision path.  Each group is one
+// contiguous block of fields (field f of a group at base + f*N); every
+// derivative is solveGrad (src/postproc.cpp:2111-2124,2284-2298): the viscous
+// central difference per direction with the periodic wrap, inner/boundary
+// forms chosen by the rank-local position as in the sweeps.
+// ---------------------------------------------------------------------------
+constexpr int NPRIM = 10, NGRAD = 21, NDER = 28, NTKE = 49, NBUD = 45;
+constexpr int POST_PART = 256;   // reduction partials per field
+
+struct PostArgs {
+    const double GMEM *q;
+    double GMEM *tp, *ap, *tg, *ag, *td, *ad, *tt, *at, *bu;
+    long N;
+    double gamma, mach, re, pr, tref, tinf;
+    int law;
+};
+
+struct PostDiffArgs {
+    const double GMEM *in;
+    double GMEM *out;
+    long N;
+    int nx, ny, d, n, nloc, tsv, vis;
+    double inv_h;
+};
+
+__global__ __launch_bounds__(256) void k_post_diff(PostDiffArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const long i = id % A.nx, r = id / A.nx, j = r % A.ny, k = r / A.ny;
+    long pos, stride;
+    if (A.d == 0) { pos = i; stride = 1; }
+    else if (A.d == 1) { pos = j; stride = A.nx; }
+    else { pos = k; stride = (long)A.nx * A.ny; }
+    const long base = id - pos * stride;
+    double L[9];
+#pragma unroll
+    for (int o = 0; o < 9; o++) {
+        long m = pos + o - 4;
+        m = m < 0 ? m + A.n : (m >= A.n ? m - A.n : m);
+        L[o] = A.in[base + m * stride];
+    }
+    const int cl = (int)(pos % A.nloc);
+    A.out[id] = central_diff<double>(L + 4, 0, cl >= A.tsv && cl < A.nloc - A.tsv, A.vis, A.inv_h);
+}
+
+// f_aver = (f_aver*averStep + f_temp)/(averStep + 1), src/postproc.cpp:1318
+// (kernel parameters are plain pointers: an address-space qualifier would
+// change the device symbol's mangled name but not the host stub's)
+__global__ __launch_bounds__(256) void k_post_avg(double *a, const double *t, long n, double step)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256)
+        a[i] = (a[i] * step + t[i]) / (step + 1.0);
+}
+
+#define PF(base, f) (base)[(long)(f) * A.N + id]
+
+// solveAveraged0: primaries (con2prim, src/physicsModel.cpp:331-344) and getMiu
+__global__ __launch_bounds__(256) void k_post_prim(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = A.q[id];
+    const double u = A.q[A.N + id] / rho, v = A.q[2 * A.N + id] / rho, w = A.q[3 * A.N + id] / rho;
+    const double E = A.q[4 * A.N + id];
+    const double p = fabs((E - 0.5 * rho * (u * u + v * v + w * w)) * (A.gamma - 1.0));
+    const double T = A.gamma * A.mach * A.mach * p / rho;
+    double miu0;
+    const double S = A.tref / A.tinf;
+    if (A.law == 0) miu0 = (1.0 + S) / (T + S) * pow(T, 1.5) / A.re;   // src/physicsModel.cpp:193-266
+    else miu0 = 1.0 / A.re;
+    PF(A.tp, 0) = rho; PF(A.tp, 1) = u; PF(A.tp, 2) = v; PF(A.tp, 3) = w; PF(A.tp, 4) = p; PF(A.tp, 5) = T;
+    PF(A.tp, 6) = p / rho / (A.gamma - 1.0);
+    PF(A.tp, 7) = sqrt(A.gamma * p / rho);
+    PF(A.tp, 8) = miu0;
+    PF(A.tp, 9) = miu0 / ((A.gamma - 1.0) * A.mach * A.mach * A.pr);
+}
+
+// vorticity and its squares, src/postproc.cpp:1296-1302
+__global__ __launch_bounds__(256) void k_post_vort(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double o0 = PF(A.tg, 7) - PF(A.tg, 5), o1 = PF(A.tg, 2) - PF(A.tg, 6), o2 = PF(A.tg, 3) - PF(A.tg, 1);
+    PF(A.tg, 15) = o0; PF(A.tg, 16) = o1; PF(A.tg, 17) = o2;
+    PF(A.tg, 18) = o0 * o0; PF(A.tg, 19) = o1 * o1; PF(A.tg, 20) = o2 * o2;
+}
+
+// solveAveraged1: derived fields, src/postproc.cpp:1414-1465
+__global__ __launch_bounds__(256) void k_post_derived(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = PF(A.tp, 0), u = PF(A.tp, 1), v = PF(A.tp, 2), w = PF(A.tp, 3), e = PF(A.tp, 6);
+    const double miu = PF(A.tp, 8);
+    const double ux = PF(A.tg, 0), uy = PF(A.tg, 1), uz = PF(A.tg, 2), vx = PF(A.tg, 3), vy = PF(A.tg, 4);
+    const double vz = PF(A.tg, 5), wx = PF(A.tg, 6), wy = PF(A.tg, 7), wz = PF(A.tg, 8);
+    PF(A.td, 0) = rho * u; PF(A.td, 1) = rho * v; PF(A.td, 2) = rho * w; PF(A.td, 3) = rho * e;
+    PF(A.td, 4) = u * u; PF(A.td, 5) = u * v; PF(A.td, 6) = u * w;
+    PF(A.td, 7) = v * v; PF(A.td, 8) = v * w; PF(A.td, 9) = w * w;
+    PF(A.td, 10) = rho * u * u; PF(A.td, 11) = rho * u * v; PF(A.td, 12) = rho * u * w;
+    PF(A.td, 13) = rho * v * v; PF(A.td, 14) = rho * v * w; PF(A.td, 15) = rho * w * w;
+    const double div = ux + vy + wz;
+    double t[6];
+    t[0] = miu * (2.0 * ux - 2.0 / 3.0 * div);
+    t[1] = miu * (uy + vx - 2.0 / 3.0 * div);
+    t[2] = miu * (uz + wx - 2.0 / 3.0 * div);
+    t[3] = miu * (2.0 * vy - 2.0 / 3.0 * div);
+    t[4] = miu * (vz + wy - 2.0 / 3.0 * div);
+    t[5] = miu * (2.0 * wz - 2.0 / 3.0 * div);
+#pragma unroll
+    for (int c = 0; c < 6; c++) { PF(A.td, 16 + c) = t[c]; PF(A.td, 22 + c) = rho * t[c]; }
+}
+
+// solveAveraged2: pointwise TKE fields, src/postproc.cpp:1641-1700
+__global__ __launch_bounds__(256) void k_post_tke(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = PF(A.tp, 0), u = PF(A.tp, 1), v = PF(A.tp, 2), w = PF(A.tp, 3), p = PF(A.tp, 4);
+    const double ux = PF(A.tg, 0), uy = PF(A.tg, 1), uz = PF(A.tg, 2), vx = PF(A.tg, 3), vy = PF(A.tg, 4);
+    const double vz = PF(A.tg, 5), wx = PF(A.tg, 6), wy = PF(A.tg, 7), wz = PF(A.tg, 8);
+    const double txx = PF(A.td, 16), txy = PF(A.td, 17), txz = PF(A.td, 18);
+    const double tyy = PF(A.td, 19), tyz = PF(A.td, 20), tzz = PF(A.td, 21);
+    PF(A.tt, 0) = u * p; PF(A.tt, 1) = v * p; PF(A.tt, 2) = w * p;
+    PF(A.tt, 3) = p * ux; PF(A.tt, 4) = p * vy; PF(A.tt, 5) = p * wz;
+    double a = u * txx, b = v * txy, c = w * txz;
+    PF(A.tt, 15) = a; PF(A.tt, 16) = b; PF(A.tt, 17) = c; PF(A.tt, 18) = a + b + c;
+    a = u * txy; b = v * tyy; c = w * tyz;
+    PF(A.tt, 20) = a; PF(A.tt, 21) = b; PF(A.tt, 22) = c; PF(A.tt, 23) = a + b + c;
+    a = u * txz; b = v * tyz; c = w * tzz;
+    PF(A.tt, 25) = a; PF(A.tt, 26) = b; PF(A.tt, 27) = c; PF(A.tt, 28) = a + b + c;
+    PF(A.tt, 30) = txx * ux; PF(A.tt, 31) = txy * vx; PF(A.tt, 32) = txz * wx;
+    PF(A.tt, 33) = txy * uy; PF(A.tt, 34) = tyy * vy; PF(A.tt, 35) = tyz * wy;
+    PF(A.tt, 36) = txz * uz; PF(A.tt, 37) = tyz * vz; PF(A.tt, 38) = tzz * wz;
+    PF(A.tt, 39) = rho * u * u * u; PF(A.tt, 40) = rho * u * u * v; PF(A.tt, 41) = rho * u * u * w;
+    PF(A.tt, 42) = rho * u * v * v; PF(A.tt, 43) = rho * u * v * w; PF(A.tt, 44) = rho * u * w * w;
+    PF(A.tt, 45) = rho * v * v * v; PF(A.tt, 46) = rho * v * v * w; PF(A.tt, 47) = rho * v * w * w;
+    PF(A.tt, 48) = rho * w * w * w;
+}
+
+// budget, stage 1: <u'p'> = <up> - <u><p>, src/postproc.cpp:1829-1833
+__global__ __launch_bounds__(256) void k_post_bud1(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double pa = PF(A.ap, 4);
+    PF(A.bu, 21) = PF(A.at, 0) - PF(A.ap, 1) * pa;
+    PF(A.bu, 22) = PF(A.at, 1) - PF(A.ap, 2) * pa;
+    PF(A.bu, 23) = PF(A.at, 2) - PF(A.ap, 3) * pa;
+}
+
+// budget, stage 2: <u_k><tau_kj>, src/postproc.cpp:1851-1861
+__global__ __launch_bounds__(256) void k_post_bud2(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double u = PF(A.ap, 1), v = PF(A.ap, 2), w = PF(A.ap, 3);
+    const double txx = PF(A.ad, 16), txy = PF(A.ad, 17), txz = PF(A.ad, 18);
+    const double tyy = PF(A.ad, 19), tyz = PF(A.ad, 20), tzz = PF(A.ad, 21);
+    PF(A.bu, 27) = u * txx + v * txy + w * txz;
+    PF(A.bu, 28) = u * txy + v * tyy + w * tyz;
+    PF(A.bu, 29) = u * txz + v * tyz + w * tzz;
+}
+
+// budget, stage 3: tke, production, pressure and viscous diffusion, rho_k u k,
+// src/postproc.cpp:1880-1960
+__global__ __launch_bounds__(256) void k_post_bud3(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = PF(A.ap, 0), u = PF(A.ap, 1), v = PF(A.ap, 2), w = PF(A.ap, 3), pa = PF(A.ap, 4);
+    const double ux = PF(A.ag, 0), uy = PF(A.ag, 1), uz = PF(A.ag, 2), vx = PF(A.ag, 3), vy = PF(A.ag, 4);
+    const double vz = PF(A.ag, 5), wx = PF(A.ag, 6), wy = PF(A.ag, 7), wz = PF(A.ag, 8);
+    const double px = PF(A.ag, 9), py = PF(A.ag, 10), pz = PF(A.ag, 11);
+    const double ru = PF(A.ad, 0), rv = PF(A.ad, 1), rw = PF(A.ad, 2);
+    const double ruu = PF(A.ad, 10), ruv = PF(A.ad, 11), ruw = PF(A.ad, 12);
+    const double rvv = PF(A.ad, 13), rvw = PF(A.ad, 14), rww = PF(A.ad, 15);
+    const double txx = PF(A.ad, 16), txy = PF(A.ad, 17), txz = PF(A.ad, 18);
+    const double tyy = PF(A.ad, 19), tyz = PF(A.ad, 20), tzz = PF(A.ad, 21);
+    const double fu = ru / rho, fv = rv / rho, fw = rw / rho;
+    const double tke = 0.5 * (ruu / rho - fu * fu) + 0.5 * (rvv / rho - fv * fv) + 0.5 * (rww / rho - fw * fw);
+    const double prod = (ruu - ru * ru / rho) * PF(A.bu, 12) + (ruv - ru * rv / rho) * PF(A.bu, 13) +
+                        (ruw - ru * rw / rho) * PF(A.bu, 14) + (ruv - ru * rv / rho) * PF(A.bu, 15) +
+                        (rvv - rv * rv / rho) * PF(A.bu, 16) + (rvw - rv * rw / rho) * PF(A.bu, 17) +
+                        (ruw - ru * rw / rho) * PF(A.bu, 18) + (rvw - rv * rw / rho) * PF(A.bu, 19) +
+                        (rww - rw * rw / rho) * PF(A.bu, 20);
+    const double du = u - fu, dv = v - fv, dw = w - fw;
+    const double p1 = -du * px - dv * py - dw * pz;
+    const double p2 = -PF(A.bu, 24) - PF(A.bu, 25) - PF(A.bu, 26);
+    const double p3 = (PF(A.at, 3) - ux * pa) + (PF(A.at, 4) - vy * pa) + (PF(A.at, 5) - wz * pa);
+    const double v1 = du * PF(A.at, 6) + dv * PF(A.at, 7) + dw * PF(A.at, 8) + du * PF(A.at, 9) + dv * PF(A.at, 10) +
+                      dw * PF(A.at, 11) + du * PF(A.at, 12) + dv * PF(A.at, 13) + dw * PF(A.at, 14);
+    const double v2 = (PF(A.at, 19) - PF(A.bu, 30)) + (PF(A.at, 24) - PF(A.bu, 31)) + (PF(A.at, 29) - PF(A.bu, 32));
+    const double v3 = -(PF(A.at, 30) - ux * txx) - (PF(A.at, 31) - vx * txy) - (PF(A.at, 32) - wx * txz) -
+                      (PF(A.at, 33) - uy * txy) - (PF(A.at, 34) - vy * tyy) - (PF(A.at, 35) - wy * tyz) -
+                      (PF(A.at, 36) - uz * txz) - (PF(A.at, 37) - vz * tyz) - (PF(A.at, 38) - wz * tzz);
+    PF(A.bu, 0) = tke;
+    PF(A.bu, 1) = prod;
+    PF(A.bu, 4) = p1; PF(A.bu, 5) = p2; PF(A.bu, 6) = p3;
+    PF(A.bu, 3) = p1 + p2 + p3;
+    PF(A.bu, 8) = v1; PF(A.bu, 9) = v2; PF(A.bu, 10) = v3;
+    PF(A.bu, 7) = v1 + v2 + v3;
+    PF(A.bu, 33) = ru * tke; PF(A.bu, 34) = rv * tke; PF(A.bu, 35) = rw * tke;
+}
+
+// budget, stage 4: convection and the turbulent transport fluxes,
+// src/postproc.cpp:1973-2004
+__global__ __launch_bounds__(256) void k_post_bud4(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = PF(A.ap, 0);
+    const double ru = PF(A.ad, 0), rv = PF(A.ad, 1), rw = PF(A.ad, 2);
+    const double ruu = PF(A.ad, 10), ruv = PF(A.ad, 11), ruw = PF(A.ad, 12);
+    const double rvv = PF(A.ad, 13), rvw = PF(A.ad, 14), rww = PF(A.ad, 15);
+    PF(A.bu, 11) = -PF(A.bu, 36) - PF(A.bu, 37) - PF(A.bu, 38);
+    const double kk = ruu - ru * ru / rho + rvv - rv * rv / rho + rww - rw * rw / rho;
+    PF(A.bu, 39) = 0.5 * PF(A.at, 39) - ruu * ru / rho + 0.5 * ru * ru * ru / rho / rho +
+                   0.5 * PF(A.at, 42) - ruv * rv / rho + 0.5 * ru * rv * rv / rho / rho +
+                   0.5 * PF(A.at, 44) - ruw * rw / rho + 0.5 * ru * rw * rw / rho / rho - ru / rho * 0.5 * kk;
+    PF(A.bu, 40) = 0.5 * PF(A.at, 40) - ruv * ru / rho + 0.5 * rv * ru * ru / rho / rho +
+                   0.5 * PF(A.at, 45) - rvv * rv / rho + 0.5 * rv * rv * rv / rho / rho +
+                   0.5 * PF(A.at, 47) - rvw * rw / rho + 0.5 * rv * rw * rw / rho / rho - rv / rho * 0.5 * kk;
+    PF(A.bu, 41) = 0.5 * PF(A.at, 41) - ruw * ru / rho + 0.5 * rw * ru * ru / rho / rho +
+                   0.5 * PF(A.at, 46) - rvw * rv / rho + 0.5 * rw * rv * rv / rho / rho +
+                   0.5 * PF(A.at, 48) - rww * rw / rho + 0.5 * rw * rw * rw / rho / rho - rw / rho * 0.5 * kk;
+}
+
+// budget, stage 5: turbulent transport, src/postproc.cpp:2097-2105
+__global__ __launch_bounds__(256) void k_post_bud5(PostArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    PF(A.bu, 2) = -PF(A.bu, 42) - PF(A.bu, 43) - PF(A.bu, 44);
+}
+#undef PF
+
+// volume means of nf consecutive fields: per-field partial sums of a fixed
+// grid, then a fixed-order reduction (deterministic run to run)
+__global__ __launch_bounds__(256) void k_post_mean_part(const double *f, long N, double *part)
+{
+    __shared__ double sm[256];
+    const double *g = f + (long)blockIdx.y * N;
+    double acc = 0.0;
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < N; i += (long)POST_PART * 256) acc += g[i];
+    sm[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(long)blockIdx.y * POST_PART + blockIdx.x] = sm[0];
+}
+
+__global__ __launch_bounds__(256) void k_post_mean_fin(const double *part, double inv_n, double *out)
+{
+    __shared__ double sm[256];
+    sm[threadIdx.x] = part[(long)blockIdx.x * POST_PART + threadIdx.x];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sm[threadIdx.x] += sm[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0] * inv_n;
+}
+
+
+// segment offsets (in doubles, multiples of N) of the averaging buffer
+struct PostLayout {
+    long tp, ap, tg, ag, td, ad, tt, at, bu, part, mean, total;
+};
+
+static PostLayout post_layout(int type, long N)
+{
+    PostLayout L{};
+    long o = 0;
+    auto seg = [&](long nf) { long r = o; o += nf * N; return r; };
+    L.tp = seg(NPRIM); L.ap = seg(NPRIM); L.tg = seg(NGRAD); L.ag = seg(NGRAD);
+    L.td = L.ad = L.tt = L.at = L.bu = -1;
+    if (type >= 2) { L.td = seg(NDER); L.ad = seg(NDER); }
+    if (type >= 3) { L.tt = seg(NTKE); L.at = seg(NTKE); L.bu = seg(NBUD); }
+    L.part = o; o += (long)NTKE * POST_PART;
+    L.mean = o; o += NTKE;
+    L.total = o;
+    return L;
+}
+
+extern "C" int nuc_average_end(nuc_ctx *x)
+{
+    if (!x) return NUC_EINVAL;
+    if (x->post_buf) (void)hipFreeAsync(x->post_buf, x->stream);
+    x->post_buf = nullptr;
+    x->post_type = 0;
+    x->post_step = 0;
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+extern "C" int nuc_average_begin(nuc_ctx *x, int type)
+{
+    if (!x) return NUC_EINVAL;
+    if (type < 1 || type > 3) { x->err = "averaging type must be 1, 2 or 3 (src/postproc.cpp:37-60)"; return NUC_EINVAL; }
+    int rc = nuc_average_end(x);
+    if (rc != NUC_OK) return rc;
+    const PostLayout L = post_layout(type, x->N);
+    if (hipMallocAsync(reinterpret_cast<void **>(&x->post_buf), (size_t)L.total * sizeof(double), x->stream) !=
+        hipSuccess) {
+        x->post_buf = nullptr;
+        x->err = "device allocation (averaged statistics)";
+        return NUC_ENOMEM;
+    }
+    // a new average starts from zero fields (TimeAverageStartStep 0)
+    NUC_CHECK(x, hipMemsetAsync(x->post_buf, 0, (size_t)L.total * sizeof(double), x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    x->post_type = type;
+    x->post_step = 0;
+    return NUC_OK;
+}
+
+extern "C" long long nuc_average_steps(const nuc_ctx *x) { return x ? x->post_step : -1; }
+
+extern "C" int nuc_average_update(nuc_ctx *x)
+{
+    if (!x) return NUC_EINVAL;
+    if (!x->post_type) { x->err = "no average running (nuc_average_begin)"; return NUC_EINVAL; }
+    const long N = x->N;
+    const PostLayout L = post_layout(x->post_type, N);
+    double *b = x->post_buf;
+    const nuc_config &c = x->c;
+    PostArgs A{};
+    A.q = gp(x->q[x->next_stage]);
+    A.tp = gp(b + L.tp); A.ap = gp(b + L.ap); A.tg = gp(b + L.tg); A.ag = gp(b + L.ag);
+    if (x->post_type >= 2) { A.td = gp(b + L.td); A.ad = gp(b + L.ad); }
+    if (x->post_type >= 3) { A.tt = gp(b + L.tt); A.at = gp(b + L.at); A.bu = gp(b + L.bu); }
+    A.N = N;
+    A.gamma = c.gamma; A.mach = c.mach; A.re = c.reynolds; A.pr = c.prandtl; A.tref = c.t_ref; A.tinf = c.t_inf;
+    A.law = c.viscosity;
+    const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
+    auto diff = [&](const double *in, double *out, int d) {
+        PostDiffArgs D{gp(in), gp(out), N, x->n[0], x->n[1], d, x->n[d], x->nloc[d], x->tsv[d],
+                       c.vis_scheme[d] == NUC_CD8 ? VIS_CD8 : VIS_CD6, 1.0 / x->h[d]};
+        hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
+    };
+    auto F = [&](long seg, int f) { return b + seg + (long)f * N; };
+    const double step = (double)x->post_step;
+    auto avg = [&](long a_seg, long t_seg, int nf) {
+        long n = (long)nf * N;
+        long nb = std::min<long>((n + 255) / 256, 65536);
+        hipLaunchKernelGGL(k_post_avg, dim3((unsigned)nb), blk, 0, x->stream, b + a_seg, (const double *)(b + t_seg), n,
+                           step);
+    };
+    // solveAveraged0 (src/postproc.cpp:1207-1349)
+    hipLaunchKernelGGL(k_post_prim, grid, blk, 0, x->stream, A);
+    for (int f = 0; f < 5; f++)            // d(u, v, w, p, T)/d(x, y, z)
+        for (int d = 0; d < 3; d++) diff(F(L.tp, 1 + f), F(L.tg, 3 * f + d), d);
+    hipLaunchKernelGGL(k_post_vort, grid, blk, 0, x->stream, A);
+    avg(L.ap, L.tp, NPRIM);
+    avg(L.ag, L.tg, NGRAD);
+    if (x->post_type >= 2) {              // solveAveraged1 (src/postproc.cpp:1351-1483)
+        hipLaunchKernelGGL(k_post_derived, grid, blk, 0, x->stream, A);
+        avg(L.ad, L.td, NDER);
+    }
+    if (x->post_type >= 3) {              // solveAveraged2 (src/postproc.cpp:1485-2110)
+        diff(F(L.td, 16), F(L.tt, 6), 0);
+        diff(F(L.td, 17), F(L.tt, 7), 0);
+        diff(F(L.td, 18), F(L.tt, 8), 0);
+        diff(F(L.td, 17), F(L.tt, 9), 1);
+        diff(F(L.td, 19), F(L.tt, 10), 1);
+        diff(F(L.td, 20), F(L.tt, 11), 1);
+        diff(F(L.td, 18), F(L.tt, 12), 2);
+        diff(F(L.td, 20), F(L.tt, 13), 2);
+        diff(F(L.td, 21), F(L.tt, 14), 2);
+        hipLaunchKernelGGL(k_post_tke, grid, blk, 0, x->stream, A);
+        diff(F(L.tt, 18), F(L.tt, 19), 0);
+        diff(F(L.tt, 23), F(L.tt, 24), 1);
+        diff(F(L.tt, 28), F(L.tt, 29), 2);
+        avg(L.at, L.tt, NTKE);
+        for (int c3 = 0; c3 < 3; c3++)
+            for (int d = 0; d < 3; d++) diff(F(L.ad, c3), F(L.bu, 12 + 3 * c3 + d), d);
+        hipLaunchKernelGGL(k_post_bud1, grid, blk, 0, x->stream, A);
+        for (int d = 0; d < 3; d++) diff(F(L.bu, 21 + d), F(L.bu, 24 + d), d);
+        hipLaunchKernelGGL(k_post_bud2, grid, blk, 0, x->stream, A);
+        for (int d = 0; d < 3; d++) diff(F(L.bu, 27 + d), F(L.bu, 30 + d), d);
+        hipLaunchKernelGGL(k_post_bud3, grid, blk, 0, x->stream, A);
+        for (int d = 0; d < 3; d++) diff(F(L.bu, 33 + d), F(L.bu, 36 + d), d);
+        hipLaunchKernelGGL(k_post_bud4, grid, blk, 0, x->stream, A);
+        for (int d = 0; d < 3; d++) diff(F(L.bu, 39 + d), F(L.bu, 42 + d), d);
+        hipLaunchKernelGGL(k_post_bud5, grid, blk, 0, x->stream, A);
+    }
+    NUC_CHECK(x, hipGetLastError());
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    x->post_step++;
+    return NUC_OK;
+}
+
+static long post_group(const nuc_ctx *x, int group, int *count)
+{
+    static const int cnt[5] = {NPRIM, NGRAD, NDER, NTKE, NBUD};
+    if (!x->post_type || group < 0 || group > 4) return -1;
+    const PostLayout L = post_layout(x->post_type, x->N);
+    const long seg[5] = {L.ap, L.ag, L.ad, L.at, L.bu};
+    if (seg[group] < 0) return -1;
+    *count = cnt[group];
+    return seg[group];
+}
+
+extern "C" int nuc_average_field(nuc_ctx *x, int group, int index, double *out)
+{
+    if (!x || !out) return NUC_EINVAL;
+    int count = 0;
+    const long seg = post_group(x, group, &count);
+    if (seg < 0 || index < 0 || index >= count) {
+        x->err = "no such averaged field at this averaging type";
+        return NUC_EINVAL;
+    }
+    NUC_CHECK(x, hipMemcpyAsync(out, x->post_buf + seg + (long)index * x->N, x->N * sizeof(double),
+                                hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+extern "C" int nuc_average_means(nuc_ctx *x, int group, double *out, int nout)
+{
+    if (!x || !out) return NUC_EINVAL;
+    int count = 0;
+    const long seg = post_group(x, group, &count);
+    if (seg < 0 || nout < count) {
+        x->err = "no such averaged group at this averaging type, or out too short";
+        return NUC_EINVAL;
+    }
+    const PostLayout L = post_layout(x->post_type, x->N);
+    double *b = x->post_buf;
+    hipLaunchKernelGGL(k_post_mean_part, dim3(POST_PART, count), dim3(256), 0, x->stream, (const double *)(b + seg),
+                       x->N, b + L.part);
+    hipLaunchKernelGGL(k_post_mean_fin, dim3(count), dim3(256), 0, x->stream, (const double *)(b + L.part),
+                       1.0 / (double)x->N, b + L.mean);
+    NUC_CHECK(x, hipGetLastError());
+    NUC_CHECK(x, hipMemcpyAsync(out, b + L.mean, count * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }
 

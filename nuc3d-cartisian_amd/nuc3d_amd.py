@@ -36,7 +36,9 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
            "nuc_server_stats_get", "nuc_server_now", "nuc_server_job_error",
            "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
-           "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage"]
+           "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage",
+           "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
+           "nuc_average_means", "nuc_average_end"]
 
 
 class NucConfig(ctypes.Structure):
@@ -135,6 +137,13 @@ def lib():
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
         L.nuc_autotune.argtypes = [vp]
+        L.nuc_average_begin.argtypes = [vp, ctypes.c_int]
+        L.nuc_average_update.argtypes = [vp]
+        L.nuc_average_steps.argtypes = [vp]
+        L.nuc_average_steps.restype = ctypes.c_longlong
+        L.nuc_average_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
+        L.nuc_average_means.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
+        L.nuc_average_end.argtypes = [vp]
         L.nuc_rk3_stage.argtypes = [vp, ctypes.c_int]
         L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
@@ -266,6 +275,34 @@ class Context:
 
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    # time-averaged statistics and TKE budget (nuc_average_*)
+    AVG_COUNTS = (10, 21, 28, 49, 45)
+
+    def average_begin(self, avg_type=3):
+        _check(lib().nuc_average_begin(self.h, int(avg_type)), "nuc_average_begin", self.h)
+
+    def average_update(self):
+        _check(lib().nuc_average_update(self.h), "nuc_average_update", self.h)
+
+    def average_steps(self):
+        return lib().nuc_average_steps(self.h)
+
+    def average_field(self, group, index):
+        out = np.zeros(self.N)
+        _check(lib().nuc_average_field(self.h, int(group), int(index), _ptr(out)), "nuc_average_field", self.h)
+        return out
+
+    def average_group(self, group):
+        return np.stack([self.average_field(group, i) for i in range(self.AVG_COUNTS[group])])
+
+    def average_means(self, group):
+        out = np.zeros(self.AVG_COUNTS[group])
+        _check(lib().nuc_average_means(self.h, int(group), _ptr(out), out.size), "nuc_average_means", self.h)
+        return out
+
+    def average_end(self):
+        _check(lib().nuc_average_end(self.h), "nuc_average_end", self.h)
 
     def rk3_stage(self, stage):
         """One RK stage (nuc_rk3_stage); stages run in order 0, 1, 2."""
