@@ -358,11 +358,11 @@ def main():
         # the autotuner's per-shape cache, as a running server holds it for
         # shapes it has seen: one tuning per distinct box shape, untimed
         seen = set()
-        for cfg, q, nj in jobs:
+        for jcfg, jq, nj in jobs:
             if nj not in seen:
                 seen.add(nj)
-                with G.Context(cfg) as tctx:
-                    tctx.upload(q)
+                with G.Context(jcfg) as tctx:
+                    tctx.upload(jq)
                     tctx.autotune()
         sst = {}
         wall_sat, _ = run_serving(G, jobs, stats=sst)
