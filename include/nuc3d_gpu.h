@@ -223,6 +223,13 @@ int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsigned long l
  *   gamma^-1/2 the temperature T = gamma M^2 p / rho of src/physicsModel.cpp:342
  *   equals p / rho. */
 int nuc_init_shu_osher(const nuc_config *cfg, double pert, unsigned long long seed, double *q);
+/* The same fields generated on the device into ctx's state (a new step starts):
+ * nuc_init_tgv_device is nuc_init_tgv; nuc_init_hit_device synthesises
+ * nuc_init_hit's seeded modes with a separable inverse DFT on the device (equal
+ * to the host field up to rounding, ~1e-15 relative) -- for large boxes and for
+ * jobs whose initial state should not cross PCIe. */
+int nuc_init_tgv_device(nuc_ctx *ctx);
+int nuc_init_hit_device(nuc_ctx *ctx, double k0, double u_rms, unsigned long long seed);
 
 #ifdef __cplusplus
 }

@@ -19,6 +19,7 @@
 #include "nuc_device.h"
 #include "nuc_lines.h"
 #include "nuc_force.h"
+#include "../host/hit_spectrum.hpp"
 
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -2221,6 +2222,216 @@ extern "C" int nuc_average_means(nuc_ctx *x, int group, double *out, int nout)
                        1.0 / (double)x->N, b + L.mean);
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipMemcpyAsync(out, b + L.mean, count * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// initial conditions on the device: the decaying-turbulence field of
+// initial_hit (src/block.cpp:397-551, the host generator's seeded spectrum)
+// synthesised by a separable inverse DFT in three passes (x, then y, then z
+// keeping the real part), rescaled to u_rms with a fixed-order reduction; and
+// the Taylor-Green field of initial_taylorgreen (src/block.cpp:359-395)
+// ---------------------------------------------------------------------------
+struct IcArgs {
+    const double2 GMEM *spec[3];   // mode amplitudes per velocity component
+    const double2 GMEM *tw[3];     // twiddles e^{+i 2 pi k x / n}, [(k+K)*n + x]
+    double2 GMEM *g;               // x pass: [c][kz][ky][x]
+    double2 GMEM *h;               // y pass: [c][kz][y][x]
+    double GMEM *vel;              // z pass: [c][cell]
+    double GMEM *q;                // state out
+    const double GMEM *mean_sq;    // <|u|^2> of the synthesised field
+    int n[3], W[3];
+    long N;
+    double gamma, mach, scale_num;
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__global__ __launch_bounds__(256) void k_ic_x(IcArgs A)
+{
+    const long tot = 3L * A.W[2] * A.W[1] * A.n[0];
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= tot) return;
+    const int x = (int)(id % A.n[0]);
+    const long r = id / A.n[0];
+    const long kzky = r % ((long)A.W[2] * A.W[1]);
+    const int c = (int)(r / ((long)A.W[2] * A.W[1]));
+    const double2 GMEM *sp = A.spec[c] + kzky * A.W[0];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int kx = 0; kx < A.W[0]; kx++) {
+        const double2 t = cmul(sp[kx], A.tw[0][(long)kx * A.n[0] + x]);
+        acc.x += t.x; acc.y += t.y;
+    }
+    A.g[id] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_ic_y(IcArgs A)
+{
+    const long tot = 3L * A.W[2] * A.n[1] * A.n[0];
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= tot) return;
+    const int x = (int)(id % A.n[0]);
+    long r = id / A.n[0];
+    const int y = (int)(r % A.n[1]);
+    r /= A.n[1];                                  // c * W2 + kz
+    double2 acc = make_double2(0.0, 0.0);
+    for (int ky = 0; ky < A.W[1]; ky++) {
+        const double2 t = cmul(A.g[(r * A.W[1] + ky) * A.n[0] + x], A.tw[1][(long)ky * A.n[1] + y]);
+        acc.x += t.x; acc.y += t.y;
+    }
+    A.h[id] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_ic_z(IcArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= 3 * A.N) return;
+    const int c = (int)(id / A.N);
+    const long cell = id % A.N;
+    const long nxy = (long)A.n[0] * A.n[1];
+    const int z = (int)(cell / nxy);
+    const long xy = cell % nxy;
+    double acc = 0.0;
+    for (int kz = 0; kz < A.W[2]; kz++) {
+        const double2 hv = A.h[((long)c * A.W[2] + kz) * nxy + xy];
+        const double2 t = A.tw[2][(long)kz * A.n[2] + z];
+        acc += hv.x * t.x - hv.y * t.y;
+    }
+    A.vel[id] = acc;
+}
+
+// |u|^2 per cell into the state's rho slot, reduced by the post mean kernels
+__global__ __launch_bounds__(256) void k_ic_sq(IcArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double u = A.vel[id], v = A.vel[A.N + id], w = A.vel[2 * A.N + id];
+    A.q[id] = u * u + v * v + w * w;
+}
+
+// rho = 1, p = 1/(gamma M^2), velocity rescaled to u_rms: scale = u_rms / sqrt(<|u|^2>/3)
+__global__ __launch_bounds__(256) void k_ic_q(IcArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double v0 = sqrt(*A.mean_sq / 3.0);
+    const double scale = (v0 > 0) ? A.scale_num / v0 : 0.0;
+    const double u = A.vel[id] * scale, v = A.vel[A.N + id] * scale, w = A.vel[2 * A.N + id] * scale;
+    const double p0 = 1.0 / (A.gamma * A.mach * A.mach);
+    A.q[id] = 1.0;
+    A.q[A.N + id] = u;
+    A.q[2 * A.N + id] = v;
+    A.q[3 * A.N + id] = w;
+    A.q[4 * A.N + id] = p0 / (A.gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
+}
+
+struct TgvArgs {
+    double GMEM *q;
+    int nx, ny, nz, nl[3];
+    long N;
+    double dx, dy, dz, lxr, lyr, lzr, gamma, mach;   // cell sizes, rank box sizes
+};
+
+__global__ __launch_bounds__(256) void k_ic_tgv(TgvArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const int i = (int)(id % A.nx);
+    const long r = id / A.nx;
+    const int j = (int)(r % A.ny), k = (int)(r / A.ny);
+    // cell centres per rank, src/block.cpp:116-118
+    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]);
+    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]);
+    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]);
+    const double g = A.gamma, m = A.mach;
+    const double u0 = sin(x0) * cos(y0) * cos(z0);
+    const double v0 = -cos(x0) * sin(y0) * cos(z0);
+    const double p0 = 1.0 / (g * m * m) + ((cos(2.0 * z0) + 2.0) * (cos(2.0 * x0) + cos(2.0 * y0)) - 2.0) / 16.0;
+    A.q[id] = 1.0;
+    A.q[A.N + id] = u0;
+    A.q[2 * A.N + id] = v0;
+    A.q[3 * A.N + id] = 0.0;
+    A.q[4 * A.N + id] = p0 / (g - 1.0) + 0.5 * (u0 * u0 + v0 * v0);
+}
+
+extern "C" int nuc_init_tgv_device(nuc_ctx *x)
+{
+    if (!x) return NUC_EINVAL;
+    const nuc_config &c = x->c;
+    TgvArgs A{};
+    A.q = gp(x->q[0]);
+    A.nx = c.nx; A.ny = c.ny; A.nz = c.nz;
+    for (int d = 0; d < 3; d++) A.nl[d] = x->nloc[d];
+    A.N = x->N;
+    A.dx = c.lx / c.nx; A.dy = c.ly / c.ny; A.dz = c.lz / c.nz;
+    A.lxr = c.lx / c.npx; A.lyr = c.ly / c.npy; A.lzr = c.lz / c.npz;
+    A.gamma = c.gamma; A.mach = c.mach;
+    x->next_stage = 0;
+    hipLaunchKernelGGL(k_ic_tgv, dim3((unsigned)((x->N + 255) / 256)), dim3(256), 0, x->stream, A);
+    NUC_CHECK(x, hipGetLastError());
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    return NUC_OK;
+}
+
+extern "C" int nuc_init_hit_device(nuc_ctx *x, double k0, double u_rms, unsigned long long seed)
+{
+    if (!x || !(k0 > 0) || !(u_rms >= 0)) return NUC_EINVAL;
+    nuc3d::HitSpectrum S;
+    nuc3d::hit_spectrum(x->c, k0, seed, S);
+    const long N = x->N;
+    const size_t nspec = (size_t)S.W[0] * S.W[1] * S.W[2];
+    const size_t ng = 3ull * S.W[2] * S.W[1] * S.n[0], nh = 3ull * S.W[2] * S.n[1] * S.n[0];
+    size_t ntw = 0;
+    for (int d = 0; d < 3; d++) ntw += S.tw[d].size();
+    // one scratch allocation: spectra, twiddles, both passes, the velocity, partials
+    const size_t n2 = 3 * nspec + ntw + ng + nh;            // double2 entries
+    const size_t bytes = n2 * sizeof(double2) + (3 * (size_t)N + POST_PART + 1) * sizeof(double);
+    char *buf = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void **>(&buf), bytes, x->stream) != hipSuccess) {
+        x->err = "device allocation (initial condition)";
+        return NUC_ENOMEM;
+    }
+    double2 *spec = reinterpret_cast<double2 *>(buf);
+    double2 *tw = spec + 3 * nspec;
+    double2 *g = tw + ntw;
+    double2 *h = g + ng;
+    double *vel = reinterpret_cast<double *>(h + nh);
+    double *part = vel + 3 * N;
+    double *mean = part + POST_PART;
+    IcArgs A{};
+    for (int cc = 0; cc < 3; cc++) {
+        NUC_CHECK(x, hipMemcpyAsync(spec + cc * nspec, S.spec[cc].data(), nspec * sizeof(double2),
+                                    hipMemcpyHostToDevice, x->stream));
+        A.spec[cc] = gp((const double2 *)(spec + cc * nspec));
+    }
+    size_t o = 0;
+    for (int d = 0; d < 3; d++) {
+        NUC_CHECK(x, hipMemcpyAsync(tw + o, S.tw[d].data(), S.tw[d].size() * sizeof(double2), hipMemcpyHostToDevice,
+                                    x->stream));
+        A.tw[d] = gp((const double2 *)(tw + o));
+        o += S.tw[d].size();
+    }
+    A.g = gp(g); A.h = gp(h); A.vel = gp(vel); A.q = gp(x->q[0]);
+    for (int d = 0; d < 3; d++) { A.n[d] = S.n[d]; A.W[d] = S.W[d]; }
+    A.N = N;
+    A.gamma = x->c.gamma; A.mach = x->c.mach; A.scale_num = u_rms;
+    x->next_stage = 0;
+    auto blocks = [](size_t n) { return dim3((unsigned)((n + 255) / 256)); };
+    hipLaunchKernelGGL(k_ic_x, blocks(ng), dim3(256), 0, x->stream, A);
+    hipLaunchKernelGGL(k_ic_y, blocks(nh), dim3(256), 0, x->stream, A);
+    hipLaunchKernelGGL(k_ic_z, blocks(3 * (size_t)N), dim3(256), 0, x->stream, A);
+    hipLaunchKernelGGL(k_ic_sq, blocks((size_t)N), dim3(256), 0, x->stream, A);
+    hipLaunchKernelGGL(k_post_mean_part, dim3(POST_PART, 1), dim3(256), 0, x->stream, (const double *)x->q[0], N, part);
+    hipLaunchKernelGGL(k_post_mean_fin, dim3(1), dim3(256), 0, x->stream, (const double *)part, 1.0 / (double)N, mean);
+    A.mean_sq = gp((const double *)mean);
+    hipLaunchKernelGGL(k_ic_q, blocks((size_t)N), dim3(256), 0, x->stream, A);
+    NUC_CHECK(x, hipGetLastError());
+    (void)hipFreeAsync(buf, x->stream);
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }

@@ -1,6 +1,7 @@
 // nuc3d_host.cpp -- host classes of nuc3d_host.hpp and host utilities
 // (initial conditions, restart files) exported through the C ABI.
 #include "nuc3d_host.hpp"
+#include "hit_spectrum.hpp"
 
 #include <cmath>
 #include <complex>
@@ -211,18 +212,20 @@ typedef std::complex<double> cplx;
 // FFTW's backward sign (e^{+i}) and no normalisation, keeping the real part
 // (src/block.cpp:504-510).  Phases come from mt19937_64(seed), three draws per
 // mode in (kz, ky, kx) order, instead of std::random_device (src/block.cpp:416-418).
-void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long long seed, std::vector<double> &q)
+void hit_spectrum(const nuc_config &c, double k0, unsigned long long seed, HitSpectrum &S)
 {
-    const int n[3] = {c.nx, c.ny, c.nz};
-    const long N = (long)n[0] * n[1] * n[2];
+    S.n[0] = c.nx; S.n[1] = c.ny; S.n[2] = c.nz;
+    const int *n = S.n;
     const double pie = 4.0 * std::atan(1.0);
-    int K[3];
-    for (int d = 0; d < 3; d++) K[d] = std::max(1, std::min(n[d] / 2 - 1, (int)std::ceil(4.0 * k0)));
-    const int W[3] = {2 * K[0] + 1, 2 * K[1] + 1, 2 * K[2] + 1};
+    int *K = S.K, *W = S.W;
+    for (int d = 0; d < 3; d++) {
+        K[d] = std::max(1, std::min(n[d] / 2 - 1, (int)std::ceil(4.0 * k0)));
+        W[d] = 2 * K[d] + 1;
+    }
     std::mt19937_64 gen(seed);
     std::uniform_real_distribution<double> dis(0.0, 2.0 * pie);
     // spectral amplitudes, comp c at [(kz*W1 + ky)*W0 + kx]
-    std::vector<cplx> spec[3];
+    std::vector<cplx> *spec = S.spec;
     for (int cc = 0; cc < 3; cc++) spec[cc].assign((size_t)W[0] * W[1] * W[2], 0.0);
     for (int kz = -K[2]; kz <= K[2]; kz++)
         for (int ky = -K[1]; ky <= K[1]; ky++)
@@ -248,7 +251,7 @@ void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long lon
                 spec[2][id] = -k12 / kmag * b;
             }
     // twiddles e^{+i 2 pi k x / n}
-    std::vector<cplx> tw[3];
+    std::vector<cplx> *tw = S.tw;
     for (int d = 0; d < 3; d++) {
         tw[d].resize((size_t)W[d] * n[d]);
         for (int k = -K[d]; k <= K[d]; k++)
@@ -259,6 +262,15 @@ void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long lon
                 tw[d][(size_t)(k + K[d]) * n[d] + x] = cplx(std::cos(ang), std::sin(ang));
             }
     }
+}
+
+void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long long seed, std::vector<double> &q)
+{
+    HitSpectrum S;
+    hit_spectrum(c, k0, seed, S);
+    const int *n = S.n, *W = S.W;
+    const std::vector<cplx> *spec = S.spec, *tw = S.tw;
+    const long N = (long)n[0] * n[1] * n[2];
     std::vector<double> vel[3];
     for (int cc = 0; cc < 3; cc++) {
         // x pass: g[kz][ky][x]

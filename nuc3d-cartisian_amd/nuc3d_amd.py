@@ -38,7 +38,7 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
            "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage",
            "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
-           "nuc_average_means", "nuc_average_end"]
+           "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device"]
 
 
 class NucConfig(ctypes.Structure):
@@ -137,6 +137,8 @@ def lib():
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
         L.nuc_autotune.argtypes = [vp]
+        L.nuc_init_tgv_device.argtypes = [vp]
+        L.nuc_init_hit_device.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]
         L.nuc_average_begin.argtypes = [vp, ctypes.c_int]
         L.nuc_average_update.argtypes = [vp]
         L.nuc_average_steps.argtypes = [vp]
@@ -275,6 +277,14 @@ class Context:
 
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    def init_tgv_device(self):
+        """The Taylor-Green field generated on the device into the state."""
+        _check(lib().nuc_init_tgv_device(self.h), "nuc_init_tgv_device", self.h)
+
+    def init_hit_device(self, k0=4.0, u_rms=1.0, seed=20261202):
+        """init_hit's seeded field synthesised on the device into the state."""
+        _check(lib().nuc_init_hit_device(self.h, float(k0), float(u_rms), int(seed)), "nuc_init_hit_device", self.h)
 
     # time-averaged statistics and TKE budget (nuc_average_*)
     AVG_COUNTS = (10, 21, 28, 49, 45)
