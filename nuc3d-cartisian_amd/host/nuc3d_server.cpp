@@ -62,6 +62,10 @@ struct Job {
     std::string err;
     long long cells = 0;
     bool packable = false;
+    // initial state generated on the device at admission instead of q
+    int ic = NUC_IC_STATE;
+    double ic_k0 = 0.0, ic_urms = 0.0;
+    unsigned long long ic_seed = 0;
 };
 
 struct PackRec {
@@ -166,7 +170,10 @@ void nuc_server::run()
             if (cfg.use_graphs) nuc_set_graph(j->ctx, 1);
             j->cells = (long long)j->cfg.nx * j->cfg.ny * j->cfg.nz;
             j->packable = nuc_pack_compatible(j->ctx, j->ctx) != 0;
-            rc = nuc_upload_state(j->ctx, j->q.data());
+            if (j->ic == NUC_IC_TGV) rc = nuc_init_tgv_device(j->ctx);
+            else if (j->ic == NUC_IC_HIT) rc = nuc_init_hit_device(j->ctx, j->ic_k0, j->ic_urms, j->ic_seed);
+            else rc = nuc_upload_state(j->ctx, j->q.data());
+            if (j->ic != NUC_IC_STATE) j->q.assign((size_t)5 * j->cfg.nx * j->cfg.ny * j->cfg.nz, 0.0);
             if (rc == NUC_OK && cfg.autotune) rc = nuc_autotune(j->ctx);
             if (rc != NUC_OK) {
                 finish(j, NUC_JOB_FAILED, rc, nuc_last_error(j->ctx));
@@ -333,6 +340,8 @@ extern "C" int nuc_server_destroy(nuc_server *s)
     return NUC_OK;
 }
 
+static int submit_job(nuc_server *s, const std::shared_ptr<Job> &j, long long *job_id);
+
 extern "C" int nuc_server_submit(nuc_server *s, const nuc_config *cfg, const double *q0, int nsteps,
                                  long long *job_id)
 {
@@ -343,6 +352,28 @@ extern "C" int nuc_server_submit(nuc_server *s, const nuc_config *cfg, const dou
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return NUC_EINVAL;
     j->q.assign(q0, q0 + n);
     j->nsteps = nsteps;
+    return submit_job(s, j, job_id);
+}
+
+extern "C" int nuc_server_submit_ic(nuc_server *s, const nuc_config *cfg, int ic, double k0, double u_rms,
+                                    unsigned long long seed, int nsteps, long long *job_id)
+{
+    if (!s || !cfg || nsteps < 0 || !job_id || (ic != NUC_IC_TGV && ic != NUC_IC_HIT)) return NUC_EINVAL;
+    if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return NUC_EINVAL;
+    if (ic == NUC_IC_HIT && (!(k0 > 0) || !(u_rms >= 0))) return NUC_EINVAL;
+    auto j = std::make_shared<Job>();
+    j->cfg = *cfg;
+    j->ic = ic;
+    j->ic_k0 = k0;
+    j->ic_urms = u_rms;
+    j->ic_seed = seed;
+    j->nsteps = nsteps;
+    return submit_job(s, j, job_id);
+}
+
+static int submit_job(nuc_server *s, const std::shared_ptr<Job> &j, long long *job_id)
+{
+    const nuc_config *cfg = &j->cfg;
     j->bytes = nuc_footprint(cfg);
     {
         std::lock_guard<std::mutex> lk(s->mu);

@@ -27,6 +27,7 @@ STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_
           6: "NUC_EBUSY"}
 NUC_OK, NUC_EBUSY = 0, 6
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
+IC_STATE, IC_TGV, IC_HIT = range(3)
 
 # every entry point include/nuc3d_gpu.h and include/nuc3d_serve.h declare
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
@@ -38,7 +39,8 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_pack_compatible", "nuc_pack_create", "nuc_pack_step_async", "nuc_pack_destroy",
            "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage",
            "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
-           "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device"]
+           "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device",
+           "nuc_server_submit_ic"]
 
 
 class NucConfig(ctypes.Structure):
@@ -129,6 +131,9 @@ def lib():
         L.nuc_server_destroy.argtypes = [vp]
         L.nuc_server_submit.argtypes = [vp, ctypes.POINTER(NucConfig), _dp, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_longlong)]
+        L.nuc_server_submit_ic.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_ulonglong, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_longlong)]
         L.nuc_server_result.argtypes = [vp, ctypes.c_longlong, _dp, ctypes.POINTER(JobResult)]
         L.nuc_server_drain.argtypes = [vp, ctypes.c_double]
         L.nuc_server_stats_get.argtypes = [vp, ctypes.POINTER(ServerStats)]
@@ -487,6 +492,14 @@ class Server:
         _check(lib().nuc_server_submit(self.h, ctypes.byref(cfg), _ptr(q0), int(nsteps), ctypes.byref(jid)),
                "nuc_server_submit")
         self._shapes[jid.value] = q0.size
+        return jid.value
+
+    def submit_ic(self, cfg, ic, nsteps, k0=4.0, u_rms=1.0, seed=0):
+        """A job whose initial state (IC_TGV or IC_HIT) is generated on the device."""
+        jid = ctypes.c_longlong()
+        _check(lib().nuc_server_submit_ic(self.h, ctypes.byref(cfg), int(ic), float(k0), float(u_rms), int(seed),
+                                          int(nsteps), ctypes.byref(jid)), "nuc_server_submit_ic")
+        self._shapes[jid.value] = 5 * cfg.nx * cfg.ny * cfg.nz
         return jid.value
 
     def result(self, jid, want_state=True):

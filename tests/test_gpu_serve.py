@@ -177,3 +177,22 @@ def test_autotuned_tile_shape_keeps_results():
             assert ctx.tile_shape() == planes
             ctx.step(3)
             assert np.array_equal(ctx.download(), ref), planes
+
+
+def test_jobs_with_device_initial_conditions():
+    """Jobs submitted as an initial-condition spec (no host state): the result
+    equals the same job started from the device-generated state alone."""
+    a = G.make_config(n=(32, 32, 32), scheme=G.CRWENO5, vis=G.CD6, mach=0.2, reynolds=900.0, cfl=0.5)
+    b = G.make_config(n=(40, 36, 32), scheme=G.WENO5Z, vis=G.CD6, mach=0.3, reynolds=600.0, cfl=0.5)
+    with G.Server(hbm_budget=4 << 30, max_active=4) as srv:
+        ja = srv.submit_ic(a, G.IC_HIT, 4, k0=3.0, u_rms=0.5, seed=12)
+        jb = srv.submit_ic(b, G.IC_TGV, 4)
+        ra, rb = _wait(srv, ja), _wait(srv, jb)
+    for cfg, res, init in ((a, ra, lambda c: c.init_hit_device(k0=3.0, u_rms=0.5, seed=12)),
+                           (b, rb, lambda c: c.init_tgv_device())):
+        with G.Context(cfg) as ctx:
+            init(ctx)
+            ctx.step(4)
+            ref = ctx.download()
+        assert res[1].state == G.JOB_DONE
+        assert np.array_equal(res[0], ref)
