@@ -174,6 +174,21 @@ int nuc_average_field(nuc_ctx *ctx, int group, int index, double *out);
 int nuc_average_means(nuc_ctx *ctx, int group, double *out, int nout);
 int nuc_average_end(nuc_ctx *ctx);
 
+/* ---- shell spectra ----------------------------------------------------------
+ * postproc::solveFFT_* (src/postproc.cpp:931-1184) with fft::solveShell
+ * (src/fft.cpp:384-572) of the current state; kmax = min(nx, ny, nz)/2 shells,
+ * the forward transform e^{-i} divided by N.
+ * nuc_spectra_kinetic: out[s*kmax + k], s = 0 total, 1 compressive,
+ *   2 solenoidal, 3 left- and 4 right-helical kinetic energy of shell k (signed
+ *   wave numbers; k = 0 counted in shell 1 as written).
+ * nuc_spectrum_scalar: |f^|^2 of shell k for density, pressure, enstrophy
+ *   0.5|omega|^2 or helicity omega.u, over the non-negative index octant as the
+ *   reference's solveShell counts it.
+ * Shell sums are FP64; their order within a workgroup is not fixed. */
+enum nuc_spectrum_field { NUC_SPEC_DENSITY = 0, NUC_SPEC_PRESSURE = 1, NUC_SPEC_ENSTROPHY = 2, NUC_SPEC_HELICITY = 3 };
+int nuc_spectra_kinetic(nuc_ctx *ctx, double *out, int kmax);
+int nuc_spectrum_scalar(nuc_ctx *ctx, int field, double *out, int kmax);
+
 /* ---- operator hooks (reference plug-in API, one Field per call) -----------
  * These mirror interoplation::interpolationInner/BoundaryL/BoundaryR
  * (src/fieldOperator.h:13-37), differential::differential* (:58-81) and

@@ -22,6 +22,7 @@
 #include "../host/hit_spectrum.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -2434,6 +2435,243 @@ extern "C" int nuc_init_hit_device(nuc_ctx *x, double k0, double u_rms, unsigned
     (void)hipFreeAsync(buf, x->stream);
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// shell spectra: postproc::solveFFT_kinetic/density/pressure/enstrophy/helicity
+// (src/postproc.cpp:931-1184) with fft::solveShell (src/fft.cpp:384-572).
+// The 3-D transforms are hipFFT Z2Z, forward (e^{-i}), divided by N as
+// src/fft.cpp:626-641; the split, the energies and the shell sums are kernels
+// of their own: per-workgroup shell histograms in LDS, then a fixed-order sum
+// over workgroups (the LDS accumulation order within a workgroup is not fixed,
+// so results may differ by rounding from run to run).
+// ---------------------------------------------------------------------------
+constexpr int SPEC_BLOCKS = 512;
+
+struct SpecArgs {
+    const double2 GMEM *f[3];      // transformed fields (not yet divided by N)
+    double GMEM *part;             // [block][spectrum][kmax+1]
+    int nx, ny, nz, kmax, nspec;
+    long N;
+};
+
+__device__ __forceinline__ double sgnk(int i, int n) { return (i >= n / 2.0) ? (double)(i - n) : (double)i; }
+
+// shell of solveShell's rounding: floor(k) or floor(k)+1 (src/fft.cpp:430-435)
+__device__ __forceinline__ int shell_of(double kmag)
+{
+    const double fl = floor(kmag);
+    return ((kmag - fl) - 0.5 <= 0) ? (int)fl : (int)fl + 1;
+}
+
+__global__ __launch_bounds__(256) void k_spec_kinetic(SpecArgs A)
+{
+    extern __shared__ double hist[];                 // 5 * (kmax + 1)
+    const int nh = 5 * (A.kmax + 1);
+    for (int i = threadIdx.x; i < nh; i += 256) hist[i] = 0.0;
+    __syncthreads();
+    const double invN = (double)A.N;
+    for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < A.N; id += (long)gridDim.x * 256) {
+        const int i = (int)(id % A.nx);
+        const long r = id / A.nx;
+        const int j = (int)(r % A.ny), k = (int)(r / A.ny);
+        const double kx = sgnk(i, A.nx), ky = sgnk(j, A.ny), kz = sgnk(k, A.nz);
+        double kmag = sqrt(kx * kx + ky * ky + kz * kz);
+        if (kmag < 1e-6) kmag = 1.0;
+        if (!(kmag < A.kmax)) continue;
+        const double2 u = A.f[0][id], v = A.f[1][id], w = A.f[2][id];
+        const double ur = u.x / invN, ui = u.y / invN, vr = v.x / invN, vi = v.y / invN;
+        const double wr = w.x / invN, wi = w.y / invN;
+        // src/postproc.cpp:1019-1075
+        const double ku_r = (kx * ur + ky * vr + kz * wr) / kmag, ku_i = (kx * ui + ky * vi + kz * wi) / kmag;
+        const double s0r = ur - ku_r * kx / kmag, s1r = vr - ku_r * ky / kmag, s2r = wr - ku_r * kz / kmag;
+        const double s0i = ui - ku_i * kx / kmag, s1i = vi - ku_i * ky / kmag, s2i = wi - ku_i * kz / kmag;
+        const double c0r = ur - s0r, c1r = vr - s1r, c2r = wr - s2r, c0i = ui - s0i, c1i = vi - s1i, c2i = wi - s2i;
+        const double x0r = -(ky * s2i - kz * s1i) / kmag, x1r = (kx * s2i - kz * s0i) / kmag;
+        const double x2r = -(kx * s1i - ky * s0i) / kmag;
+        const double x0i = (ky * s2r - kz * s1r) / kmag, x1i = -(kx * s2r - kz * s0r) / kmag;
+        const double x2i = (kx * s1r - ky * s0r) / kmag;
+        auto e6 = [](double a, double b, double c, double d, double e, double f) {
+            return 0.5 * (a * a + b * b + c * c + d * d + e * e + f * f);
+        };
+        const double E = e6(ur, vr, wr, ui, vi, wi);
+        const double Ec = e6(c0r, c1r, c2r, c0i, c1i, c2i);
+        const double Es = e6(s0r, s1r, s2r, s0i, s1i, s2i);
+        const double El = e6(0.5 * (s0r - x0r), 0.5 * (s1r - x1r), 0.5 * (s2r - x2r), 0.5 * (s0i - x0i),
+                             0.5 * (s1i - x1i), 0.5 * (s2i - x2i));
+        const double Er = e6(0.5 * (s0r + x0r), 0.5 * (s1r + x1r), 0.5 * (s2r + x2r), 0.5 * (s0i + x0i),
+                             0.5 * (s1i + x1i), 0.5 * (s2i + x2i));
+        const int ki = shell_of(kmag);
+        const int K1 = A.kmax + 1;
+        atomicAdd(&hist[0 * K1 + ki], E);
+        atomicAdd(&hist[1 * K1 + ki], Ec);
+        atomicAdd(&hist[2 * K1 + ki], Es);
+        atomicAdd(&hist[3 * K1 + ki], El);
+        atomicAdd(&hist[4 * K1 + ki], Er);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nh; i += 256) A.part[(long)blockIdx.x * nh + i] = hist[i];
+}
+
+// |f^|^2 over non-negative indices with i^2+j^2+k^2 < kmax^2 (src/fft.cpp:513-528)
+__global__ __launch_bounds__(256) void k_spec_scalar(SpecArgs A)
+{
+    extern __shared__ double hist[];
+    const int nh = A.kmax + 1;
+    for (int i = threadIdx.x; i < nh; i += 256) hist[i] = 0.0;
+    __syncthreads();
+    const double invN = (double)A.N;
+    for (long id = (long)blockIdx.x * 256 + threadIdx.x; id < A.N; id += (long)gridDim.x * 256) {
+        const int i = (int)(id % A.nx);
+        const long r = id / A.nx;
+        const int j = (int)(r % A.ny), k = (int)(r / A.ny);
+        const long k0 = (long)i * i + (long)j * j + (long)k * k;
+        if (!(k0 < (long)A.kmax * A.kmax)) continue;
+        const double2 f = A.f[0][id];
+        const double fr = f.x / invN, fi = f.y / invN;
+        atomicAdd(&hist[shell_of(sqrt((double)k0))], fr * fr + fi * fi);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nh; i += 256) A.part[(long)blockIdx.x * nh + i] = hist[i];
+}
+
+__global__ __launch_bounds__(256) void k_spec_sum(const double *part, int nh, int nblocks, double *out)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= nh) return;
+    double acc = 0.0;
+    for (int b = 0; b < nblocks; b++) acc += part[(long)b * nh + t];
+    out[t] = acc;
+}
+
+// real field(s) -> complex, and the velocity / pressure / vorticity inputs
+struct SpecPrep {
+    const double GMEM *q;
+    double2 GMEM *c[3];
+    double GMEM *uvw;              // 3N: u, v, w (vorticity inputs)
+    const double GMEM *grad;       // 9N: d(u,v,w)/d(x,y,z)
+    int what;                      // 0 velocity, 1 density, 2 pressure, 3 enstrophy, 4 helicity
+    long N;
+    double gamma;
+};
+
+__global__ __launch_bounds__(256) void k_spec_prep(SpecPrep A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const double rho = A.q[id];
+    const double u = A.q[A.N + id] / rho, v = A.q[2 * A.N + id] / rho, w = A.q[3 * A.N + id] / rho;
+    if (A.what == 0) {
+        A.c[0][id] = make_double2(u, 0.0);
+        A.c[1][id] = make_double2(v, 0.0);
+        A.c[2][id] = make_double2(w, 0.0);
+    } else if (A.what == 1) {
+        A.c[0][id] = make_double2(rho, 0.0);
+    } else if (A.what == 2) {
+        const double p = fabs((A.q[4 * A.N + id] - 0.5 * rho * (u * u + v * v + w * w)) * (A.gamma - 1.0));
+        A.c[0][id] = make_double2(p, 0.0);
+    } else if (A.grad == nullptr) {
+        A.uvw[id] = u; A.uvw[A.N + id] = v; A.uvw[2 * A.N + id] = w;
+    } else {
+        // vorticity from the gradients, src/postproc.cpp:815-835
+        const double GMEM *g = A.grad;
+        const double uy = g[1 * A.N + id], uz = g[2 * A.N + id], vx = g[3 * A.N + id], vz = g[5 * A.N + id];
+        const double wx = g[6 * A.N + id], wy = g[7 * A.N + id];
+        const double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
+        const double val = (A.what == 3) ? (w0 * w0 + w1 * w1 + w2 * w2) * 0.5 : w0 * u + w1 * v + w2 * w;
+        A.c[0][id] = make_double2(val, 0.0);
+    }
+}
+
+static int spectra_run(nuc_ctx *x, int what, double *out, int kmax_out)
+{
+    const long N = x->N;
+    const int kmax = std::min(std::min(x->n[0], x->n[1]), x->n[2]) / 2;
+    if (kmax_out < kmax) { x->err = "output holds fewer than kmax = min(n)/2 shells"; return NUC_EINVAL; }
+    const int ns = (what == 0) ? 5 : 1;
+    const int nf = (what == 0) ? 3 : 1;
+    const int nh = ns * (kmax + 1);
+    const bool vort = (what >= 3);
+    size_t bytes = (size_t)nf * N * sizeof(double2) + (size_t)SPEC_BLOCKS * nh * sizeof(double) + nh * sizeof(double);
+    if (vort) bytes += 12 * (size_t)N * sizeof(double);
+    char *buf = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void **>(&buf), bytes, x->stream) != hipSuccess) {
+        x->err = "device allocation (spectra)";
+        return NUC_ENOMEM;
+    }
+    double2 *cf = reinterpret_cast<double2 *>(buf);
+    double *part = reinterpret_cast<double *>(cf + (size_t)nf * N);
+    double *sum = part + (size_t)SPEC_BLOCKS * nh;
+    double *uvw = sum + nh, *grad = uvw + 3 * N;
+    SpecPrep P{};
+    P.q = gp((const double *)x->q[x->next_stage]);
+    for (int c = 0; c < nf; c++) P.c[c] = gp(cf + (size_t)c * N);
+    P.what = what;
+    P.N = N;
+    P.gamma = x->c.gamma;
+    const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
+    if (vort) {
+        P.uvw = gp(uvw);
+        P.grad = nullptr;
+        hipLaunchKernelGGL(k_spec_prep, grid, blk, 0, x->stream, P);
+        for (int f = 0; f < 3; f++)
+            for (int d = 0; d < 3; d++) {
+                PostDiffArgs D{gp((const double *)(uvw + (size_t)f * N)), gp(grad + (size_t)(3 * f + d) * N), N,
+                               x->n[0], x->n[1], d, x->n[d], x->nloc[d], x->tsv[d],
+                               x->c.vis_scheme[d] == NUC_CD8 ? VIS_CD8 : VIS_CD6, 1.0 / x->h[d]};
+                hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
+            }
+        P.grad = gp((const double *)grad);
+    }
+    hipLaunchKernelGGL(k_spec_prep, grid, blk, 0, x->stream, P);
+    hipfftHandle plan;
+    if (hipfftPlan3d(&plan, x->n[2], x->n[1], x->n[0], HIPFFT_Z2Z) != HIPFFT_SUCCESS) {
+        (void)hipFreeAsync(buf, x->stream);
+        x->err = "hipfftPlan3d";
+        return NUC_EHIP;
+    }
+    int frc = hipfftSetStream(plan, x->stream);
+    for (int c = 0; c < nf && frc == HIPFFT_SUCCESS; c++) {
+        hipfftDoubleComplex *d = reinterpret_cast<hipfftDoubleComplex *>(cf + (size_t)c * N);
+        frc = hipfftExecZ2Z(plan, d, d, HIPFFT_FORWARD);
+    }
+    SpecArgs S{};
+    for (int c = 0; c < nf; c++) S.f[c] = gp((const double2 *)(cf + (size_t)c * N));
+    S.part = gp(part);
+    S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
+    S.kmax = kmax; S.nspec = ns; S.N = N;
+    const size_t lds = (size_t)nh * sizeof(double);
+    if (frc == HIPFFT_SUCCESS) {
+        if (what == 0) hipLaunchKernelGGL(k_spec_kinetic, dim3(SPEC_BLOCKS), blk, lds, x->stream, S);
+        else hipLaunchKernelGGL(k_spec_scalar, dim3(SPEC_BLOCKS), blk, lds, x->stream, S);
+        hipLaunchKernelGGL(k_spec_sum, dim3((nh + 255) / 256), blk, 0, x->stream, (const double *)part, nh,
+                           SPEC_BLOCKS, sum);
+    }
+    std::vector<double> h(nh);
+    hipError_t e = hipMemcpyAsync(h.data(), sum, nh * sizeof(double), hipMemcpyDeviceToHost, x->stream);
+    (void)hipStreamSynchronize(x->stream);
+    (void)hipfftDestroy(plan);
+    (void)hipFreeAsync(buf, x->stream);
+    (void)hipStreamSynchronize(x->stream);
+    if (frc != HIPFFT_SUCCESS) { x->err = "hipfftExecZ2Z"; return NUC_EHIP; }
+    NUC_CHECK(x, e);
+    NUC_CHECK(x, hipGetLastError());
+    for (int sidx = 0; sidx < ns; sidx++)
+        for (int k = 0; k < kmax; k++) out[sidx * kmax_out + k] = h[sidx * (kmax + 1) + k];
+    return NUC_OK;
+}
+
+extern "C" int nuc_spectra_kinetic(nuc_ctx *x, double *out, int kmax)
+{
+    if (!x || !out) return NUC_EINVAL;
+    return spectra_run(x, 0, out, kmax);
+}
+
+extern "C" int nuc_spectrum_scalar(nuc_ctx *x, int field, double *out, int kmax)
+{
+    if (!x || !out || field < NUC_SPEC_DENSITY || field > NUC_SPEC_HELICITY) return NUC_EINVAL;
+    return spectra_run(x, 1 + field, out, kmax);
 }
 
 // ---------------------------------------------------------------------------

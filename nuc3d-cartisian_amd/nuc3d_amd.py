@@ -28,6 +28,7 @@ STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_
 NUC_OK, NUC_EBUSY = 0, 6
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
 IC_STATE, IC_TGV, IC_HIT = range(3)
+SPEC_DENSITY, SPEC_PRESSURE, SPEC_ENSTROPHY, SPEC_HELICITY = range(4)
 
 # every entry point include/nuc3d_gpu.h and include/nuc3d_serve.h declare
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
@@ -40,7 +41,7 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage",
            "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
            "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device",
-           "nuc_server_submit_ic"]
+           "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar"]
 
 
 class NucConfig(ctypes.Structure):
@@ -142,6 +143,8 @@ def lib():
         L.nuc_server_job_error.argtypes = [vp, ctypes.c_longlong]
         L.nuc_server_job_error.restype = ctypes.c_char_p
         L.nuc_autotune.argtypes = [vp]
+        L.nuc_spectra_kinetic.argtypes = [vp, _dp, ctypes.c_int]
+        L.nuc_spectrum_scalar.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
         L.nuc_init_tgv_device.argtypes = [vp]
         L.nuc_init_hit_device.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]
         L.nuc_average_begin.argtypes = [vp, ctypes.c_int]
@@ -282,6 +285,20 @@ class Context:
 
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    def spectra_kinetic(self):
+        """(5, kmax): total, compressive, solenoidal, left, right kinetic energy per shell."""
+        kmax = min(self.cfg.nx, self.cfg.ny, self.cfg.nz) // 2
+        out = np.zeros(5 * kmax)
+        _check(lib().nuc_spectra_kinetic(self.h, _ptr(out), kmax), "nuc_spectra_kinetic", self.h)
+        return out.reshape(5, kmax)
+
+    def spectrum_scalar(self, field):
+        """|f^|^2 per shell of SPEC_DENSITY, SPEC_PRESSURE, SPEC_ENSTROPHY or SPEC_HELICITY."""
+        kmax = min(self.cfg.nx, self.cfg.ny, self.cfg.nz) // 2
+        out = np.zeros(kmax)
+        _check(lib().nuc_spectrum_scalar(self.h, int(field), _ptr(out), kmax), "nuc_spectrum_scalar", self.h)
+        return out
 
     def init_tgv_device(self):
         """The Taylor-Green field generated on the device into the state."""
