@@ -40,8 +40,9 @@ enum nuc_riemann { NUC_LF = 0, NUC_AUSM = 1 };
 enum nuc_model { NUC_EULER3D = 0, NUC_NAVIERSTOKES3D = 1 };
 /* viscosity law (src/physicsModel.cpp:45-49) */
 enum nuc_viscosity { NUC_SUTHERLAND = 0, NUC_CONSTANT = 1 };
-/* forcing (src/physicsModel.cpp:50-56) */
-enum nuc_forcing { NUC_FORCING_NONE = 0, NUC_FORCING_CONST = 1 };
+/* forcing (src/physicsModel.cpp:50-56): Const (FFT band forcing), ABC and TG
+ * (analytic fields at the cell centres, src/physicsModel.cpp:1679-1799) */
+enum nuc_forcing { NUC_FORCING_NONE = 0, NUC_FORCING_CONST = 1, NUC_FORCING_ABC = 2, NUC_FORCING_TG = 3 };
 /* arithmetic of the stencils; the conserved state is FP64 in both */
 enum nuc_precision { NUC_FP64 = 0, NUC_MIXED = 1 };
 
@@ -56,6 +57,11 @@ typedef struct nuc_config {
     double gamma, mach, reynolds, prandtl, t_ref, t_inf;
     double forcing_amp, forcing_amp_hel, forcing_k;
     double cfl;
+    /* rotating-frame sources of PhysModel.in (src/physicsModel.cpp:1280-1319,
+     * 1801-1913): Coriolis / Centrifugal switches (0 = "no"), Rossby number,
+     * centre and radius of the centrifugal well */
+    int coriolis, centrifugal;
+    double rossby, x_center, y_center, well_rad;
 } nuc_config;
 
 typedef struct nuc_step_info {

@@ -1053,6 +1053,7 @@ struct nuc_ctx {
     long post_step = 0;
     double *post_buf = nullptr;
     bool forcing = false;
+    bool sources = false;   // ABC/TG forcing or rotating-frame sources (k_source)
     bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
     // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
     int gkz = 0;
@@ -1121,7 +1122,8 @@ static int validate(const nuc_config *c, std::string &err)
     }
     if ((long)c->npx * c->npy * c->npz > MAXRANK) { err = "at most 64 ranks"; return NUC_EINVAL; }
     if (c->riemann < 0 || c->riemann > 1 || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
-    if (c->forcing != NUC_FORCING_NONE && c->forcing != NUC_FORCING_CONST) { err = "unknown forcing id"; return NUC_EINVAL; }
+    if (c->forcing < NUC_FORCING_NONE || c->forcing > NUC_FORCING_TG) { err = "unknown forcing id"; return NUC_EINVAL; }
+    if (c->centrifugal && !(c->well_rad > 0)) { err = "centrifugal well radius must be positive"; return NUC_EINVAL; }
     if (c->precision != NUC_FP64 && c->precision != NUC_MIXED) { err = "unknown precision id"; return NUC_EINVAL; }
     if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) { err = "box lengths must be positive"; return NUC_EINVAL; }
     return NUC_OK;
@@ -1185,6 +1187,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
         if (!dalloc(&x->q[b], 5 * N * sizeof(double))) return fail("device allocation (state)");
     if (!dalloc(&x->rhs, 5 * N * sizeof(double))) return fail("device allocation (rhs)");
     x->forcing = (cfg->forcing == NUC_FORCING_CONST);
+    x->sources = cfg->forcing == NUC_FORCING_ABC || cfg->forcing == NUC_FORCING_TG || cfg->coriolis || cfg->centrifugal;
     x->mixed = (cfg->precision == NUC_MIXED);
     if (x->forcing)
         if (!dalloc(&x->prim, 4 * N * sizeof(double))) return fail("device allocation (prim)");
@@ -1320,7 +1323,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         for (int v = 0; v < 5; v++) { S.q[v] = gp(qi + v * N); S.rhs[v] = gp(x->rhs + v * N); }
         if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(x->fv + (long)fvmap[d][o] * N);
         S.stage = stage;
-        S.accum = (d == 0 && x->forcing) ? 1 : 0;
+        S.accum = (d == 0 && (x->forcing || x->sources)) ? 1 : 0;
         S.mixed = x->mixed ? 1 : 0;
         S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
         S.fmean = (d == 2 && x->forcing) ? gp(x->scal + 4) : nullptr;
@@ -1334,6 +1337,78 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
     }
     sa.do_res = epilogue && stage == 2;
     sa.res = ResArgs{gp(x->res_part), (int)x->res_nb, 1.0 / ((double)N * 5.0), gp(x->scal)};
+}
+
+// AddForcing_ABC / AddForcing_TG (src/physicsModel.cpp:1679-1799),
+// AddCoriolisForce (:1801-1840) and AddCentrifugalForce (:1842-1913, whose
+// energy term goes to the y-momentum RHS as written), in solveSource's order
+// (:1280-1319), at the per-rank cell centres (src/block.cpp:116-118)
+struct SourceArgs {
+    const double GMEM *q;
+    double GMEM *rhs;
+    long N;
+    int nx, ny, nl[3], forcing, coriolis, centrifugal, accumulate;
+    double dx, dy, dz, lxr, lyr, lzr, A, kf, Ro, xc, yc, r0;
+};
+
+__global__ __launch_bounds__(256) void k_source(SourceArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const int i = (int)(id % A.nx);
+    const long rr = id / A.nx;
+    const int j = (int)(rr % A.ny), k = (int)(rr / A.ny);
+    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]);
+    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]);
+    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]);
+    const double rho = A.q[id];
+    const double u0 = A.q[A.N + id] / rho, v0 = A.q[2 * A.N + id] / rho, w0 = A.q[3 * A.N + id] / rho;
+    double r1 = 0.0, r2 = 0.0, r3 = 0.0, r4 = 0.0;
+    if (A.accumulate) { r1 = A.rhs[A.N + id]; r2 = A.rhs[2 * A.N + id]; r3 = A.rhs[3 * A.N + id]; r4 = A.rhs[4 * A.N + id]; }
+    if (A.forcing == NUC_FORCING_ABC) {
+        const double fx = A.A * 1.0 * (cos(A.kf * y0) + 1.1 * sin(A.kf * z0));
+        const double fy = A.A * 1.0 * (1.1 * cos(A.kf * z0) + 0.9 * sin(A.kf * x0));
+        const double fz = A.A * 1.0 * (0.9 * cos(A.kf * x0) + sin(A.kf * y0));
+        r1 -= fx; r2 -= fy; r3 -= fz;
+        r4 -= fx * u0 + fy * v0 + fz * w0;
+    } else if (A.forcing == NUC_FORCING_TG) {
+        const double fx = A.A * 2.0 * (sin(A.kf * x0) * cos(A.kf * y0) * sin(A.kf * z0));
+        const double fy = -A.A * 2.0 * (cos(A.kf * x0) * sin(A.kf * y0) * sin(A.kf * z0));
+        r1 -= fx; r2 -= fy;
+        r4 -= (fx * u0 + fy * v0);
+    }
+    if (A.coriolis) {
+        r1 -= rho * A.Ro * v0;
+        r2 -= -rho * A.Ro * u0;
+    }
+    if (A.centrifugal) {
+        const double r = sqrt((x0 - A.xc) * (x0 - A.xc) + (y0 - A.yc) * (y0 - A.yc));
+        const double fx = rho * A.Ro * A.Ro * (x0 - A.xc) / (1.0 + pow(r / A.r0, 20.0)) / 4.0;
+        const double fy = rho * A.Ro * A.Ro * (y0 - A.yc) / (1.0 + pow(r / A.r0, 20.0)) / 4.0;
+        r1 -= fx;
+        r2 -= fy;
+        r2 -= fx * u0 + fy * v0;
+    }
+    if (!A.accumulate) A.rhs[id] = 0.0;
+    A.rhs[A.N + id] = r1; A.rhs[2 * A.N + id] = r2; A.rhs[3 * A.N + id] = r3; A.rhs[4 * A.N + id] = r4;
+}
+
+static SourceArgs source_args(nuc_ctx *x, int in, bool accumulate)
+{
+    const nuc_config &c = x->c;
+    SourceArgs A{};
+    A.q = gp((const double *)x->q[in]);
+    A.rhs = gp(x->rhs);
+    A.N = x->N;
+    A.nx = c.nx; A.ny = c.ny;
+    for (int d = 0; d < 3; d++) A.nl[d] = x->nloc[d];
+    A.forcing = c.forcing; A.coriolis = c.coriolis; A.centrifugal = c.centrifugal;
+    A.accumulate = accumulate ? 1 : 0;
+    A.dx = c.lx / c.nx; A.dy = c.ly / c.ny; A.dz = c.lz / c.nz;
+    A.lxr = c.lx / c.npx; A.lyr = c.ly / c.npy; A.lzr = c.lz / c.npz;
+    A.A = c.forcing_amp; A.kf = c.forcing_k;
+    A.Ro = c.rossby; A.xc = c.x_center; A.yc = c.y_center; A.r0 = c.well_rad;
+    return A;
 }
 
 static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
@@ -1361,6 +1436,13 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
         for (int v = 0; v < 5; v++) r[v] = x->rhs + v * N;
         ProfScope ps(x, 7);
         nucforce::enqueue(x->fplan, x->prim, x->prim + N, x->prim + 2 * N, r, x->scal + 4, x->stream);
+    }
+    if (x->sources) {
+        // ABC/TG forcing and the rotating-frame sources initialise rhs (or add
+        // to the Const forcing's initialisation); the x sweep accumulates onto it
+        SourceArgs SA = source_args(x, in, x->forcing);
+        ProfScope ps(x, 7);
+        hipLaunchKernelGGL(k_source, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, x->stream, SA);
     }
     for (int d = 0; d < 3; d++) {
         ProfScope ps(x, 3 + d);
@@ -2679,7 +2761,7 @@ extern "C" int nuc_spectrum_scalar(nuc_ctx *x, int field, double *out, int kmax)
 // ---------------------------------------------------------------------------
 static bool packable(const nuc_ctx *x)
 {
-    if (x->forcing) return false;
+    if (x->forcing || x->sources) return false;
     for (int d = 0; d < 3; d++)
         if (x->cpt[d] > PACK_RMAX) return false;
     return true;

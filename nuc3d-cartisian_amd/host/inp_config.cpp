@@ -102,9 +102,9 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
         else { err = "PhysModel.in: PDE model '" + model + "' does not exist"; return NUC_EINVAL; }
         c.viscosity = (visc == "Constant") ? NUC_CONSTANT : NUC_SUTHERLAND;  // unknown -> Sutherland
         if (forcing == "Const") c.forcing = NUC_FORCING_CONST;
-        else if (forcing == "None") c.forcing = NUC_FORCING_NONE;
-        else if (forcing == "ABC" || forcing == "TG") { err = "PhysModel.in: forcing '" + forcing + "' is not available"; return NUC_EINVAL; }
-        else c.forcing = NUC_FORCING_NONE;  // unknown name -> None (src/physicsModel.cpp:104-110)
+        else if (forcing == "ABC") c.forcing = NUC_FORCING_ABC;
+        else if (forcing == "TG") c.forcing = NUC_FORCING_TG;
+        else c.forcing = NUC_FORCING_NONE;  // "None", or an unknown name -> None (src/physicsModel.cpp:104-110)
         std::string a, b;
         while (f >> a >> b) {
             if (par.count(a)) {
@@ -119,11 +119,12 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
                 return NUC_EINVAL;
             }
         }
-        if (sw["Coriolis"] != "no" || sw["Centrifugal"] != "no") {
-            err = "PhysModel.in: Coriolis/Centrifugal sources are not available";
-            return NUC_EINVAL;
-        }
     }
+    // any value other than "no" switches a source on (src/physicsModel.cpp:1295,1307)
+    c.coriolis = (sw["Coriolis"] != "no") ? 1 : 0;
+    c.centrifugal = (sw["Centrifugal"] != "no") ? 1 : 0;
+    c.rossby = par["Rossby"]; c.x_center = par["x_center"]; c.y_center = par["y_center"];
+    c.well_rad = par["well_rad"];
     c.gamma = par["Gamma"]; c.mach = par["Mach"]; c.reynolds = par["Reynolds"]; c.prandtl = par["Pt"];
     c.t_ref = par["T_ref"]; c.t_inf = par["T_inf"];
     c.forcing_amp = par["Forcing_amplitude"]; c.forcing_amp_hel = par["Forcing_amplitude_hel"];

@@ -784,6 +784,57 @@ static void add_forcing_const(orc_solver *s)
     free(uh); free(vs); free(force);
 }
 
+/* The analytic forcings and the rotating-frame sources, in the order of
+ * physicsModel::solveSource (src/physicsModel.cpp:1280-1319): AddForcing_ABC
+ * (:1679-1739) or AddForcing_TG (:1741-1799), then AddCoriolisForce
+ * (:1801-1840), then AddCentrifugalForce (:1842-1913), whose energy term is
+ * subtracted from the y-momentum RHS as written (:1900).  Positions are the
+ * per-rank cell centres (src/block.cpp:116-118); u, v, w the stage input's. */
+static void add_sources(orc_solver *s)
+{
+    const orc_config *c = &s->c;
+    if (c->forcing != ORC_FORCE_ABC && c->forcing != ORC_FORCE_TG && !c->coriolis && !c->centrifugal) return;
+    const long N = s->N;
+    const double A = c->forcing_amp, kf = c->forcing_k;
+    const double Ro = c->rossby, xc = c->x_center, yc = c->y_center, r0 = c->well_rad;
+    const double dx = c->lx / c->nx, dy = c->ly / c->ny, dz = c->lz / c->nz;
+    double *r1 = s->rhs + N, *r2 = s->rhs + 2 * N, *r3 = s->rhs + 3 * N, *r4 = s->rhs + 4 * N;
+    for (int k = 0; k < s->n[2]; k++)
+        for (int j = 0; j < s->n[1]; j++)
+            for (int i = 0; i < s->n[0]; i++) {
+                const long id = gidx(s, i, j, k);
+                const double x0 = c->lx / c->npx * (i / s->nloc[0]) + dx * (0.5 + i % s->nloc[0]);
+                const double y0 = c->ly / c->npy * (j / s->nloc[1]) + dy * (0.5 + j % s->nloc[1]);
+                const double z0 = c->lz / c->npz * (k / s->nloc[2]) + dz * (0.5 + k % s->nloc[2]);
+                const double rho = s->prim[id], u0 = s->prim[N + id], v0 = s->prim[2 * N + id], w0 = s->prim[3 * N + id];
+                if (c->forcing == ORC_FORCE_ABC) {
+                    const double rho0 = 1.0;
+                    double fx = A * rho0 * (cos(kf * y0) + 1.1 * sin(kf * z0));
+                    double fy = A * rho0 * (1.1 * cos(kf * z0) + 0.9 * sin(kf * x0));
+                    double fz = A * rho0 * (0.9 * cos(kf * x0) + sin(kf * y0));
+                    r1[id] -= fx; r2[id] -= fy; r3[id] -= fz;
+                    r4[id] -= fx * u0 + fy * v0 + fz * w0;
+                } else if (c->forcing == ORC_FORCE_TG) {
+                    double fx = A * 2.0 * (sin(kf * x0) * cos(kf * y0) * sin(kf * z0));
+                    double fy = -A * 2.0 * (cos(kf * x0) * sin(kf * y0) * sin(kf * z0));
+                    r1[id] -= fx; r2[id] -= fy;
+                    r4[id] -= (fx * u0 + fy * v0);
+                }
+                if (c->coriolis) {
+                    r1[id] -= rho * Ro * v0;
+                    r2[id] -= -rho * Ro * u0;
+                }
+                if (c->centrifugal) {
+                    double r = sqrt((x0 - xc) * (x0 - xc) + (y0 - yc) * (y0 - yc));
+                    double fx = rho * Ro * Ro * (x0 - xc) / (1.0 + pow(r / r0, 20)) / 4.0;
+                    double fy = rho * Ro * Ro * (y0 - yc) / (1.0 + pow(r / r0, 20)) / 4.0;
+                    r1[id] -= fx;
+                    r2[id] -= fy;
+                    r2[id] -= fx * u0 + fy * v0;
+                }
+            }
+}
+
 void orc_rhs(orc_solver *s, double *rhs_out, double *dt_local)
 {
     const long N = s->N;
@@ -841,6 +892,7 @@ void orc_rhs(orc_solver *s, double *rhs_out, double *dt_local)
             s->rhs[(size_t)v * N + id] = r;
         }
     if (s->c.forcing == ORC_FORCE_CONST) add_forcing_const(s);
+    add_sources(s);
     /* getDt per rank, then allReduceMin (src/euler3d.cpp:459-466) */
     int P = s->np[0] * s->np[1] * s->np[2];
     double dtmin = 0.0;

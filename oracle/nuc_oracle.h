@@ -24,7 +24,7 @@ enum { ORC_CD6 = 0, ORC_CD8 = 1 };
 enum { ORC_LF = 0, ORC_AUSM = 1 };
 enum { ORC_EULER = 0, ORC_NS = 1 };
 enum { ORC_SUTHERLAND = 0, ORC_CONSTANT = 1 };
-enum { ORC_FORCE_NONE = 0, ORC_FORCE_CONST = 1 };
+enum { ORC_FORCE_NONE = 0, ORC_FORCE_CONST = 1, ORC_FORCE_ABC = 2, ORC_FORCE_TG = 3 };
 
 typedef struct {
     int nx, ny, nz;          /* global grid */
@@ -37,6 +37,8 @@ typedef struct {
     double forcing_amp, forcing_amp_hel, forcing_k;
     double cfl;
     int nthreads;            /* OpenMP threads for line loops (CPU baseline) */
+    int coriolis, centrifugal;                        /* source switches */
+    double rossby, x_center, y_center, well_rad;      /* PhysModel.in */
 } orc_config;
 
 /* --- operator level: one line of n cells, f valid on [-bf, n+bf) --------- */

@@ -17,7 +17,7 @@ CD6, CD8 = 0, 1
 LF, AUSM = 0, 1
 EULER, NS = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
-FORCE_NONE, FORCE_CONST = 0, 1
+FORCE_NONE, FORCE_CONST, FORCE_ABC, FORCE_TG = 0, 1, 2, 3
 SCHEME_NAMES = {"weno5js": WENO5JS, "weno5z": WENO5Z, "weno7js": WENO7JS, "crweno5": CRWENO5, "hccs": HCCS}
 
 
@@ -33,6 +33,9 @@ class OrcConfig(ctypes.Structure):
         ("prandtl", ctypes.c_double), ("t_ref", ctypes.c_double), ("t_inf", ctypes.c_double),
         ("forcing_amp", ctypes.c_double), ("forcing_amp_hel", ctypes.c_double),
         ("forcing_k", ctypes.c_double), ("cfl", ctypes.c_double), ("nthreads", ctypes.c_int),
+        ("coriolis", ctypes.c_int), ("centrifugal", ctypes.c_int),
+        ("rossby", ctypes.c_double), ("x_center", ctypes.c_double), ("y_center", ctypes.c_double),
+        ("well_rad", ctypes.c_double),
     ]
 
 
@@ -189,7 +192,8 @@ def viscosity(law, T, re, mach, pr, gamma, t_ref, t_inf):
 def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, riemann=LF, model=NS,
                 viscosity=SUTHERLAND, forcing=FORCE_NONE, gamma=1.4, mach=0.1, reynolds=1600.0,
                 prandtl=0.72, t_ref=110.4, t_inf=288.0, forcing_amp=0.01, forcing_amp_hel=0.01,
-                forcing_k=1.0, cfl=0.5, nthreads=1):
+                forcing_k=1.0, cfl=0.5, nthreads=1, coriolis=False, centrifugal=False, rossby=0.0,
+                x_center=3.1415926, y_center=3.1415926, well_rad=1.0):
     c = OrcConfig()
     c.nx, c.ny, c.nz = n
     c.npx, c.npy, c.npz = npr
@@ -207,6 +211,8 @@ def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, 
     c.forcing_amp, c.forcing_amp_hel, c.forcing_k = forcing_amp, forcing_amp_hel, forcing_k
     c.cfl = cfl
     c.nthreads = nthreads
+    c.coriolis, c.centrifugal = int(bool(coriolis)), int(bool(centrifugal))
+    c.rossby, c.x_center, c.y_center, c.well_rad = rossby, x_center, y_center, well_rad
     return c
 
 

@@ -149,3 +149,15 @@ def test_shu_osher_initial_condition():
     assert np.abs(div).max() <= 1e-10 * mag.max()
     with pytest.raises(G.NucError):
         G.init_shu_osher(cfg, pert=-1.0)
+
+
+def test_inp_abc_forcing_and_rotating_frame_sources():
+    """Forcing ABC and the Coriolis/Centrifugal switches with their parameters
+    (src/physicsModel.cpp:22-56,133-156); any value but "no" switches a source on."""
+    c, s, e = G.config_from_inp(os.path.join(GOLD, "inp_abc_rotating"))
+    assert c.forcing == G.FORCING_ABC and (c.forcing_amp, c.forcing_k) == (0.05, 1.0)
+    assert (c.coriolis, c.centrifugal) == (1, 1)
+    assert (c.rossby, c.x_center, c.y_center, c.well_rad) == (0.5, 3.0, 3.2, 1.5)
+    assert (s, e, c.cfl) == (0, 50, 0.4)
+    c2, _, _ = G.config_from_inp(os.path.join(GOLD, "inp_tgv64"))
+    assert (c2.coriolis, c2.centrifugal, c2.rossby, c2.well_rad) == (0, 0, 0.0, 1.0)
