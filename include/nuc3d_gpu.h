@@ -233,6 +233,13 @@ int nuc_config_from_inp(const char *dir, nuc_config *cfg, int *start_step, int *
  *   with phases drawn from mt19937_64(seed) instead of std::random_device and
  *   rescaled to the given u_rms; rho = 1, p = 1/(gamma Mach^2). */
 int nuc_init_tgv(const nuc_config *cfg, double *q);
+/* nuc_init_default: fluid at rest, rho = 1, p = 1/(gamma Mach^2), as
+ *   initial_default (src/block.cpp:285-314) whose u_init, v_init, w_init
+ *   PhysModel.in cannot set (always 0).
+ * nuc_init_ivc: isentropic vortex of initial_ivc (src/block.cpp:316-357),
+ *   centre (5, 5) in x-y, strength 0.5, carried by u = 0.5, p = rho^gamma. */
+int nuc_init_default(const nuc_config *cfg, double *q);
+int nuc_init_ivc(const nuc_config *cfg, double *q);
 int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsigned long long seed, double *q);
 /* nuc_init_shu_osher: Mach-3 shock / entropy-wave profile along x, measured
  *   from the box centre: (rho, u, p) = (3.857143, 2.629369, 10.33333) for
@@ -245,11 +252,14 @@ int nuc_init_hit(const nuc_config *cfg, double k0, double u_rms, unsigned long l
  *   equals p / rho. */
 int nuc_init_shu_osher(const nuc_config *cfg, double pert, unsigned long long seed, double *q);
 /* The same fields generated on the device into ctx's state (a new step starts):
- * nuc_init_tgv_device is nuc_init_tgv; nuc_init_hit_device synthesises
+ * nuc_init_tgv_device, nuc_init_default_device and nuc_init_ivc_device are
+ * nuc_init_tgv, nuc_init_default and nuc_init_ivc; nuc_init_hit_device synthesises
  * nuc_init_hit's seeded modes with a separable inverse DFT on the device (equal
  * to the host field up to rounding, ~1e-15 relative) -- for large boxes and for
  * jobs whose initial state should not cross PCIe. */
 int nuc_init_tgv_device(nuc_ctx *ctx);
+int nuc_init_default_device(nuc_ctx *ctx);
+int nuc_init_ivc_device(nuc_ctx *ctx);
 int nuc_init_hit_device(nuc_ctx *ctx, double k0, double u_rms, unsigned long long seed);
 
 #ifdef __cplusplus

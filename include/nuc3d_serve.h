@@ -62,9 +62,10 @@ int nuc_server_destroy(nuc_server *srv);
 /* copies q0 (5*nx*ny*nz doubles); job_id receives the job's id */
 int nuc_server_submit(nuc_server *srv, const nuc_config *cfg, const double *q0, int nsteps, long long *job_id);
 /* a job whose initial state is generated on the device at admission (no host
- * state crosses PCIe): NUC_IC_TGV (nuc_init_tgv_device) or NUC_IC_HIT
- * (nuc_init_hit_device with k0, u_rms, seed) */
-enum nuc_initial_condition { NUC_IC_STATE = 0, NUC_IC_TGV = 1, NUC_IC_HIT = 2 };
+ * state crosses PCIe): NUC_IC_TGV (nuc_init_tgv_device), NUC_IC_HIT
+ * (nuc_init_hit_device with k0, u_rms, seed), NUC_IC_IVC (nuc_init_ivc_device)
+ * or NUC_IC_DEFAULT (nuc_init_default_device) */
+enum nuc_initial_condition { NUC_IC_STATE = 0, NUC_IC_TGV = 1, NUC_IC_HIT = 2, NUC_IC_IVC = 3, NUC_IC_DEFAULT = 4 };
 int nuc_server_submit_ic(nuc_server *srv, const nuc_config *cfg, int ic, double k0, double u_rms,
                          unsigned long long seed, int nsteps, long long *job_id);
 /* NUC_OK with r filled (state DONE or FAILED) and, if q_out is not NULL and the

@@ -2413,14 +2413,43 @@ __global__ __launch_bounds__(256) void k_ic_q(IcArgs A)
     A.q[4 * A.N + id] = p0 / (A.gamma - 1.0) + 0.5 * (u * u + v * v + w * w);
 }
 
-struct TgvArgs {
+// analytic initial conditions of src/block.cpp:285-395, by IcKind
+enum IcKind { IC_DEFAULT = 0, IC_IVC = 1, IC_TGV = 2 };
+
+struct AnalyticIcArgs {
     double GMEM *q;
-    int nx, ny, nz, nl[3];
+    int nx, ny, nz, nl[3], kind;
     long N;
     double dx, dy, dz, lxr, lyr, lzr, gamma, mach;   // cell sizes, rank box sizes
 };
 
-__global__ __launch_bounds__(256) void k_ic_tgv(TgvArgs A)
+// initial_default (src/block.cpp:285-314): at rest, p = 1/(gamma M^2)
+__device__ void ic_default(const AnalyticIcArgs &A, long id)
+{
+    A.q[id] = 1.0;
+    A.q[A.N + id] = 0.0;
+    A.q[2 * A.N + id] = 0.0;
+    A.q[3 * A.N + id] = 0.0;
+    A.q[4 * A.N + id] = 1.0 / (A.mach * A.mach * A.gamma) / (A.gamma - 1.0);
+}
+
+// initial_ivc (src/block.cpp:316-357): vortex at (5, 5), strength 0.5, on u = 0.5
+__device__ void ic_ivc(const AnalyticIcArgs &A, long id, double x0, double y0)
+{
+    const double g = A.gamma, b = 0.5, pi = 3.141592653589793;
+    const double dx = x0 - 5.0, dy = y0 - 5.0;
+    const double r2 = dx * dx + dy * dy;
+    const double rho = pow(1.0 - (g - 1.0) * b * b * exp(1.0 - r2) / (8.0 * g * pi * pi), 2.5);
+    const double a = b / (2.0 * pi) * exp(0.5 * (1.0 - r2));
+    const double u = 0.5 - a * dy, v = a * dx;
+    A.q[id] = rho;
+    A.q[A.N + id] = rho * u;
+    A.q[2 * A.N + id] = rho * v;
+    A.q[3 * A.N + id] = 0.0;
+    A.q[4 * A.N + id] = pow(rho, g) / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+}
+
+__global__ __launch_bounds__(256) void k_ic_analytic(AnalyticIcArgs A)
 {
     const long id = (long)blockIdx.x * 256 + threadIdx.x;
     if (id >= A.N) return;
@@ -2431,6 +2460,8 @@ __global__ __launch_bounds__(256) void k_ic_tgv(TgvArgs A)
     const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]);
     const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]);
     const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]);
+    if (A.kind == IC_DEFAULT) { ic_default(A, id); return; }
+    if (A.kind == IC_IVC) { ic_ivc(A, id, x0, y0); return; }
     const double g = A.gamma, m = A.mach;
     const double u0 = sin(x0) * cos(y0) * cos(z0);
     const double v0 = -cos(x0) * sin(y0) * cos(z0);
@@ -2442,11 +2473,12 @@ __global__ __launch_bounds__(256) void k_ic_tgv(TgvArgs A)
     A.q[4 * A.N + id] = p0 / (g - 1.0) + 0.5 * (u0 * u0 + v0 * v0);
 }
 
-extern "C" int nuc_init_tgv_device(nuc_ctx *x)
+static int init_analytic(nuc_ctx *x, int kind)
 {
     if (!x) return NUC_EINVAL;
     const nuc_config &c = x->c;
-    TgvArgs A{};
+    AnalyticIcArgs A{};
+    A.kind = kind;
     A.q = gp(x->q[0]);
     A.nx = c.nx; A.ny = c.ny; A.nz = c.nz;
     for (int d = 0; d < 3; d++) A.nl[d] = x->nloc[d];
@@ -2455,11 +2487,15 @@ extern "C" int nuc_init_tgv_device(nuc_ctx *x)
     A.lxr = c.lx / c.npx; A.lyr = c.ly / c.npy; A.lzr = c.lz / c.npz;
     A.gamma = c.gamma; A.mach = c.mach;
     x->next_stage = 0;
-    hipLaunchKernelGGL(k_ic_tgv, dim3((unsigned)((x->N + 255) / 256)), dim3(256), 0, x->stream, A);
+    hipLaunchKernelGGL(k_ic_analytic, dim3((unsigned)((x->N + 255) / 256)), dim3(256), 0, x->stream, A);
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }
+
+extern "C" int nuc_init_tgv_device(nuc_ctx *x) { return init_analytic(x, IC_TGV); }
+extern "C" int nuc_init_default_device(nuc_ctx *x) { return init_analytic(x, IC_DEFAULT); }
+extern "C" int nuc_init_ivc_device(nuc_ctx *x) { return init_analytic(x, IC_IVC); }
 
 extern "C" int nuc_init_hit_device(nuc_ctx *x, double k0, double u_rms, unsigned long long seed)
 {

@@ -202,6 +202,47 @@ void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vecto
             }
 }
 
+// Fluid at rest, p = 1/(gamma M^2) (src/block.cpp:285-314).  The reference's
+// u_init, v_init, w_init come from getValue on a map that does not hold them
+// and that PhysModel.in cannot extend, so they are always 0.
+void initial_default(const nuc_config &c, std::vector<double> &q)
+{
+    const long N = (long)c.nx * c.ny * c.nz;
+    q.assign(5 * N, 0.0);
+    const double e = 1.0 / (c.mach * c.mach * c.gamma) / (c.gamma - 1.0);
+    for (long id = 0; id < N; id++) {
+        q[id] = 1.0;
+        q[4 * N + id] = e;
+    }
+}
+
+// Isentropic vortex of initial_ivc (src/block.cpp:316-357): centre (5, 5) in
+// the x-y plane, strength 0.5, carried by u = 0.5; p = rho^gamma with the
+// density exponent 2.5 as the reference writes it.
+void initial_ivc(const nuc_config &c, const double origin[3], std::vector<double> &q)
+{
+    const long N = (long)c.nx * c.ny * c.nz;
+    q.assign(5 * N, 0.0);
+    int nl[2] = {c.nx / c.npx, c.ny / c.npy};
+    const double dx = c.lx / c.nx, dy = c.ly / c.ny, g = c.gamma;
+    const double pi = 4.0 * std::atan(1.0), b = 0.5, xc = 5.0, yc = 5.0;
+    for (int k = 0; k < c.nz; k++)
+        for (int j = 0; j < c.ny; j++)
+            for (int i = 0; i < c.nx; i++) {
+                const double x0 = c.lx / c.npx * (i / nl[0]) + dx * (0.5 + i % nl[0]) - origin[0] - xc;
+                const double y0 = c.ly / c.npy * (j / nl[1]) + dy * (0.5 + j % nl[1]) - origin[1] - yc;
+                const double r2 = x0 * x0 + y0 * y0;
+                const double rho = std::pow(1.0 - (g - 1.0) * b * b * std::exp(1.0 - r2) / (8.0 * g * pi * pi), 2.5);
+                const double a = b / (2.0 * pi) * std::exp(0.5 * (1.0 - r2));
+                const double u = 0.5 - a * y0, v = a * x0;
+                const long id = i + (long)c.nx * (j + (long)c.ny * k);
+                q[id] = rho;
+                q[N + id] = rho * u;
+                q[2 * N + id] = rho * v;
+                q[4 * N + id] = std::pow(rho, g) / (g - 1.0) + 0.5 * rho * (u * u + v * v);
+            }
+}
+
 typedef std::complex<double> cplx;
 
 // Seeded random-phase solenoidal field.  The spectrum
@@ -387,6 +428,26 @@ extern "C" int nuc_init_tgv(const nuc_config *cfg, double *q)
     std::vector<double> v;
     double o[3] = {0, 0, 0};
     nuc3d::initial_taylorgreen(*cfg, o, v);
+    std::memcpy(q, v.data(), v.size() * sizeof(double));
+    return NUC_OK;
+}
+
+extern "C" int nuc_init_default(const nuc_config *cfg, double *q)
+{
+    if (!cfg || !q) return NUC_EINVAL;
+    std::vector<double> v;
+    nuc3d::initial_default(*cfg, v);
+    std::memcpy(q, v.data(), v.size() * sizeof(double));
+    return NUC_OK;
+}
+
+extern "C" int nuc_init_ivc(const nuc_config *cfg, double *q)
+{
+    if (!cfg || !q) return NUC_EINVAL;
+    if (cfg->npx < 1 || cfg->npy < 1 || cfg->nx % cfg->npx != 0 || cfg->ny % cfg->npy != 0) return NUC_EINVAL;
+    std::vector<double> v;
+    double o[3] = {0, 0, 0};
+    nuc3d::initial_ivc(*cfg, o, v);
     std::memcpy(q, v.data(), v.size() * sizeof(double));
     return NUC_OK;
 }

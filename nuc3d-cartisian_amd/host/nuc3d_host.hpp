@@ -173,6 +173,10 @@ public:
     void solve(int stage);
 };
 
+// Fluid at rest of src/block.cpp:285-314
+void initial_default(const nuc_config &c, std::vector<double> &q);
+// Isentropic vortex of src/block.cpp:316-357 on the rank grid
+void initial_ivc(const nuc_config &c, const double origin[3], std::vector<double> &q);
 // Taylor-Green initial condition of src/block.cpp:359-395 on the rank grid
 void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vector<double> &q);
 // Seeded random-phase solenoidal field with E(k) ~ k^4 exp(-2 (k/k0)^2)

@@ -1,6 +1,6 @@
 // nuc3d_run -- the reference's executable flow (src/NUC3d.cpp:3-14,
 // src/singleBlock.cpp:31-71) on one GPU: read inp/*.in, set the initial
-// condition (Benchmark 2 = Taylor-Green, 3 = seeded HIT) or read the restart
+// condition (Benchmark 0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = seeded HIT) or read the restart
 // file when startStep != 0, then step while ifsolve() holds
 // (src/IOcontroller.cpp:182-192), printing the status line of
 // block::printStatus (src/block.cpp:217-220) and writing restart files every
@@ -42,6 +42,10 @@ int main(int argc, char **argv)
             fprintf(stderr, "cannot read restart file %s\n", path);
             return 1;
         }
+    } else if (rc.benchmark == 0) {
+        nuc3d::initial_default(cfg, q);
+    } else if (rc.benchmark == 1) {
+        nuc3d::initial_ivc(cfg, rc.origin, q);
     } else if (rc.benchmark == 2) {
         nuc3d::initial_taylorgreen(cfg, rc.origin, q);
     } else if (rc.benchmark == 3) {
@@ -49,7 +53,7 @@ int main(int argc, char **argv)
         double u_rms = rc.mach_t / cfg.mach / std::sqrt(3.0);
         nuc3d::initial_hit(cfg, rc.k_peak, u_rms, seed, q);
     } else {
-        fprintf(stderr, "Benchmark %d is not available (2 = Taylor-Green, 3 = HIT, or a restart)\n", rc.benchmark);
+        fprintf(stderr, "Benchmark %d is not available (0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = HIT, or a restart)\n", rc.benchmark);
         return 1;
     }
     try {

@@ -172,6 +172,8 @@ void nuc_server::run()
             j->packable = nuc_pack_compatible(j->ctx, j->ctx) != 0;
             if (j->ic == NUC_IC_TGV) rc = nuc_init_tgv_device(j->ctx);
             else if (j->ic == NUC_IC_HIT) rc = nuc_init_hit_device(j->ctx, j->ic_k0, j->ic_urms, j->ic_seed);
+            else if (j->ic == NUC_IC_IVC) rc = nuc_init_ivc_device(j->ctx);
+            else if (j->ic == NUC_IC_DEFAULT) rc = nuc_init_default_device(j->ctx);
             else rc = nuc_upload_state(j->ctx, j->q.data());
             if (j->ic != NUC_IC_STATE) j->q.assign((size_t)5 * j->cfg.nx * j->cfg.ny * j->cfg.nz, 0.0);
             if (rc == NUC_OK && cfg.autotune) rc = nuc_autotune(j->ctx);
@@ -358,7 +360,7 @@ extern "C" int nuc_server_submit(nuc_server *s, const nuc_config *cfg, const dou
 extern "C" int nuc_server_submit_ic(nuc_server *s, const nuc_config *cfg, int ic, double k0, double u_rms,
                                     unsigned long long seed, int nsteps, long long *job_id)
 {
-    if (!s || !cfg || nsteps < 0 || !job_id || (ic != NUC_IC_TGV && ic != NUC_IC_HIT)) return NUC_EINVAL;
+    if (!s || !cfg || nsteps < 0 || !job_id || ic < NUC_IC_TGV || ic > NUC_IC_DEFAULT) return NUC_EINVAL;
     if (cfg->nx < 1 || cfg->ny < 1 || cfg->nz < 1) return NUC_EINVAL;
     if (ic == NUC_IC_HIT && (!(k0 > 0) || !(u_rms >= 0))) return NUC_EINVAL;
     auto j = std::make_shared<Job>();

@@ -27,7 +27,7 @@ STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_
           6: "NUC_EBUSY"}
 NUC_OK, NUC_EBUSY = 0, 6
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = range(4)
-IC_STATE, IC_TGV, IC_HIT = range(3)
+IC_STATE, IC_TGV, IC_HIT, IC_IVC, IC_DEFAULT = range(5)
 SPEC_DENSITY, SPEC_PRESSURE, SPEC_ENSTROPHY, SPEC_HELICITY = range(4)
 
 # every entry point include/nuc3d_gpu.h and include/nuc3d_serve.h declare
@@ -41,7 +41,8 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_pack_last_error", "nuc_autotune", "nuc_tile_shape", "nuc_set_tile_shape", "nuc_rk3_stage",
            "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
            "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device",
-           "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar"]
+           "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar", "nuc_init_default",
+           "nuc_init_ivc", "nuc_init_default_device", "nuc_init_ivc_device"]
 
 
 class NucConfig(ctypes.Structure):
@@ -125,6 +126,8 @@ def lib():
         L.nuc_config_from_inp.argtypes = [ctypes.c_char_p, ctypes.POINTER(NucConfig),
                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         L.nuc_init_tgv.argtypes = [ctypes.POINTER(NucConfig), _dp]
+        L.nuc_init_default.argtypes = [ctypes.POINTER(NucConfig), _dp]
+        L.nuc_init_ivc.argtypes = [ctypes.POINTER(NucConfig), _dp]
         L.nuc_init_hit.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_double,
                                    ctypes.c_ulonglong, _dp]
         L.nuc_init_shu_osher.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_double, ctypes.c_ulonglong, _dp]
@@ -149,6 +152,8 @@ def lib():
         L.nuc_spectra_kinetic.argtypes = [vp, _dp, ctypes.c_int]
         L.nuc_spectrum_scalar.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
         L.nuc_init_tgv_device.argtypes = [vp]
+        L.nuc_init_default_device.argtypes = [vp]
+        L.nuc_init_ivc_device.argtypes = [vp]
         L.nuc_init_hit_device.argtypes = [vp, ctypes.c_double, ctypes.c_double, ctypes.c_ulonglong]
         L.nuc_average_begin.argtypes = [vp, ctypes.c_int]
         L.nuc_average_update.argtypes = [vp]
@@ -223,6 +228,20 @@ def config_from_inp(directory):
 def init_tgv(cfg):
     q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
     _check(lib().nuc_init_tgv(ctypes.byref(cfg), _ptr(q)), "nuc_init_tgv")
+    return q
+
+
+def init_default(cfg):
+    """Fluid at rest, p = 1/(gamma M^2) (nuc_init_default)."""
+    q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
+    _check(lib().nuc_init_default(ctypes.byref(cfg), _ptr(q)), "nuc_init_default")
+    return q
+
+
+def init_ivc(cfg):
+    """Isentropic vortex at (5, 5) carried by u = 0.5 (nuc_init_ivc)."""
+    q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
+    _check(lib().nuc_init_ivc(ctypes.byref(cfg), _ptr(q)), "nuc_init_ivc")
     return q
 
 
@@ -309,6 +328,14 @@ class Context:
     def init_tgv_device(self):
         """The Taylor-Green field generated on the device into the state."""
         _check(lib().nuc_init_tgv_device(self.h), "nuc_init_tgv_device", self.h)
+
+    def init_default_device(self):
+        """init_default's field generated on the device into the state."""
+        _check(lib().nuc_init_default_device(self.h), "nuc_init_default_device", self.h)
+
+    def init_ivc_device(self):
+        """init_ivc's field generated on the device into the state."""
+        _check(lib().nuc_init_ivc_device(self.h), "nuc_init_ivc_device", self.h)
 
     def init_hit_device(self, k0=4.0, u_rms=1.0, seed=20261202):
         """init_hit's seeded field synthesised on the device into the state."""
@@ -518,7 +545,7 @@ class Server:
         return jid.value
 
     def submit_ic(self, cfg, ic, nsteps, k0=4.0, u_rms=1.0, seed=0):
-        """A job whose initial state (IC_TGV or IC_HIT) is generated on the device."""
+        """A job whose initial state (IC_TGV, IC_HIT, IC_IVC or IC_DEFAULT) is generated on the device."""
         jid = ctypes.c_longlong()
         _check(lib().nuc_server_submit_ic(self.h, ctypes.byref(cfg), int(ic), float(k0), float(u_rms), int(seed),
                                           int(nsteps), ctypes.byref(jid)), "nuc_server_submit_ic")

@@ -976,6 +976,53 @@ void orc_init_tgv(const orc_config *c, double *q)
             }
 }
 
+/* initial_default (src/block.cpp:285-314): fluid at rest.  u_init, v_init and
+ * w_init are read with getValue, whose map does not hold them and which
+ * PhysModel.in cannot set (an unknown word is an error), so they are 0. */
+void orc_init_default(const orc_config *c, double *q)
+{
+    long N = (long)c->nx * c->ny * c->nz;
+    double gamma = c->gamma, mach = c->mach;
+    for (long id = 0; id < N; id++) {
+        q[id] = 1.0;
+        q[N + id] = 0.0;
+        q[2 * N + id] = 0.0;
+        q[3 * N + id] = 0.0;
+        q[4 * N + id] = 1.0 / (mach * mach * gamma) / (gamma - 1.0);
+    }
+}
+
+/* initial_ivc (src/block.cpp:316-357): the isentropic vortex centred at
+ * (5, 5), strength b = 0.5, advected by u = 0.5; p = rho^gamma, unscaled by
+ * the Mach number, and the density exponent 2.5 as written. */
+void orc_init_ivc(const orc_config *c, double *q)
+{
+    int nl[3] = {c->nx / c->npx, c->ny / c->npy, c->nz / c->npz};
+    double dx = c->lx / c->nx, dy = c->ly / c->ny;
+    long N = (long)c->nx * c->ny * c->nz;
+    double gamma = c->gamma;
+    double pie = 4.0 * atan(1.0);
+    double b = 0.5, x_c = 5.0, y_c = 5.0;
+    for (int k = 0; k < c->nz; k++)
+        for (int j = 0; j < c->ny; j++)
+            for (int i = 0; i < c->nx; i++) {
+                double x0 = c->lx / c->npx * (i / nl[0]) + dx * (0.5 + i % nl[0]);
+                double y0 = c->ly / c->npy * (j / nl[1]) + dy * (0.5 + j % nl[1]);
+                double r = sqrt(pow(x0 - x_c, 2) + pow(y0 - y_c, 2));
+                double rho0 = pow((1 - (gamma - 1.0) * b * b * exp(1 - r * r) / (8.0 * gamma * pie * pie)), 2.5);
+                double u0 = 0.5 - b / (2.0 * pie) * exp((1 - r * r) / 2) * (y0 - y_c);
+                double v0 = b / (2.0 * pie) * exp((1 - r * r) / 2) * (x0 - x_c);
+                double w0 = 0.0;
+                double p0 = pow(rho0, gamma);
+                long id = (long)i + (long)c->nx * ((long)j + (long)c->ny * k);
+                q[id] = rho0;
+                q[N + id] = rho0 * u0;
+                q[2 * N + id] = rho0 * v0;
+                q[3 * N + id] = rho0 * w0;
+                q[4 * N + id] = p0 / (gamma - 1.0) + 0.5 * rho0 * (u0 * u0 + v0 * v0 + w0 * w0);
+            }
+}
+
 void orc_stats(orc_solver *s, double *out)
 {
     const long N = s->N;

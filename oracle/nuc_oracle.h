@@ -78,6 +78,8 @@ void orc_rhs(orc_solver *s, double *rhs, double *dt_local);
 void orc_step(orc_solver *s, int nsteps, double *dt, double *res, double *time);
 /* Taylor-Green initial condition of src/block.cpp:359-395 on the rank grid. */
 void orc_init_tgv(const orc_config *cfg, double *q);
+void orc_init_default(const orc_config *c, double *q);
+void orc_init_ivc(const orc_config *c, double *q);
 /* Global statistics of postproc::solveQ (src/postproc.cpp:736-929):
  * out[0..15] = means of ux..wz(9), omega(3), enstrophy, Q, helicity, KE;
  * out[16..24] = u_rms, M_t, sqrt(div_aver), miu_aver, Re_lambda, lambda, rho_bar, tau_t, skew. */

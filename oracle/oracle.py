@@ -74,6 +74,8 @@ def lib():
         L.orc_rhs.argtypes = [ctypes.c_void_p, _dp, _dp]
         L.orc_step.argtypes = [ctypes.c_void_p, ctypes.c_int, _dp, _dp, _dp]
         L.orc_init_tgv.argtypes = [ctypes.POINTER(OrcConfig), _dp]
+        L.orc_init_default.argtypes = [ctypes.POINTER(OrcConfig), _dp]
+        L.orc_init_ivc.argtypes = [ctypes.POINTER(OrcConfig), _dp]
         L.orc_stats.argtypes = [ctypes.c_void_p, _dp]
         L.orc_post_create.restype = ctypes.c_void_p
         L.orc_post_create.argtypes = [ctypes.c_void_p, ctypes.c_int]
@@ -220,6 +222,20 @@ def init_tgv(cfg):
     N = cfg.nx * cfg.ny * cfg.nz
     q = np.zeros(5 * N)
     lib().orc_init_tgv(ctypes.byref(cfg), _ptr(q))
+    return q
+
+
+def init_default(cfg):
+    N = cfg.nx * cfg.ny * cfg.nz
+    q = np.zeros(5 * N)
+    lib().orc_init_default(ctypes.byref(cfg), _ptr(q))
+    return q
+
+
+def init_ivc(cfg):
+    N = cfg.nx * cfg.ny * cfg.nz
+    q = np.zeros(5 * N)
+    lib().orc_init_ivc(ctypes.byref(cfg), _ptr(q))
     return q
 
 

@@ -1,13 +1,15 @@
 """Initial conditions generated on the device (nuc_init_tgv_device,
-nuc_init_hit_device) against the host generators: the same fields up to
-rounding of the transcendental functions and of the synthesis sums, and the
-same solution after stepping."""
+nuc_init_hit_device, nuc_init_ivc_device, nuc_init_default_device) against the
+host generators and the oracle: the same fields up to rounding of the
+transcendental functions and of the synthesis sums, and the same solution after
+stepping."""
 import time
 
 import numpy as np
 import pytest
 
 import nuc3d_amd as G
+import oracle as O
 
 pytestmark = pytest.mark.gpu
 
@@ -43,6 +45,42 @@ def test_hit_device_equals_host(n, L, k0):
     err = np.abs(a - b).max(axis=1) / scale
     print(f"HIT {n} device vs host", err, flush=True)
     assert np.all(err <= 1e-13), err
+
+
+@pytest.mark.parametrize("npr", [(1, 1, 1), (2, 2, 1)], ids=str)
+def test_ivc_and_default_device_equal_host_and_oracle(npr):
+    kw = dict(n=(40, 32, 8), L=(10.0, 10.0, 2.5), npr=npr, mach=0.3)
+    cfg, oc = G.make_config(**kw), O.make_config(**kw)
+    N = 40 * 32 * 8
+    for name, host, orc, init in (("ivc", G.init_ivc, O.init_ivc, lambda c: c.init_ivc_device()),
+                                  ("default", G.init_default, O.init_default, lambda c: c.init_default_device())):
+        dev = _download_after(cfg, init).reshape(5, N)
+        for ref in (host(cfg).reshape(5, N), orc(oc).reshape(5, N)):
+            mom = np.abs(ref[1:4]).max() or np.abs(ref).max()   # the default field's momenta are 0
+            scale = np.maximum(np.abs(ref).max(axis=1), mom)
+            err = np.abs(dev - ref).max(axis=1) / scale
+            print(f"{name} {npr} device vs reference", err, flush=True)
+            assert np.all(err <= 1e-14), (name, err)
+
+
+def test_isentropic_vortex_ten_steps_match_oracle():
+    """The reference's Benchmark 1 case (Euler, isentropic vortex on a 10 x 10
+    box) from the device initial state, against the oracle from its own."""
+    kw = dict(n=(32, 32, 8), L=(10.0, 10.0, 2.5), npr=(2, 1, 1), scheme=G.WENO5JS, vis=G.CD6,
+              riemann=G.AUSM, model=G.EULER3D, cfl=0.5)
+    cfg, oc = G.make_config(**kw), O.make_config(**kw)
+    s = O.Solver(oc)
+    s.set_state(O.init_ivc(oc))
+    s.step(10)
+    ref = s.get_state().reshape(5, -1)
+    with G.Context(cfg) as ctx:
+        ctx.init_ivc_device()
+        ctx.step(10)
+        got = ctx.download().reshape(5, -1)
+    scale = np.maximum(np.abs(ref).max(axis=1), np.abs(ref[1:4]).max())
+    err = np.abs(got - ref).max(axis=1) / scale
+    print("isentropic vortex, 10 steps, device vs oracle", err, flush=True)
+    assert np.all(err <= 1e-12), err
 
 
 def test_stepping_from_device_initial_state():

@@ -184,12 +184,18 @@ def test_jobs_with_device_initial_conditions():
     equals the same job started from the device-generated state alone."""
     a = G.make_config(n=(32, 32, 32), scheme=G.CRWENO5, vis=G.CD6, mach=0.2, reynolds=900.0, cfl=0.5)
     b = G.make_config(n=(40, 36, 32), scheme=G.WENO5Z, vis=G.CD6, mach=0.3, reynolds=600.0, cfl=0.5)
+    c = G.make_config(n=(32, 32, 8), L=(10.0, 10.0, 2.5), scheme=G.WENO5JS, model=G.EULER3D, cfl=0.5)
+    d = G.make_config(n=(16, 16, 16), scheme=G.WENO5JS, mach=0.3, cfl=0.5)
     with G.Server(hbm_budget=4 << 30, max_active=4) as srv:
         ja = srv.submit_ic(a, G.IC_HIT, 4, k0=3.0, u_rms=0.5, seed=12)
         jb = srv.submit_ic(b, G.IC_TGV, 4)
-        ra, rb = _wait(srv, ja), _wait(srv, jb)
+        jc = srv.submit_ic(c, G.IC_IVC, 4)
+        jd = srv.submit_ic(d, G.IC_DEFAULT, 4)
+        ra, rb, rc, rd = _wait(srv, ja), _wait(srv, jb), _wait(srv, jc), _wait(srv, jd)
     for cfg, res, init in ((a, ra, lambda c: c.init_hit_device(k0=3.0, u_rms=0.5, seed=12)),
-                           (b, rb, lambda c: c.init_tgv_device())):
+                           (b, rb, lambda c: c.init_tgv_device()),
+                           (c, rc, lambda c: c.init_ivc_device()),
+                           (d, rd, lambda c: c.init_default_device())):
         with G.Context(cfg) as ctx:
             init(ctx)
             ctx.step(4)
