@@ -868,7 +868,7 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
     // The 7-point schemes keep FP64 stencils in the mixed path too: their
     // smoothness indicators cancel terms up to 17246 f^2, so FP32 inputs alone
     // move the state by ~1e-4 in a few steps.
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS);
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78);
     const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
     if constexpr (fp32_ok) {
@@ -908,6 +908,9 @@ void launch_sweep(int scheme, int R, const SweepArgs &A, hipStream_t s)
     case WENO5Z: launch_sweep_s<WENO5Z>(R, A, s); break;
     case WENO7JS: launch_sweep_s<WENO7JS>(R, A, s); break;
     case CRWENO5: launch_sweep_s<CRWENO5>(R, A, s); break;
+    case WENO3JS: launch_sweep_s<WENO3JS>(R, A, s); break;
+    case UPWIND1ST: launch_sweep_s<UPWIND1ST>(R, A, s); break;
+    case HCCS78: launch_sweep_s<HCCS78>(R, A, s); break;
     default: launch_sweep_s<HCCS>(R, A, s); break;
     }
 }
@@ -970,7 +973,7 @@ struct PackLaunch {
 template <int SCH, int RIE, int R>
 void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
 {
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS);
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78);
     const dim3 grid((unsigned)P.nblocks), block(64 * SWEEP_LPB);
     if constexpr (fp32_ok) {
         if (P.mixed) {
@@ -1004,6 +1007,9 @@ void launch_sweep_pack(int scheme, int riemann, int R, const PackLaunch &P, hipS
     case WENO5Z: launch_sweep_pack_s<WENO5Z>(riemann, R, P, s); break;
     case WENO7JS: launch_sweep_pack_s<WENO7JS>(riemann, R, P, s); break;
     case CRWENO5: launch_sweep_pack_s<CRWENO5>(riemann, R, P, s); break;
+    case WENO3JS: launch_sweep_pack_s<WENO3JS>(riemann, R, P, s); break;
+    case UPWIND1ST: launch_sweep_pack_s<UPWIND1ST>(riemann, R, P, s); break;
+    case HCCS78: launch_sweep_pack_s<HCCS78>(riemann, R, P, s); break;
     default: launch_sweep_pack_s<HCCS>(riemann, R, P, s); break;
     }
 }
@@ -1109,14 +1115,34 @@ struct ProfScope {
         }                                                                                     \
     } while (0)
 
+// bufferSize per scheme (src/fieldOperator.cpp:128-181)
+static int scheme_halo(int scheme)
+{
+    if (scheme == NUC_WENO7JS || scheme == NUC_HCCS || scheme == NUC_HCCS78) return 4;
+    if (scheme == NUC_WENO3JS) return 2;
+    if (scheme == NUC_UPWIND1ST) return 1;
+    return 3;
+}
+
+// cells on each side a viscous scheme reads (cd2 1, cd6 3, cd8 4)
+static int vis_reach(int vis) { return vis == NUC_CD8 ? 4 : (vis == NUC_CD2 ? 1 : 3); }
+
+static int vis_id(int vis) { return vis == NUC_CD8 ? VIS_CD8 : (vis == NUC_CD2 ? VIS_CD2 : VIS_CD6); }
+
+// The reference hands the viscous operators the inviscid halo as their
+// tilesize (src/fieldOperator.cpp:520-526) and sizes the ghost strips by the
+// inviscid halos (src/block.cpp:170-171): cd8 behind a 3-wide scheme and cd6
+// behind weno3js / upwind1st read outside the strip.  Widened to the reach.
+static int vis_tilesize(int scheme, int vis) { return std::max(scheme_halo(scheme), vis_reach(vis)); }
+
 static int validate(const nuc_config *c, std::string &err)
 {
     int n[3] = {c->nx, c->ny, c->nz}, np[3] = {c->npx, c->npy, c->npz};
     for (int d = 0; d < 3; d++) {
         if (np[d] < 1 || n[d] % np[d] != 0) { err = "grid size must divide evenly by the rank grid (src/block.cpp:56-58)"; return NUC_EINVAL; }
-        if (c->scheme[d] < 0 || c->scheme[d] > 4) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
-        if (c->vis_scheme[d] < 0 || c->vis_scheme[d] > 1) { err = "unknown viscous scheme id"; return NUC_EINVAL; }
-        int ts = (c->scheme[d] == NUC_WENO7JS || c->scheme[d] == NUC_HCCS) ? 4 : 3;
+        if (c->scheme[d] < 0 || c->scheme[d] > NUC_HCCS78) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
+        if (c->vis_scheme[d] < 0 || c->vis_scheme[d] > NUC_CD2) { err = "unknown viscous scheme id"; return NUC_EINVAL; }
+        int ts = vis_tilesize(c->scheme[d], c->vis_scheme[d]);
         if (n[d] / np[d] < 2 * ts + 2) { err = "rank-local line too short for the scheme's halo"; return NUC_EINVAL; }
         if (n[d] > 512) { err = "line length above 512 is not supported"; return NUC_EINVAL; }
     }
@@ -1158,18 +1184,13 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     double Lb[3] = {cfg->lx, cfg->ly, cfg->lz};
     x->N = (long)cfg->nx * cfg->ny * cfg->nz;
     x->P = cfg->npx * cfg->npy * cfg->npz;
-    bool any_cd8 = false;
     for (int d = 0; d < 3; d++) {
         x->nloc[d] = x->n[d] / x->np[d];
         x->h[d] = Lb[d] / x->n[d];
-        x->ts[d] = (cfg->scheme[d] == NUC_WENO7JS || cfg->scheme[d] == NUC_HCCS) ? 4 : 3;
-        // cd8 needs four ghost cells; the reference hands it the inviscid halo
-        // (src/fieldOperator.cpp:520-526) and reads outside the strip when that is 3
-        x->tsv[d] = (cfg->vis_scheme[d] == NUC_CD8 && x->ts[d] < 4) ? 4 : x->ts[d];
+        x->ts[d] = scheme_halo(cfg->scheme[d]);
+        x->tsv[d] = vis_tilesize(cfg->scheme[d], cfg->vis_scheme[d]);
         x->cpt[d] = rows_for(x->n[d]);
-        if (cfg->vis_scheme[d] == NUC_CD8) any_cd8 = true;
     }
-    (void)any_cd8;
     const size_t N = (size_t)x->N;
     auto fail = [&](const char *what) {
         x->err = what;
@@ -1300,7 +1321,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         G.nx = x->n[0]; G.ny = x->n[1]; G.nz = x->n[2];
         for (int d = 0; d < 3; d++) {
             G.nloc[d] = x->nloc[d]; G.tsv[d] = x->tsv[d];
-            G.vis[d] = (c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6;
+            G.vis[d] = vis_id(c.vis_scheme[d]);
             G.inv_h[d] = 1.0 / x->h[d];
         }
         G.law = c.viscosity; G.re = c.reynolds; G.mach = c.mach; G.pr = c.prandtl; G.gamma = c.gamma;
@@ -1314,7 +1335,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
     for (int d = 0; d < 3; d++) {
         SweepArgs &S = sa.sw[d];
         S.dir = d; S.n = x->n[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
-        S.vis = ns ? ((c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6) : VIS_NONE;
+        S.vis = ns ? vis_id(c.vis_scheme[d]) : VIS_NONE;
         S.riemann = c.riemann;
         S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
         S.nlines = N / x->n[d];
@@ -1719,7 +1740,7 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
     S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
     for (int d = 0; d < 3; d++) {
         S.nloc[d] = x->nloc[d]; S.tsv[d] = x->tsv[d];
-        S.vis[d] = (c.vis_scheme[d] == NUC_CD8) ? VIS_CD8 : VIS_CD6;
+        S.vis[d] = vis_id(c.vis_scheme[d]);
         S.inv_h[d] = 1.0 / x->h[d];
     }
     S.gamma = c.gamma; S.mach = c.mach; S.tref = c.t_ref; S.tinf = c.t_inf; S.part = x->stat_part;
@@ -1773,7 +1794,7 @@ extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const do
 {
     if (!in || !out || n1 < 2 * tilesize + 1 || n2 < 1 || n3 < 1 || (upwind != 1 && upwind != -1)) return NUC_EINVAL;
     if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < tilesize)) return NUC_EINVAL;
-    if (scheme < 0 || scheme > 4 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
+    if (scheme < 0 || scheme > NUC_HCCS78 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
     int cpt = rows_for(n1 + 1);
     if (cpt < 0) return NUC_EINVAL;
     long nl = (long)n2 * n3;
@@ -1795,6 +1816,9 @@ extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const do
     case WENO5Z: launch_op_sch<WENO5Z>(cpt, A, 0); break;
     case WENO7JS: launch_op_sch<WENO7JS>(cpt, A, 0); break;
     case CRWENO5: launch_op_sch<CRWENO5>(cpt, A, 0); break;
+    case WENO3JS: launch_op_sch<WENO3JS>(cpt, A, 0); break;
+    case UPWIND1ST: launch_op_sch<UPWIND1ST>(cpt, A, 0); break;
+    case HCCS78: launch_op_sch<HCCS78>(cpt, A, 0); break;
     default: launch_op_sch<HCCS>(cpt, A, 0); break;
     }
     if (hipGetLastError() != hipSuccess) return NUC_EHIP;
@@ -1806,9 +1830,10 @@ extern "C" int nuc_op_diff(int vis_scheme, int tilesize, const double *in, doubl
                            const double *gL, const double *gR, int bf, double *out, int zones)
 {
     if (!in || !out || n1 < 2 * tilesize || n2 < 1 || n3 < 1 || !(h != 0.0)) return NUC_EINVAL;
-    int need = (vis_scheme == NUC_CD8) ? 4 : 3;
+    if (vis_scheme < 0 || vis_scheme > NUC_CD2) return NUC_EINVAL;
+    int need = vis_reach(vis_scheme);
     if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < need)) return NUC_EINVAL;
-    if (vis_scheme == NUC_CD8 && tilesize < 4) return NUC_EINVAL;  // the reference's out-of-stencil read
+    if (tilesize < need) return NUC_EINVAL;  // the reference's out-of-stencil read
     long nl = (long)n2 * n3;
     DevBuf di, dl, dr, dout;
     if (!di.alloc(nl * n1) || !dout.alloc(nl * n1)) return NUC_ENOMEM;
@@ -1822,7 +1847,7 @@ extern "C" int nuc_op_diff(int vis_scheme, int tilesize, const double *in, doubl
         if (!dr.alloc(nl * bf)) return NUC_ENOMEM;
         if (hipMemcpy(dr.p, gR, nl * bf * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) return NUC_EHIP;
     }
-    OpDiffArgs A{di.p, dl.p, dr.p, dout.p, n1, bf, tilesize, vis_scheme == NUC_CD8 ? VIS_CD8 : VIS_CD6, zones, nl,
+    OpDiffArgs A{di.p, dl.p, dr.p, dout.p, n1, bf, tilesize, vis_id(vis_scheme), zones, nl,
                  1.0 / h};
     long tot = nl * n1;
     hipLaunchKernelGGL(k_op_diff, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, 0, A);
@@ -2207,7 +2232,7 @@ extern "C" int nuc_average_update(nuc_ctx *x)
     const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
     auto diff = [&](const double *in, double *out, int d) {
         PostDiffArgs D{gp(in), gp(out), N, x->n[0], x->n[1], d, x->n[d], x->nloc[d], x->tsv[d],
-                       c.vis_scheme[d] == NUC_CD8 ? VIS_CD8 : VIS_CD6, 1.0 / x->h[d]};
+                       vis_id(c.vis_scheme[d]), 1.0 / x->h[d]};
         hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
     };
     auto F = [&](long seg, int f) { return b + seg + (long)f * N; };
@@ -2737,7 +2762,7 @@ static int spectra_run(nuc_ctx *x, int what, double *out, int kmax_out)
             for (int d = 0; d < 3; d++) {
                 PostDiffArgs D{gp((const double *)(uvw + (size_t)f * N)), gp(grad + (size_t)(3 * f + d) * N), N,
                                x->n[0], x->n[1], d, x->n[d], x->nloc[d], x->tsv[d],
-                               x->c.vis_scheme[d] == NUC_CD8 ? VIS_CD8 : VIS_CD6, 1.0 / x->h[d]};
+                               vis_id(x->c.vis_scheme[d]), 1.0 / x->h[d]};
                 hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
             }
         P.grad = gp((const double *)grad);

@@ -9,14 +9,15 @@
 
 namespace nucdev {
 
-enum { WENO5JS = 0, WENO5Z = 1, WENO7JS = 2, CRWENO5 = 3, HCCS = 4 };
-enum { VIS_NONE = 0, VIS_CD6 = 1, VIS_CD8 = 2 };
+enum { WENO5JS = 0, WENO5Z = 1, WENO7JS = 2, CRWENO5 = 3, HCCS = 4, WENO3JS = 5, UPWIND1ST = 6, HCCS78 = 7 };
+enum { VIS_NONE = 0, VIS_CD6 = 1, VIS_CD8 = 2, VIS_CD2 = 3 };
 enum { RIE_LF = 0, RIE_AUSM = 1 };
 
 template <typename T> __device__ __forceinline__ T sq(T x) { return x * x; }
 
 // 7-point WENO with Z weights, src/weno7js.cpp:79-133 (eps 1e-20); hccs
-// closures use the same formula with eps 1e-40 (src/hccs.cpp:85-135).
+// closures use the same formula with eps 1e-40 (src/hccs.cpp:85-135), hccs78
+// edges with 1e-20 (src/hccs78.cpp:246-304).
 // The smoothness indicators are expanded quadratic forms whose coefficients
 // (up to 17246) cancel to O(h^2) on smooth data, so any reassociation or FMA
 // moves them by ~1e-6 relative and shifts the Z weights.  This function is
@@ -24,8 +25,9 @@ template <typename T> __device__ __forceinline__ T sq(T x) { return x * x; }
 // order, which makes it bit-identical to the reference for the indicators.
 // T: indicators and candidate values; TW: the nonlinear weights (the Z
 // weights square tau/(eps+is) with eps 1e-20/1e-40, outside FP32's range).
+// asum receives the sum of the unnormalised weights (hccs78's blend factor).
 template <typename T, typename TW>
-__device__ __forceinline__ TW weno7z(const T *f, TW eps)
+__device__ __forceinline__ TW weno7z(const T *f, TW eps, TW &asum)
 {
 #pragma clang fp contract(off)
     T is0 = f[0] * (T(547) * f[0] - T(3882) * f[1] + T(4642) * f[2] - T(1854) * f[3])
@@ -50,8 +52,41 @@ __device__ __forceinline__ TW weno7z(const T *f, TW eps)
     TW a1 = TW(12.0 / 35.0) * (TW(1) + sq(tau / (eps + b1)));
     TW a2 = TW(18.0 / 35.0) * (TW(1) + sq(tau / (eps + b2)));
     TW a3 = TW(4.0 / 35.0) * (TW(1) + sq(tau / (eps + b3)));
-    TW asum = a0 + a1 + a2 + a3;
+    asum = a0 + a1 + a2 + a3;
     return a0 / asum * TW(q0) + a1 / asum * TW(q1) + a2 / asum * TW(q2) + a3 / asum * TW(q3);
+}
+
+template <typename T, typename TW>
+__device__ __forceinline__ TW weno7z(const T *f, TW eps)
+{
+    TW asum;
+    return weno7z<T, TW>(f, eps, asum);
+}
+
+// hccs78 inner point, src/hccs78.cpp:66-117: the WENO7-Z value (eps 1e-20)
+// blended with the 8-point central value through theta = 1/(1 + (asum-1)^2).
+// f[0..7]: one downwind point more than WENO7.
+template <typename T, typename TW>
+__device__ __forceinline__ TW hccs78_inner(const T *f)
+{
+    TW asum;
+    TW w = weno7z<T, TW>(f, TW(1.0e-20), asum);
+    TW q8 = (TW(-3) * (TW(f[7]) + TW(f[0])) + TW(29) * (TW(f[6]) + TW(f[1])) - TW(139) * (TW(f[5]) + TW(f[2]))
+             + TW(533) * (TW(f[4]) + TW(f[3]))) / TW(840);
+    TW theta = TW(1) / (TW(1) + sq(asum - TW(1)));
+    return (TW(1) - theta) * w + theta * q8;
+}
+
+// WENO3-JS, src/weno3js.cpp:80-93 (eps 1e-6): f[0..2] upwind
+template <typename T, typename TW>
+__device__ __forceinline__ TW weno3js(const T *f)
+{
+    T is0 = sq(f[0] - f[1]), is1 = sq(f[1] - f[2]);
+    T q0 = T(-0.5) * f[0] + T(1.5) * f[1];
+    T q1 = T(0.5) * f[1] + T(0.5) * f[2];
+    TW a0 = TW(1.0 / 3.0) / sq(TW(1.0e-6) + TW(is0));
+    TW a1 = TW(2.0 / 3.0) / sq(TW(1.0e-6) + TW(is1));
+    return (a0 * TW(q0) + a1 * TW(q1)) / (a0 + a1);
 }
 
 // hccs interior row, src/hccs.cpp:138-183: Z-type CRWENO weights (eps 1e-40)

@@ -55,7 +55,13 @@ __device__ __forceinline__ void w5_weights(TW b0, TW b1, TW b2, TW c0, TW c1, TW
     }
 }
 
-template <int SCH> struct Radius { static constexpr int value = (SCH == WENO7JS || SCH == HCCS) ? 3 : 2; };
+// upwind reach of a scheme's stencil, and the points it reads beyond the
+// symmetric window (hccs78's 8-point central part)
+template <int SCH> struct Radius {
+    static constexpr int value = (SCH == WENO7JS || SCH == HCCS || SCH == HCCS78) ? 3
+                               : (SCH == WENO3JS || SCH == UPWIND1ST) ? 1 : 2;
+};
+template <int SCH> struct Extra { static constexpr int value = (SCH == HCCS78) ? 1 : 0; };
 
 // Skewed LDS line layout: position y (y >= -B) lives at slot (y+B) + (y+B)/R,
 // one pad slot after every R positions, B = HALO+1 rounded up to a multiple of
@@ -251,7 +257,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                                           TS out[R], bool full = false)
 {
     constexpr int RAD = Radius<SCH>::value;
-    constexpr int W = R + 2 * RAD;
+    constexpr int W = R + 2 * RAD + Extra<SCH>::value;
     T win[W];
 #pragma unroll
     for (int t = 0; t < W; t++) win[t] = Lw[Skew<R>::at(t - RAD - 1)];
@@ -300,6 +306,21 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
     } else if constexpr (SCH == WENO7JS) {
 #pragma unroll
         for (int r = 0; r < R; r++) out[r] = TS(weno7z<T, TW>(win + r, TW(1.0e-20)));
+    } else if constexpr (SCH == WENO3JS) {
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] = TS(weno3js<T, TW>(win + r));
+    } else if constexpr (SCH == UPWIND1ST) {
+        // src/upwind1st.cpp:56-113: the upwind cell itself
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] = TS(win[r + 1]);
+    } else if constexpr (SCH == HCCS78) {
+        // blend on the inner interfaces ts..nloc-ts (src/hccs78.cpp:56-60),
+        // WENO7-Z on the edge interfaces (:207-304)
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            bool edge = ml[r] < z.ts || ml[r] > z.nloc - z.ts;
+            out[r] = edge ? TS(weno7z<T, TW>(win + r, TW(1.0e-20))) : TS(hccs78_inner<T, TW>(win + r));
+        }
     } else if constexpr (SCH == CRWENO5) {
         Chunk<R, TS> ch;
 #pragma unroll
@@ -356,12 +377,13 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
     }
 }
 
-// First derivative at cell c of a line buffer L (cd6 / cd8); `inner` selects
+// First derivative at cell c of a line buffer L (cd2 / cd6 / cd8); `inner` selects
 // the inner-loop form of cd8 (src/centraldifference6th.cpp:42-48,
 // src/centraldifference8th.cpp:52-66 inner, :127-131 edges).
 template <typename T>
 __device__ __forceinline__ T central_diff(const T *L, int c, bool inner, int vis, T inv_h)
 {
+    if (vis == VIS_CD2) return T(0.5) * (L[c + 1] - L[c - 1]) * inv_h;   // src/centraldifference2nd.cpp:38-40
     if (vis == VIS_CD6) {
         return (T(1.0 / 60.0) * (L[c + 3] - L[c - 3]) + T(-3.0 / 20.0) * (L[c + 2] - L[c - 2])
                 + T(3.0 / 4.0) * (L[c + 1] - L[c - 1])) * inv_h;

@@ -35,6 +35,9 @@ int scheme_from_name(const std::string &s, int *halo)
     if (s == "weno5js") { *halo = 3; return NUC_WENO5JS; }
     if (s == "weno5z") { *halo = 3; return NUC_WENO5Z; }
     if (s == "crweno5") { *halo = 3; return NUC_CRWENO5; }
+    if (s == "hccs78") { *halo = 4; return NUC_HCCS78; }
+    if (s == "weno3js") { *halo = 2; return NUC_WENO3JS; }
+    if (s == "upwind1st") { *halo = 1; return NUC_UPWIND1ST; }
     *halo = 3;
     return -1;
 }
@@ -78,7 +81,7 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
             const std::string &v = method[vis[d]];
             if (v == "cd8") c.vis_scheme[d] = NUC_CD8;
             else if (v == "cd6") c.vis_scheme[d] = NUC_CD6;
-            else if (v == "cd2") { err = "Method.in: cd2 is not available"; return NUC_EINVAL; }
+            else if (v == "cd2") c.vis_scheme[d] = NUC_CD2;
             else c.vis_scheme[d] = NUC_CD6;  // src/fieldOperator.cpp:309-312 default
         }
     }

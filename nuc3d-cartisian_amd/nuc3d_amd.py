@@ -14,8 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnuc3d_amd.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 
-WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS = range(5)
-CD6, CD8 = 0, 1
+WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78 = range(8)
+CD6, CD8, CD2 = 0, 1, 2
 LF, AUSM = 0, 1
 EULER3D, NAVIERSTOKES3D = 0, 1
 SUTHERLAND, CONSTANT = 0, 1

@@ -57,9 +57,19 @@ static const double CD6[3] = {3.0 / 4.0, -3.0 / 20.0, 1.0 / 60.0};
 static volatile double P_MEMBER = 2.0;
 #define POWP(x) pow((x), P_MEMBER)
 
+/* bufferSize per scheme, src/fieldOperator.cpp:128-181 */
 int orc_scheme_halo(int scheme)
 {
-    return (scheme == ORC_WENO7JS || scheme == ORC_HCCS) ? 4 : 3;
+    if (scheme == ORC_WENO7JS || scheme == ORC_HCCS || scheme == ORC_HCCS78) return 4;
+    if (scheme == ORC_WENO3JS) return 2;
+    if (scheme == ORC_UPWIND1ST) return 1;
+    return 3;
+}
+
+/* cells on each side a viscous scheme reads: cd2 1, cd6 3, cd8 4 */
+static int vis_reach(int vis)
+{
+    return vis == ORC_CD8 ? 4 : (vis == ORC_CD2 ? 1 : 3);
 }
 
 /* ---------------------------------------------------------------------------
@@ -111,8 +121,9 @@ static double weno5z_pt(const double *f, int typo)
 }
 
 /* 7-point WENO with Z weights (src/weno7js.cpp:79-133; hccs closures use the
- * same formula with eps=1e-40, src/hccs.cpp:85-135).  f[3+z], z=-3..3. */
-static double weno7z_pt(const double *f, double eps, int member_p)
+ * same formula with eps=1e-40, src/hccs.cpp:85-135; hccs78 edges with 1e-20,
+ * src/hccs78.cpp:246-304).  f[3+z], z=-3..3.  *asum receives alphaSum. */
+static double weno7z_core(const double *f, double eps, int member_p, double *asum)
 {
     double is0 = f[0] * (547.0 * f[0] - 3882.0 * f[1] + 4642.0 * f[2] - 1854.0 * f[3])
                + f[1] * (7043.0 * f[1] - 17246.0 * f[2] + 7042.0 * f[3])
@@ -144,7 +155,40 @@ static double weno7z_pt(const double *f, double eps, int member_p)
     double omega1 = alpha1 / alphaSum;
     double omega2 = alpha2 / alphaSum;
     double omega3 = alpha3 / alphaSum;
+    *asum = alphaSum;
     return omega0 * q40 + omega1 * q41 + omega2 * q42 + omega3 * q43;
+}
+
+static double weno7z_pt(const double *f, double eps, int member_p)
+{
+    double asum;
+    return weno7z_core(f, eps, member_p, &asum);
+}
+
+/* hccs78 inner point (src/hccs78.cpp:66-117): the WENO7-Z value blended with
+ * the 8-point central value q8 through theta = 1/(1 + (alphaSum-1)^2).
+ * f[3+z], z=-3..4 (one downwind point more than WENO7). */
+static double hccs78_pt(const double *f)
+{
+    double q8 = (-3.0 * (f[7] + f[0]) + 29.0 * (f[6] + f[1]) - 139.0 * (f[5] + f[2]) + 533.0 * (f[4] + f[3])) / 840.0;
+    double alphaSum;
+    double w = weno7z_core(f, EPS_7, 1, &alphaSum);
+    double theta = 1.0 / (1.0 + POWP(alphaSum - 1.0));
+    return (1.0 - theta) * w + theta * q8;
+}
+
+/* WENO3-JS, src/weno3js.cpp:80-93: f[1+z], z=-1..1 */
+static double weno3js_pt(const double *f)
+{
+    double is0 = pow((f[0] - f[1]), 2);
+    double is1 = pow((f[1] - f[2]), 2);
+    double q20 = -0.5 * f[0] + 1.5 * f[1];
+    double q21 = 0.5 * f[1] + 0.5 * f[2];
+    double aa0 = 1.0 / 3.0 / POWP(EPS_JS + is0);
+    double aa1 = 2.0 / 3.0 / POWP(EPS_JS + is1);
+    double w0 = aa0 / (aa0 + aa1);
+    double w1 = aa1 / (aa0 + aa1);
+    return w0 * q20 + w1 * q21;
 }
 
 /* CRWENO5 interior row, upwind coordinates (src/crweno5.cpp:133-162): the
@@ -200,19 +244,39 @@ static void hccs_row(const double *f, double *lo, double *di, double *hi, double
     }
 }
 
-/* gather a 5- or 7-point stencil for interface m in upwind coordinates */
-static inline void gather(const double *f, int m, int upwind, int r, double *st)
+/* gather a stencil z = -r .. r+extra for interface m in upwind coordinates */
+static inline void gather_ext(const double *f, int m, int upwind, int r, int extra, double *st)
 {
     if (upwind > 0)
-        for (int z = -r; z <= r; z++) st[r + z] = f[m - 1 + z];
+        for (int z = -r; z <= r + extra; z++) st[r + z] = f[m - 1 + z];
     else
-        for (int z = -r; z <= r; z++) st[r + z] = f[m - z];
+        for (int z = -r; z <= r + extra; z++) st[r + z] = f[m - z];
+}
+
+static inline void gather(const double *f, int m, int upwind, int r, double *st)
+{
+    gather_ext(f, m, upwind, r, 0, st);
 }
 
 static double explicit_pt(int scheme, int upwind, const double *f, int m, int inner)
 {
-    double st[7];
+    double st[8];
     switch (scheme) {
+    case ORC_WENO3JS:
+        gather(f, m, upwind, 1, st);
+        return weno3js_pt(st);
+    case ORC_UPWIND1ST:
+        /* src/upwind1st.cpp:56-74 (+: cell m-1), :95-113 (-: cell m) */
+        gather(f, m, upwind, 1, st);
+        return st[1];
+    case ORC_HCCS78:
+        /* inner: blend (src/hccs78.cpp:47-117); edges: WENO7-Z (:207-304) */
+        if (!inner) {
+            gather(f, m, upwind, 3, st);
+            return weno7z_pt(st, EPS_7, 1);
+        }
+        gather_ext(f, m, upwind, 3, 1, st);
+        return hccs78_pt(st);
     case ORC_WENO5JS:
     case ORC_CRWENO5:
         gather(f, m, upwind, 2, st);
@@ -281,6 +345,11 @@ void orc_line_reconstruct(int scheme, int upwind, const double *f, int n, int ts
 
 void orc_line_diff(int scheme, const double *f, int n, int ts, double h, double *out)
 {
+    if (scheme == ORC_CD2) {
+        /* src/centraldifference2nd.cpp:38-40 (inner) and :75-78, :114-117 (edges) */
+        for (int i = 0; i < n; i++) out[i] = 0.5 * (f[i + 1] - f[i - 1]) / h;
+        return;
+    }
     if (scheme == ORC_CD6) {
         /* src/centraldifference6th.cpp:42-48 (inner) and :113-115,175-177 (edges): same expression */
         for (int i = 0; i < n; i++)
@@ -527,22 +596,22 @@ orc_solver *orc_create(const orc_config *cfg)
     s->np[0] = cfg->npx; s->np[1] = cfg->npy; s->np[2] = cfg->npz;
     double L[3] = {cfg->lx, cfg->ly, cfg->lz};
     s->N = (long)cfg->nx * cfg->ny * cfg->nz;
-    int any_cd8 = 0;
     for (int d = 0; d < 3; d++) {
         s->nloc[d] = s->n[d] / s->np[d];
         s->h[d] = L[d] / s->n[d];                 /* dx = xl/nx_global, src/block.cpp:62-64 */
         s->ts[d] = orc_scheme_halo(cfg->scheme[d]);
-        if (cfg->vis_scheme[d] == ORC_CD8) any_cd8 = 1;
     }
     s->bf = 0;
     for (int d = 0; d < 3; d++) {
-        /* cd8 needs 4 ghost cells; the reference passes the inviscid halo as its
-         * tilesize (src/fieldOperator.cpp:520-526) and reads outside its ghost
-         * strip when that halo is 3.  Widened here to 4. */
-        s->tsv[d] = (cfg->vis_scheme[d] == ORC_CD8 && s->ts[d] < 4) ? 4 : s->ts[d];
-        if (s->ts[d] > s->bf) s->bf = s->ts[d];
+        /* The reference passes the inviscid halo as the viscous tilesize
+         * (src/fieldOperator.cpp:520-526) and sizes the ghost strips by the
+         * inviscid halos only (src/block.cpp:170-171), so cd8 behind a 3-wide
+         * scheme, and cd6 behind weno3js or upwind1st, read outside their
+         * strips.  Widened here to the viscous stencil's reach. */
+        int need = vis_reach(cfg->vis_scheme[d]);
+        s->tsv[d] = s->ts[d] < need ? need : s->ts[d];
+        if (s->tsv[d] > s->bf) s->bf = s->tsv[d];
     }
-    if (any_cd8 && s->bf < 4) s->bf = 4;
     size_t N = (size_t)s->N;
     s->q = (double *)calloc(N * 5, sizeof(double));
     s->qn = (double *)calloc(N * 5, sizeof(double));

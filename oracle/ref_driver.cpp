@@ -1,7 +1,9 @@
 // ref_driver.cpp -- extern "C" entry points around the reference's own scheme
 // classes, compiled together with the unmodified reference sources
 // (src/weno5js.cpp, src/weno5z.cpp, src/weno7js.cpp, src/crweno5.cpp,
-// src/centraldifference6th.cpp, src/centraldifference8th.cpp, src/TVD-RK3.cpp)
+// src/weno3js.cpp, src/upwind1st.cpp, src/hccs78.cpp,
+// src/centraldifference2nd.cpp, src/centraldifference6th.cpp,
+// src/centraldifference8th.cpp, src/TVD-RK3.cpp)
 // into oracle/_ref/libnucref.so by oracle/Makefile.
 //
 // TEST INFRASTRUCTURE ONLY: used by tests/ to pin the C restatement in
@@ -13,6 +15,10 @@
 #include "weno5z.hpp"
 #include "weno7js.hpp"
 #include "crweno5.hpp"
+#include "weno3js.hpp"
+#include "upwind1st.hpp"
+#include "hccs78.hpp"
+#include "centraldifference2nd.hpp"
 #include "centraldifference6th.h"
 #include "centraldifference8th.hpp"
 #include "TVD-RK3.h"
@@ -29,7 +35,8 @@ static Field make_field(const double *src, int n1, int n2, int n3)
 
 extern "C" {
 
-// scheme ids match oracle/nuc_oracle.h: 0 weno5js, 1 weno5z, 2 weno7js, 3 crweno5
+// scheme ids match oracle/nuc_oracle.h: 0 weno5js, 1 weno5z, 2 weno7js, 3 crweno5,
+// 5 weno3js, 6 upwind1st, 7 hccs78
 int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int n1, int n2, int n3,
                     const double *gL, const double *gR, int bf, double *out)
 {
@@ -42,6 +49,9 @@ int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int 
     nuc3d::weno5z s1;
     nuc3d::weno7js s2;
     nuc3d::crweno5 s3;
+    nuc3d::weno3js s5;
+    nuc3d::upwind1st s6;
+    nuc3d::hccs78 s7;
     switch (scheme) {
     case 0: s0.interpolationInner(fin, upwind, fo, tilesize); s0.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
             s0.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
@@ -51,6 +61,12 @@ int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int 
             s2.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
     case 3: s3.interpolationInner(fin, upwind, fo, tilesize); s3.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
             s3.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
+    case 5: s5.interpolationInner(fin, upwind, fo, tilesize); s5.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
+            s5.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
+    case 6: s6.interpolationInner(fin, upwind, fo, tilesize); s6.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
+            s6.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
+    case 7: s7.interpolationInner(fin, upwind, fo, tilesize); s7.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
+            s7.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
     default: return -1;
     }
     (void)op;
@@ -58,7 +74,7 @@ int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int 
     return 0;
 }
 
-// scheme: 0 cd6, 1 cd8
+// scheme: 0 cd6, 1 cd8, 2 cd2
 int ref_diff(int scheme, int tilesize, const double *in, double h, int n1, int n2, int n3,
              const double *gL, const double *gR, int bf, double *out)
 {
@@ -73,6 +89,11 @@ int ref_diff(int scheme, int tilesize, const double *in, double h, int n1, int n
         d.differentialBoundaryR(fin, h, fr, fo, tilesize);
     } else if (scheme == 1) {
         nuc3d::centraldifference8th d;
+        d.differentialInner(fin, h, fo, tilesize);
+        d.differentialBoundaryL(fin, h, fl, fo, tilesize);
+        d.differentialBoundaryR(fin, h, fr, fo, tilesize);
+    } else if (scheme == 2) {
+        nuc3d::centraldifference2nd d;
         d.differentialInner(fin, h, fo, tilesize);
         d.differentialBoundaryL(fin, h, fl, fo, tilesize);
         d.differentialBoundaryR(fin, h, fr, fo, tilesize);

@@ -37,7 +37,8 @@ def _close(a, b):
     return err.max()
 
 
-@pytest.mark.parametrize("scheme", [G.WENO5JS, G.WENO5Z, G.WENO7JS, G.CRWENO5, G.HCCS])
+@pytest.mark.parametrize("scheme", [G.WENO5JS, G.WENO5Z, G.WENO7JS, G.CRWENO5, G.HCCS, G.WENO3JS, G.UPWIND1ST,
+                                    G.HCCS78])
 @pytest.mark.parametrize("upwind", [1, -1])
 @pytest.mark.parametrize("kind", ["smooth", "random"])
 def test_reconstruct_matches_oracle(scheme, upwind, kind):
@@ -51,7 +52,7 @@ def test_reconstruct_matches_oracle(scheme, upwind, kind):
         assert _close(got, ref) <= TOL, (n1, _close(got, ref))
 
 
-@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5, G.HCCS])
+@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5, G.HCCS, G.WENO3JS, G.HCCS78])
 def test_reconstruct_discontinuous(scheme):
     """Step data drives the WENO weights and the hccs theta switch far from ideal."""
     rng = np.random.default_rng(5 + scheme)
@@ -64,7 +65,7 @@ def test_reconstruct_discontinuous(scheme):
             assert _close(got, ref) <= 1e-11, (n1, up, _close(got, ref))
 
 
-@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5])
+@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5, G.HCCS78])
 def test_reconstruct_zones(scheme):
     """interpolationInner writes only out[ts..n-ts]; BoundaryL/R only the edges."""
     rng = np.random.default_rng(2)
@@ -79,7 +80,7 @@ def test_reconstruct_zones(scheme):
     assert _close(left[:, :, :ts], ref[:, :, :ts]) <= TOL and np.all(left[:, :, ts:] == sentinel)
 
 
-@pytest.mark.parametrize("scheme,ts", [(G.CD6, 3), (G.CD6, 4), (G.CD8, 4)])
+@pytest.mark.parametrize("scheme,ts", [(G.CD6, 3), (G.CD6, 4), (G.CD8, 4), (G.CD2, 1), (G.CD2, 2), (G.CD2, 4)])
 def test_diff_matches_oracle(scheme, ts):
     rng = np.random.default_rng(9)
     for (n1, n2, n3) in SHAPES[:6]:

@@ -39,7 +39,7 @@ def _perturbed_tgv(oc, seed, amp=0.2):
     return q.ravel().copy()
 
 
-ILL_CONDITIONED = (G.WENO7JS, G.HCCS)
+ILL_CONDITIONED = (G.WENO7JS, G.HCCS, G.HCCS78)
 
 
 def _ulp_floor(oc, q, steps):
@@ -79,6 +79,15 @@ CASES = [
     ((32, 32, 32), (1, 2, 1), G.HCCS, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
     ((32, 24, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.EULER3D),
     ((20, 20, 20), (1, 1, 1), G.WENO7JS, G.CD6, G.AUSM, G.EULER3D),
+    # weno3js, upwind1st, hccs78 and cd2 (src/weno3js.cpp, src/upwind1st.cpp,
+    # src/hccs78.cpp, src/centraldifference2nd.cpp)
+    ((16, 16, 16), (1, 1, 1), G.WENO3JS, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((24, 20, 18), (2, 1, 1), G.UPWIND1ST, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((16, 16, 16), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((32, 32, 32), (1, 2, 1), G.HCCS78, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((24, 20, 18), (1, 1, 1), G.WENO5JS, G.CD2, G.LF, G.NAVIERSTOKES3D),
+    ((20, 20, 20), (2, 2, 1), G.WENO3JS, G.CD2, G.AUSM, G.NAVIERSTOKES3D),
+    ((32, 24, 16), (1, 1, 1), G.UPWIND1ST, G.CD6, G.LF, G.EULER3D),
 ]
 
 
@@ -261,6 +270,9 @@ LONG_CASES = [
     ((256, 12, 12), (1, 1, 1), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
     ((12, 160, 12), (1, 1, 1), G.WENO7JS, G.CD8, G.LF, G.NAVIERSTOKES3D),
     ((12, 12, 512), (1, 1, 1), G.WENO5Z, G.CD6, G.LF, G.EULER3D),
+    ((256, 12, 12), (1, 1, 1), G.WENO3JS, G.CD2, G.LF, G.NAVIERSTOKES3D),
+    ((12, 160, 12), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 12, 100), (1, 1, 2), G.UPWIND1ST, G.CD8, G.LF, G.NAVIERSTOKES3D),
 ]
 
 

@@ -28,7 +28,7 @@ def _data(rng, n1, n2, n3, bf, kind):
             np.ascontiguousarray(line[:, :, bf + n1:]))
 
 
-@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.WENO7JS, O.CRWENO5])
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.WENO7JS, O.CRWENO5, O.WENO3JS, O.UPWIND1ST, O.HCCS78])
 @pytest.mark.parametrize("upwind", [1, -1])
 @pytest.mark.parametrize("kind", ["smooth", "random", "step"])
 def test_reconstruct_bitexact(scheme, upwind, kind):
@@ -42,7 +42,7 @@ def test_reconstruct_bitexact(scheme, upwind, kind):
         assert np.array_equal(a, b), (scheme, upwind, n1, np.abs(a - b).max())
 
 
-@pytest.mark.parametrize("scheme,ts", [(O.CD6, 3), (O.CD6, 4), (O.CD8, 4)])
+@pytest.mark.parametrize("scheme,ts", [(O.CD6, 3), (O.CD6, 4), (O.CD8, 4), (O.CD2, 1), (O.CD2, 2), (O.CD2, 4)])
 def test_diff_bitexact(scheme, ts):
     rng = np.random.default_rng(7 + ts + 10 * scheme)
     for (n1, n2, n3) in SHAPES:
