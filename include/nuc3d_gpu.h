@@ -36,7 +36,7 @@ enum nuc_scheme { NUC_WENO5JS = 0, NUC_WENO5Z = 1, NUC_WENO7JS = 2, NUC_CRWENO5 
 /* viscous first-derivative schemes (src/fieldOperator.cpp:298-345) */
 enum nuc_vis_scheme { NUC_CD6 = 0, NUC_CD8 = 1, NUC_CD2 = 2 };
 /* flux-vector splitting (src/physicsModel.cpp:16-20) */
-enum nuc_riemann { NUC_LF = 0, NUC_AUSM = 1 };
+enum nuc_riemann { NUC_LF = 0, NUC_AUSM = 1, NUC_AUSMP = 2 };
 /* PDE model (src/block.cpp:146-157) */
 enum nuc_model { NUC_EULER3D = 0, NUC_NAVIERSTOKES3D = 1 };
 /* viscosity law (src/physicsModel.cpp:45-49) */

@@ -341,7 +341,7 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A) { gradq_body<T>
 // Compute mapping: wave w owns line w (both signs, see nuc_lines.h).
 // ---------------------------------------------------------------------------
 template <int RIE, typename T>
-__device__ __forceinline__ void split_params(const T q[5], int d, T gamma, T sp[5])
+__device__ __forceinline__ void split_params(const T q[5], int d, T gamma, T sp[5], bool ausmp = false)
 {
     const T gm1 = gamma - T(1);
     T rho = q[0], ir = T(1) / rho;
@@ -358,8 +358,15 @@ __device__ __forceinline__ void split_params(const T q[5], int d, T gamma, T sp[
         if (fabs(mach) < T(1)) {
             mp = T(0.25) * sq(mach + T(1)) + T(0.125) * sq(mach * mach - T(1));
             mn = T(-0.25) * sq(mach - T(1)) - T(0.125) * sq(mach * mach - T(1));
-            pp = p * (mach + T(1)) / T(2);
-            pn = -p * (mach - T(1)) / T(2);
+            if (ausmp) {
+                // AUSM+ pressure splitting, src/physicsModel.cpp:1257-1277
+                const T a = T(0.1875) * mach * sq(mach * mach - T(1));
+                pp = p * (T(0.25) * sq(mach + T(1)) * (T(2) - mach) + a);
+                pn = p * (T(0.25) * sq(mach - T(1)) * (T(2) + mach) - a);
+            } else {
+                pp = p * (mach + T(1)) / T(2);
+                pn = -p * (mach - T(1)) / T(2);
+            }
         } else {
             mp = T(0.5) * (mach + fabs(mach));
             mn = T(0.5) * (mach - fabs(mach));
@@ -460,7 +467,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 #pragma unroll
         for (int v = 0; v < 5; v++) q[v] = ok ? T(A.q[v][id]) : (v == 0 ? T(1) : T(0));
         T tmp[5];
-        split_params<RIE, T>(q, d, T(A.gamma), tmp);
+        split_params<RIE, T>(q, d, T(A.gamma), tmp, A.riemann == RIE_AUSMP);
 #pragma unroll
         for (int j = 0; j < NSP; j++) sp[k][j] = tmp[j];
         int cl = c % A.nloc;
@@ -1142,12 +1149,15 @@ static int validate(const nuc_config *c, std::string &err)
         if (np[d] < 1 || n[d] % np[d] != 0) { err = "grid size must divide evenly by the rank grid (src/block.cpp:56-58)"; return NUC_EINVAL; }
         if (c->scheme[d] < 0 || c->scheme[d] > NUC_HCCS78) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
         if (c->vis_scheme[d] < 0 || c->vis_scheme[d] > NUC_CD2) { err = "unknown viscous scheme id"; return NUC_EINVAL; }
-        int ts = vis_tilesize(c->scheme[d], c->vis_scheme[d]);
-        if (n[d] / np[d] < 2 * ts + 2) { err = "rank-local line too short for the scheme's halo"; return NUC_EINVAL; }
+        int ts = scheme_halo(c->scheme[d]), tsv = vis_tilesize(c->scheme[d], c->vis_scheme[d]);
+        if (n[d] / np[d] < 2 * ts + 2 || n[d] / np[d] < 2 * tsv) {
+            err = "rank-local line too short for the scheme's halo";
+            return NUC_EINVAL;
+        }
         if (n[d] > 512) { err = "line length above 512 is not supported"; return NUC_EINVAL; }
     }
     if ((long)c->npx * c->npy * c->npz > MAXRANK) { err = "at most 64 ranks"; return NUC_EINVAL; }
-    if (c->riemann < 0 || c->riemann > 1 || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
+    if (c->riemann < 0 || c->riemann > NUC_AUSMP || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
     if (c->forcing < NUC_FORCING_NONE || c->forcing > NUC_FORCING_TG) { err = "unknown forcing id"; return NUC_EINVAL; }
     if (c->centrifugal && !(c->well_rad > 0)) { err = "centrifugal well radius must be positive"; return NUC_EINVAL; }
     if (c->precision != NUC_FP64 && c->precision != NUC_MIXED) { err = "unknown precision id"; return NUC_EINVAL; }

@@ -11,7 +11,7 @@ namespace nucdev {
 
 enum { WENO5JS = 0, WENO5Z = 1, WENO7JS = 2, CRWENO5 = 3, HCCS = 4, WENO3JS = 5, UPWIND1ST = 6, HCCS78 = 7 };
 enum { VIS_NONE = 0, VIS_CD6 = 1, VIS_CD8 = 2, VIS_CD2 = 3 };
-enum { RIE_LF = 0, RIE_AUSM = 1 };
+enum { RIE_LF = 0, RIE_AUSM = 1, RIE_AUSMP = 2 };   // AUSM+ runs the AUSM kernels with its pressure splitting
 
 template <typename T> __device__ __forceinline__ T sq(T x) { return x * x; }
 

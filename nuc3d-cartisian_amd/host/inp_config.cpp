@@ -99,6 +99,7 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
         f >> k >> neqs >> k >> eos >> k >> riemann >> k >> model >> k >> visc >> k >> forcing;
         if (riemann == "LF") c.riemann = NUC_LF;
         else if (riemann == "AUSM") c.riemann = NUC_AUSM;
+        else if (riemann == "AUSMp") c.riemann = NUC_AUSMP;
         else { err = "PhysModel.in: Riemann solver '" + riemann + "' is not available"; return NUC_EINVAL; }
         if (model == "NavierStokes3d") c.model = NUC_NAVIERSTOKES3D;
         else if (model == "Euler3d") c.model = NUC_EULER3D;

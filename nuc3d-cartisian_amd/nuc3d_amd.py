@@ -16,7 +16,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78 = range(8)
 CD6, CD8, CD2 = 0, 1, 2
-LF, AUSM = 0, 1
+LF, AUSM, AUSMP = 0, 1, 2
 EULER3D, NAVIERSTOKES3D = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
 FORCING_NONE, FORCING_CONST, FORCING_ABC, FORCING_TG = 0, 1, 2, 3

@@ -465,9 +465,26 @@ static double presR(double mach, double p)
     return 0.5 * p * (mach - fabs(mach)) / mach;
 }
 
+/* AUSM+ pressure splittings, src/physicsModel.cpp:1257-1277 (getMachLp/Rp at
+ * :1231-1255 equal getMachL/R) */
+static double presLp(double mach, double p)
+{
+    if (fabs(mach) < 1.0)
+        return p * (0.25 * pow((mach + 1.0), 2.0) * (2.0 - mach) + 0.1875 * mach * pow((mach * mach - 1.0), 2.0));
+    return 0.50 * p * (mach + fabs(mach)) / mach;
+}
+static double presRp(double mach, double p)
+{
+    if (fabs(mach) < 1.0)
+        return p * (0.25 * pow(mach - 1.0, 2.0) * (2.0 + mach) - 0.1875 * mach * pow((mach * mach - 1.0), 2.0));
+    return 0.5 * p * (mach - fabs(mach)) / mach;
+}
+
 /* Flux-vector splitting at one point in direction dir (0,1,2):
  * LF: src/physicsModel.cpp:540-565 (x), :587-619 (y), :641-674 (z);
- * AUSM: src/physicsModel.cpp:790-820 (x), :843-868 (y), :896-921 (z). */
+ * AUSM: src/physicsModel.cpp:790-820 (x), :843-868 (y), :896-921 (z);
+ * AUSM+: the same fluxes with the AUSM+ pressure splitting,
+ * src/physicsModel.cpp:1040-1063 (x), :1093-1118 (y), :1148-1173 (z). */
 void orc_split(int riemann, int dir, const double *q, const double *w, double *fp, double *fm)
 {
     double ud = w[1 + dir], p0 = w[4], alpha0 = w[7];
@@ -487,7 +504,8 @@ void orc_split(int riemann, int dir, const double *q, const double *w, double *f
     } else {
         double mach = ud / alpha0;
         double machp = machL(mach), machn = machR(mach);
-        double p_p = presL(mach, p0), p_n = presR(mach, p0);
+        double p_p = (riemann == ORC_AUSMP) ? presLp(mach, p0) : presL(mach, p0);
+        double p_n = (riemann == ORC_AUSMP) ? presRp(mach, p0) : presR(mach, p0);
         for (int v = 0; v < 4; v++) {
             fp[v] = machp * alpha0 * q[v];
             fm[v] = machn * alpha0 * q[v];

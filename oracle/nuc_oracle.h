@@ -22,7 +22,7 @@ extern "C" {
 enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_WENO7JS = 2, ORC_CRWENO5 = 3, ORC_HCCS = 4, ORC_WENO3JS = 5,
        ORC_UPWIND1ST = 6, ORC_HCCS78 = 7 };
 enum { ORC_CD6 = 0, ORC_CD8 = 1, ORC_CD2 = 2 };
-enum { ORC_LF = 0, ORC_AUSM = 1 };
+enum { ORC_LF = 0, ORC_AUSM = 1, ORC_AUSMP = 2 };
 enum { ORC_EULER = 0, ORC_NS = 1 };
 enum { ORC_SUTHERLAND = 0, ORC_CONSTANT = 1 };
 enum { ORC_FORCE_NONE = 0, ORC_FORCE_CONST = 1, ORC_FORCE_ABC = 2, ORC_FORCE_TG = 3 };

@@ -14,7 +14,7 @@ _dp = ctypes.POINTER(ctypes.c_double)
 
 WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78 = range(8)
 CD6, CD8, CD2 = 0, 1, 2
-LF, AUSM = 0, 1
+LF, AUSM, AUSMP = 0, 1, 2
 EULER, NS = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
 FORCE_NONE, FORCE_CONST, FORCE_ABC, FORCE_TG = 0, 1, 2, 3

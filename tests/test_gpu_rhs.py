@@ -88,6 +88,10 @@ CASES = [
     ((24, 20, 18), (1, 1, 1), G.WENO5JS, G.CD2, G.LF, G.NAVIERSTOKES3D),
     ((20, 20, 20), (2, 2, 1), G.WENO3JS, G.CD2, G.AUSM, G.NAVIERSTOKES3D),
     ((32, 24, 16), (1, 1, 1), G.UPWIND1ST, G.CD6, G.LF, G.EULER3D),
+    # AUSM+ splitting (src/physicsModel.cpp:932-1179)
+    ((16, 16, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSMP, G.NAVIERSTOKES3D),
+    ((24, 20, 18), (2, 2, 2), G.CRWENO5, G.CD8, G.AUSMP, G.NAVIERSTOKES3D),
+    ((20, 20, 20), (1, 1, 1), G.WENO3JS, G.CD2, G.AUSMP, G.EULER3D),
 ]
 
 

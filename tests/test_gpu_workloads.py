@@ -184,9 +184,9 @@ def test_forced_turbulence_512_full_size():
 
 # ---- Shu-Osher shock / entropy waves (WENO5-Z, AUSM, cd6) ------------------
 
-def _shu_kw(n, npr=(1, 1, 1), precision=G.FP64):
+def _shu_kw(n, npr=(1, 1, 1), precision=G.FP64, riemann=G.AUSM):
     return dict(n=n, npr=npr, L=(10.0, 10.0 * n[1] / n[0], 10.0 * n[2] / n[0]), scheme=G.WENO5Z, vis=G.CD6,
-                riemann=G.AUSM, model=G.NAVIERSTOKES3D, viscosity=G.SUTHERLAND, mach=1.0 / np.sqrt(1.4),
+                riemann=riemann, model=G.NAVIERSTOKES3D, viscosity=G.SUTHERLAND, mach=1.0 / np.sqrt(1.4),
                 reynolds=2000.0, prandtl=0.72, cfl=0.4, precision=precision)
 
 
@@ -194,6 +194,14 @@ def _shu_kw(n, npr=(1, 1, 1), precision=G.FP64):
 def test_shu_osher_small_matches_oracle(n, npr):
     kw = _shu_kw(n, npr)
     q = G.init_shu_osher(G.make_config(**kw), pert=0.05, seed=20261602)
+    _vs_oracle(kw, q, mom_scale=True)
+
+
+def test_shu_osher_ausmp_matches_oracle():
+    """AUSM+ (src/physicsModel.cpp:932-1179) through the Mach-3 shock: both the
+    subsonic and the supersonic branches of its splittings."""
+    kw = _shu_kw((64, 12, 12), riemann=G.AUSMP)
+    q = G.init_shu_osher(G.make_config(**kw), pert=0.05, seed=20261603)
     _vs_oracle(kw, q, mom_scale=True)
 
 

@@ -77,6 +77,16 @@ def test_inp_forced_hccs():
     assert (c.nx, c.npx) == (512, 4)
 
 
+def test_inp_remaining_operators():
+    """hccs78, weno3js, upwind1st, cd2 and AUSMp as Method.in / PhysModel.in
+    name them (src/fieldOperator.cpp:128-345, src/physicsModel.cpp:16-20)."""
+    c, s, e = G.config_from_inp(os.path.join(GOLD, "inp_operators"))
+    assert list(c.scheme) == [G.HCCS78, G.WENO3JS, G.UPWIND1ST]
+    assert list(c.vis_scheme) == [G.CD2, G.CD8, G.CD2]
+    assert c.riemann == G.AUSMP
+    assert (c.nx, c.ny, c.nz, c.npy, c.reynolds, c.mach) == (48, 32, 40, 2, 900.0, 0.4)
+
+
 def test_inp_missing_directory_is_an_error():
     with pytest.raises(G.NucError):
         G.config_from_inp(os.path.join(GOLD, "does_not_exist"))

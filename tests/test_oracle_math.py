@@ -76,6 +76,65 @@ def test_ausm_split_consistency(d):
                 assert np.allclose(fm if vel > 0 else fp, 0.0)
 
 
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_ausmp_split_consistency_and_difference_from_ausm(d):
+    """AUSM+ (src/physicsModel.cpp:932-1179): f+ + f- = f(q) as for AUSM; the
+    mass and energy parts equal AUSM's (same Mach splitting), the pressure part
+    differs for 0 < |M| < 1 only."""
+    rng = np.random.default_rng(20 + d)
+    for scale in (0.3, 3.0):
+        for _ in range(100):
+            q, (rho, u, v, w, p) = _state(rng)
+            q = q.copy()
+            q[1:4] *= scale
+            q[4] = p / 0.4 + 0.5 * (q[1] ** 2 + q[2] ** 2 + q[3] ** 2) / rho
+            W = O.con2prim(q, 1.4, 0.3)
+            fp, fm = O.split(O.AUSMP, d, q, W)
+            vel = W[1 + d]
+            F = vel * q
+            F[1 + d] += W[4]
+            F[4] = vel * (q[4] + W[4])
+            np.testing.assert_allclose(fp + fm, F, rtol=1e-11, atol=1e-11)
+            ap, am = O.split(O.AUSM, d, q, W)
+            other = [v_ for v_ in range(5) if v_ != 1 + d]
+            assert np.array_equal(fp[other], ap[other]) and np.array_equal(fm[other], am[other])
+            M = vel / W[7]
+            if abs(M) >= 1:
+                assert fp[1 + d] == ap[1 + d] and fm[1 + d] == am[1 + d]
+            elif abs(M) > 1e-3:
+                assert fp[1 + d] != ap[1 + d]
+
+
+def test_ausmp_pressure_splitting_shape():
+    """The split pressure P+(M) p of AUSM+: P+(0) = 1/2, P+(M) + P+(-M) = 1,
+    C1 continuous at |M| = 1 (AUSM's linear splitting is not), and
+    P+'(0) = 3/4 + alpha with alpha = 3/16 (Liou's AUSM+ polynomial)."""
+    p, c = 2.0, 1.0
+    gamma = 1.4
+    rho = gamma * p / c ** 2          # sound speed 1: M = u
+
+    def pplus(M, rie=O.AUSMP):
+        q = np.array([rho, rho * M, 0.0, 0.0, p / (gamma - 1) + 0.5 * rho * M * M])
+        W = O.con2prim(q, gamma, 0.3)
+        fp, _ = O.split(rie, 0, q, W)
+        mp = fp[0] / (rho * W[7])      # M+ c rho / (rho c)
+        return (fp[1] - mp * W[7] * q[1]) / W[4]
+
+    assert abs(pplus(0.0) - 0.5) <= 1e-15
+    for M in (0.1, 0.37, 0.8, 0.99):
+        assert abs(pplus(M) + pplus(-M) - 1.0) <= 1e-14
+    h = 1e-6
+    for Mj in (1.0, -1.0):
+        left = (pplus(Mj - h) - pplus(Mj - 2 * h)) / h
+        right = (pplus(Mj + 2 * h) - pplus(Mj + h)) / h
+        assert abs(left) < 1e-4 and abs(right) < 1e-4                      # flat on both sides
+        inside = -Mj * h                                                   # step into |M| < 1
+        ausm = (pplus(Mj + inside, O.AUSM) - pplus(Mj + 2 * inside, O.AUSM)) / h
+        assert abs(abs(ausm) - 0.5) < 1e-4                                 # AUSM: slope 1/2, a kink
+    d0 = (pplus(h) - pplus(-h)) / (2 * h)
+    assert abs(d0 - (0.75 + 3.0 / 16.0)) <= 1e-8
+
+
 def test_viscosity_laws():
     m, c = O.viscosity(O.SUTHERLAND, 1.0, 1600.0, 0.1, 0.72, 1.4, 110.4, 288.0)
     assert np.isclose(m, 1 / 1600.0, rtol=1e-15)                    # mu(T=1) = 1/Re
