@@ -32,7 +32,7 @@ enum nuc_status {
 
 /* inviscid schemes, names as accepted in inp/Method.in (src/fieldOperator.cpp:128-220) */
 enum nuc_scheme { NUC_WENO5JS = 0, NUC_WENO5Z = 1, NUC_WENO7JS = 2, NUC_CRWENO5 = 3, NUC_HCCS = 4,
-                  NUC_WENO3JS = 5, NUC_UPWIND1ST = 6, NUC_HCCS78 = 7 };
+                  NUC_WENO3JS = 5, NUC_UPWIND1ST = 6, NUC_HCCS78 = 7, NUC_HCCS77 = 8 };
 /* viscous first-derivative schemes (src/fieldOperator.cpp:298-345) */
 enum nuc_vis_scheme { NUC_CD6 = 0, NUC_CD8 = 1, NUC_CD2 = 2 };
 /* flux-vector splitting (src/physicsModel.cpp:16-20) */

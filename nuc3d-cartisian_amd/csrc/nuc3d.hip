@@ -875,7 +875,7 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
     // The 7-point schemes keep FP64 stencils in the mixed path too: their
     // smoothness indicators cancel terms up to 17246 f^2, so FP32 inputs alone
     // move the state by ~1e-4 in a few steps.
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78);
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78 && SCH != HCCS77);
     const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
     if constexpr (fp32_ok) {
@@ -918,6 +918,7 @@ void launch_sweep(int scheme, int R, const SweepArgs &A, hipStream_t s)
     case WENO3JS: launch_sweep_s<WENO3JS>(R, A, s); break;
     case UPWIND1ST: launch_sweep_s<UPWIND1ST>(R, A, s); break;
     case HCCS78: launch_sweep_s<HCCS78>(R, A, s); break;
+    case HCCS77: launch_sweep_s<HCCS77>(R, A, s); break;
     default: launch_sweep_s<HCCS>(R, A, s); break;
     }
 }
@@ -980,7 +981,7 @@ struct PackLaunch {
 template <int SCH, int RIE, int R>
 void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
 {
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78);
+    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78 && SCH != HCCS77);
     const dim3 grid((unsigned)P.nblocks), block(64 * SWEEP_LPB);
     if constexpr (fp32_ok) {
         if (P.mixed) {
@@ -1017,6 +1018,7 @@ void launch_sweep_pack(int scheme, int riemann, int R, const PackLaunch &P, hipS
     case WENO3JS: launch_sweep_pack_s<WENO3JS>(riemann, R, P, s); break;
     case UPWIND1ST: launch_sweep_pack_s<UPWIND1ST>(riemann, R, P, s); break;
     case HCCS78: launch_sweep_pack_s<HCCS78>(riemann, R, P, s); break;
+    case HCCS77: launch_sweep_pack_s<HCCS77>(riemann, R, P, s); break;
     default: launch_sweep_pack_s<HCCS>(riemann, R, P, s); break;
     }
 }
@@ -1125,7 +1127,7 @@ struct ProfScope {
 // bufferSize per scheme (src/fieldOperator.cpp:128-181)
 static int scheme_halo(int scheme)
 {
-    if (scheme == NUC_WENO7JS || scheme == NUC_HCCS || scheme == NUC_HCCS78) return 4;
+    if (scheme == NUC_WENO7JS || scheme == NUC_HCCS || scheme == NUC_HCCS78 || scheme == NUC_HCCS77) return 4;
     if (scheme == NUC_WENO3JS) return 2;
     if (scheme == NUC_UPWIND1ST) return 1;
     return 3;
@@ -1147,7 +1149,7 @@ static int validate(const nuc_config *c, std::string &err)
     int n[3] = {c->nx, c->ny, c->nz}, np[3] = {c->npx, c->npy, c->npz};
     for (int d = 0; d < 3; d++) {
         if (np[d] < 1 || n[d] % np[d] != 0) { err = "grid size must divide evenly by the rank grid (src/block.cpp:56-58)"; return NUC_EINVAL; }
-        if (c->scheme[d] < 0 || c->scheme[d] > NUC_HCCS78) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
+        if (c->scheme[d] < 0 || c->scheme[d] > NUC_HCCS77) { err = "unknown inviscid scheme id"; return NUC_EINVAL; }
         if (c->vis_scheme[d] < 0 || c->vis_scheme[d] > NUC_CD2) { err = "unknown viscous scheme id"; return NUC_EINVAL; }
         int ts = scheme_halo(c->scheme[d]), tsv = vis_tilesize(c->scheme[d], c->vis_scheme[d]);
         if (n[d] / np[d] < 2 * ts + 2 || n[d] / np[d] < 2 * tsv) {
@@ -1804,7 +1806,7 @@ extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const do
 {
     if (!in || !out || n1 < 2 * tilesize + 1 || n2 < 1 || n3 < 1 || (upwind != 1 && upwind != -1)) return NUC_EINVAL;
     if (((zones & 2) && !gL) || ((zones & 4) && !gR) || ((zones & 6) && bf < tilesize)) return NUC_EINVAL;
-    if (scheme < 0 || scheme > NUC_HCCS78 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
+    if (scheme < 0 || scheme > NUC_HCCS77 || tilesize < 1 || tilesize > 4 || bf > 16) return NUC_EINVAL;
     int cpt = rows_for(n1 + 1);
     if (cpt < 0) return NUC_EINVAL;
     long nl = (long)n2 * n3;
@@ -1829,6 +1831,7 @@ extern "C" int nuc_op_reconstruct(int scheme, int upwind, int tilesize, const do
     case WENO3JS: launch_op_sch<WENO3JS>(cpt, A, 0); break;
     case UPWIND1ST: launch_op_sch<UPWIND1ST>(cpt, A, 0); break;
     case HCCS78: launch_op_sch<HCCS78>(cpt, A, 0); break;
+    case HCCS77: launch_op_sch<HCCS77>(cpt, A, 0); break;
     default: launch_op_sch<HCCS>(cpt, A, 0); break;
     }
     if (hipGetLastError() != hipSuccess) return NUC_EHIP;

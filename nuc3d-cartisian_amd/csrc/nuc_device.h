@@ -9,7 +9,7 @@
 
 namespace nucdev {
 
-enum { WENO5JS = 0, WENO5Z = 1, WENO7JS = 2, CRWENO5 = 3, HCCS = 4, WENO3JS = 5, UPWIND1ST = 6, HCCS78 = 7 };
+enum { WENO5JS = 0, WENO5Z = 1, WENO7JS = 2, CRWENO5 = 3, HCCS = 4, WENO3JS = 5, UPWIND1ST = 6, HCCS78 = 7, HCCS77 = 8 };
 enum { VIS_NONE = 0, VIS_CD6 = 1, VIS_CD8 = 2, VIS_CD2 = 3 };
 enum { RIE_LF = 0, RIE_AUSM = 1, RIE_AUSMP = 2 };   // AUSM+ runs the AUSM kernels with its pressure splitting
 
@@ -75,6 +75,26 @@ __device__ __forceinline__ TW hccs78_inner(const T *f)
              + TW(533) * (TW(f[4]) + TW(f[3]))) / TW(840);
     TW theta = TW(1) / (TW(1) + sq(asum - TW(1)));
     return (TW(1) - theta) * w + theta * q8;
+}
+
+// hccs77 interior row, src/hccs77.cpp:86-163: WENO7-Z (eps 1e-20) and the
+// 7th-order upwind compact row blended through theta = 1/(1 + (asum-1)^2);
+// theta > 0.5 keeps the compact row alone.  f[0..6]; TW: weights and the row.
+template <typename T, typename TW>
+__device__ __forceinline__ void hccs77_row(const T *f, TW &lo, TW &di, TW &hi, TW &rhs)
+{
+    TW asum;
+    TW w = weno7z<T, TW>(f, TW(1.0e-20), asum);
+    TW q7 = (-TW(f[1]) + TW(19) * TW(f[2]) + TW(239) * TW(f[3]) + TW(159) * TW(f[4]) + TW(4) * TW(f[5])) / TW(420);
+    TW theta = TW(1) / (TW(1) + sq(asum - TW(1)));
+    if (theta > TW(0.5)) {
+        lo = TW(2.0 / 7.0); di = TW(4.0 / 7.0); hi = TW(1.0 / 7.0); rhs = q7;
+    } else {
+        lo = TW(2.0 / 7.0) * theta;
+        di = TW(1) + (TW(4.0 / 7.0) - TW(1)) * theta;
+        hi = TW(1.0 / 7.0) * theta;
+        rhs = w * (TW(1) - theta) + q7 * theta;
+    }
 }
 
 // WENO3-JS, src/weno3js.cpp:80-93 (eps 1e-6): f[0..2] upwind

@@ -58,7 +58,7 @@ __device__ __forceinline__ void w5_weights(TW b0, TW b1, TW b2, TW c0, TW c1, TW
 // upwind reach of a scheme's stencil, and the points it reads beyond the
 // symmetric window (hccs78's 8-point central part)
 template <int SCH> struct Radius {
-    static constexpr int value = (SCH == WENO7JS || SCH == HCCS || SCH == HCCS78) ? 3
+    static constexpr int value = (SCH == WENO7JS || SCH == HCCS || SCH == HCCS78 || SCH == HCCS77) ? 3
                                : (SCH == WENO3JS || SCH == UPWIND1ST) ? 1 : 2;
 };
 template <int SCH> struct Extra { static constexpr int value = (SCH == HCCS78) ? 1 : 0; };
@@ -349,6 +349,26 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TS c = (valid && !expl) ? TS(hi) : TS(0);
             TS d = valid ? TS(expl ? wexp : rhs) : TS(0);
             ch.row(r, a, b, c, d);
+        }
+        ch.solve(h);
+#pragma unroll
+        for (int r = 0; r < R; r++) out[r] = ch.Df[r];
+    } else if constexpr (SCH == HCCS77) {
+        // compact rows on ts+1..nloc-ts-1, WENO7-Z rows at ts and nloc-ts and
+        // on the edges (src/hccs77.cpp:86-163, :349-455)
+        Chunk<R, TS> ch;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const T *f = win + r;
+            int m = m0r + mstep * r;
+            bool valid = full || m < z.nrows;
+            bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
+            TW a = TW(0), b = TW(1), c = TW(0), d = TW(0);
+            if (valid) {
+                if (expl) d = weno7z<T, TW>(f, TW(1.0e-20));
+                else hccs77_row<T, TW>(f, a, b, c, d);
+            }
+            ch.row(r, TS(a), TS(b), TS(c), TS(d));
         }
         ch.solve(h);
 #pragma unroll

@@ -36,6 +36,7 @@ int scheme_from_name(const std::string &s, int *halo)
     if (s == "weno5z") { *halo = 3; return NUC_WENO5Z; }
     if (s == "crweno5") { *halo = 3; return NUC_CRWENO5; }
     if (s == "hccs78") { *halo = 4; return NUC_HCCS78; }
+    if (s == "hccs77") { *halo = 4; return NUC_HCCS77; }
     if (s == "weno3js") { *halo = 2; return NUC_WENO3JS; }
     if (s == "upwind1st") { *halo = 1; return NUC_UPWIND1ST; }
     *halo = 3;

@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libnuc3d_amd.so")
 _dp = ctypes.POINTER(ctypes.c_double)
 
-WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78 = range(8)
+WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78, HCCS77 = range(9)
 CD6, CD8, CD2 = 0, 1, 2
 LF, AUSM, AUSMP = 0, 1, 2
 EULER3D, NAVIERSTOKES3D = 0, 1

@@ -60,7 +60,7 @@ static volatile double P_MEMBER = 2.0;
 /* bufferSize per scheme, src/fieldOperator.cpp:128-181 */
 int orc_scheme_halo(int scheme)
 {
-    if (scheme == ORC_WENO7JS || scheme == ORC_HCCS || scheme == ORC_HCCS78) return 4;
+    if (scheme == ORC_WENO7JS || scheme == ORC_HCCS || scheme == ORC_HCCS78 || scheme == ORC_HCCS77) return 4;
     if (scheme == ORC_WENO3JS) return 2;
     if (scheme == ORC_UPWIND1ST) return 1;
     return 3;
@@ -177,6 +177,26 @@ static double hccs78_pt(const double *f)
     return (1.0 - theta) * w + theta * q8;
 }
 
+/* hccs77 interior row, src/hccs77.cpp:86-163 (positive; the negative sweep
+ * :227-305 swaps the roles of a and c): WENO7-Z (eps 1e-20) and the 7th-order
+ * upwind compact row blended through theta = 1/(1 + (alphaSum-1)^2); theta >
+ * 0.5 takes the compact row alone.  f[3+z], z=-3..3. */
+static void hccs77_row(const double *f, double *lo, double *di, double *hi, double *rhs)
+{
+    double alphaSum;
+    double w = weno7z_core(f, EPS_7, 1, &alphaSum);
+    double q7 = (-f[1] + 19.0 * f[2] + 239.0 * f[3] + 159.0 * f[4] + 4.0 * f[5]) / 420.0;
+    double theta = 1.0 / (1.0 + POWP(alphaSum - 1.0));
+    if (theta > 0.5) {
+        *lo = 2.0 / 7.0; *di = 4.0 / 7.0; *hi = 1.0 / 7.0; *rhs = q7;
+    } else {
+        *lo = CUC[0] * theta;
+        *di = 1.0 + (CUC[1] - 1.0) * theta;
+        *hi = CUC[2] * theta;
+        *rhs = w * (1.0 - theta) + q7 * theta;
+    }
+}
+
 /* WENO3-JS, src/weno3js.cpp:80-93: f[1+z], z=-1..1 */
 static double weno3js_pt(const double *f)
 {
@@ -269,6 +289,11 @@ static double explicit_pt(int scheme, int upwind, const double *f, int m, int in
         /* src/upwind1st.cpp:56-74 (+: cell m-1), :95-113 (-: cell m) */
         gather(f, m, upwind, 1, st);
         return st[1];
+    case ORC_HCCS77:
+        /* edges (src/hccs77.cpp:349-455) and the explicit first/last compact
+         * rows (:150-157): WENO7-Z with eps 1e-20 */
+        gather(f, m, upwind, 3, st);
+        return weno7z_pt(st, EPS_7, 1);
     case ORC_HCCS78:
         /* inner: blend (src/hccs78.cpp:47-117); edges: WENO7-Z (:207-304) */
         if (!inner) {
@@ -311,7 +336,7 @@ void orc_line_reconstruct(int scheme, int upwind, const double *f, int n, int ts
      * out[0..ts-1] and out[n-ts+1..n] */
     for (int m = 0; m < ts; m++) out[m] = explicit_pt(scheme, upwind, f, m, 0);
     for (int m = n - ts + 1; m <= n; m++) out[m] = explicit_pt(scheme, upwind, f, m, 0);
-    if (scheme != ORC_CRWENO5 && scheme != ORC_HCCS) {
+    if (scheme != ORC_CRWENO5 && scheme != ORC_HCCS && scheme != ORC_HCCS77) {
         /* inner sweep: i = ts-1 .. n-ts-1 writes out[i+1] (src/weno5js.cpp:56-111) */
         for (int m = ts; m <= n - ts; m++) out[m] = explicit_pt(scheme, upwind, f, m, 1);
         return;
@@ -327,10 +352,15 @@ void orc_line_reconstruct(int scheme, int upwind, const double *f, int n, int ts
             a[r] = 0.0; b[r] = 1.0; c[r] = 0.0;
             d[r] = explicit_pt(scheme, upwind, f, m, 1);
         } else {
-            double st[5], lo, di, hi, rhs;
-            gather(f, m, upwind, 2, st);
-            if (scheme == ORC_CRWENO5) crweno5_row(st, &lo, &di, &hi, &rhs);
-            else hccs_row(st, &lo, &di, &hi, &rhs);
+            double st[7], lo, di, hi, rhs;
+            if (scheme == ORC_HCCS77) {
+                gather(f, m, upwind, 3, st);
+                hccs77_row(st, &lo, &di, &hi, &rhs);
+            } else {
+                gather(f, m, upwind, 2, st);
+                if (scheme == ORC_CRWENO5) crweno5_row(st, &lo, &di, &hi, &rhs);
+                else hccs_row(st, &lo, &di, &hi, &rhs);
+            }
             /* positive: lo multiplies interface m-1 (sub-diagonal); negative:
              * the roles of a and c swap (src/crweno5.cpp:329-332) */
             if (upwind > 0) { a[r] = lo; b[r] = di; c[r] = hi; }

@@ -20,7 +20,7 @@ extern "C" {
 #endif
 
 enum { ORC_WENO5JS = 0, ORC_WENO5Z = 1, ORC_WENO7JS = 2, ORC_CRWENO5 = 3, ORC_HCCS = 4, ORC_WENO3JS = 5,
-       ORC_UPWIND1ST = 6, ORC_HCCS78 = 7 };
+       ORC_UPWIND1ST = 6, ORC_HCCS78 = 7, ORC_HCCS77 = 8 };
 enum { ORC_CD6 = 0, ORC_CD8 = 1, ORC_CD2 = 2 };
 enum { ORC_LF = 0, ORC_AUSM = 1, ORC_AUSMP = 2 };
 enum { ORC_EULER = 0, ORC_NS = 1 };

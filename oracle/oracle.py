@@ -12,14 +12,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 _dp = ctypes.POINTER(ctypes.c_double)
 
-WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78 = range(8)
+WENO5JS, WENO5Z, WENO7JS, CRWENO5, HCCS, WENO3JS, UPWIND1ST, HCCS78, HCCS77 = range(9)
 CD6, CD8, CD2 = 0, 1, 2
 LF, AUSM, AUSMP = 0, 1, 2
 EULER, NS = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
 FORCE_NONE, FORCE_CONST, FORCE_ABC, FORCE_TG = 0, 1, 2, 3
 SCHEME_NAMES = {"weno5js": WENO5JS, "weno5z": WENO5Z, "weno7js": WENO7JS, "crweno5": CRWENO5, "hccs": HCCS,
-                "weno3js": WENO3JS, "upwind1st": UPWIND1ST, "hccs78": HCCS78}
+                "weno3js": WENO3JS, "upwind1st": UPWIND1ST, "hccs78": HCCS78, "hccs77": HCCS77}
 
 
 class OrcConfig(ctypes.Structure):

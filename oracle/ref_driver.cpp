@@ -1,7 +1,7 @@
 // ref_driver.cpp -- extern "C" entry points around the reference's own scheme
 // classes, compiled together with the unmodified reference sources
 // (src/weno5js.cpp, src/weno5z.cpp, src/weno7js.cpp, src/crweno5.cpp,
-// src/weno3js.cpp, src/upwind1st.cpp, src/hccs78.cpp,
+// src/weno3js.cpp, src/upwind1st.cpp, src/hccs78.cpp, src/hccs77.cpp,
 // src/centraldifference2nd.cpp, src/centraldifference6th.cpp,
 // src/centraldifference8th.cpp, src/TVD-RK3.cpp)
 // into oracle/_ref/libnucref.so by oracle/Makefile.
@@ -18,6 +18,7 @@
 #include "weno3js.hpp"
 #include "upwind1st.hpp"
 #include "hccs78.hpp"
+#include "hccs77.hpp"
 #include "centraldifference2nd.hpp"
 #include "centraldifference6th.h"
 #include "centraldifference8th.hpp"
@@ -36,7 +37,7 @@ static Field make_field(const double *src, int n1, int n2, int n3)
 extern "C" {
 
 // scheme ids match oracle/nuc_oracle.h: 0 weno5js, 1 weno5z, 2 weno7js, 3 crweno5,
-// 5 weno3js, 6 upwind1st, 7 hccs78
+// 5 weno3js, 6 upwind1st, 7 hccs78, 8 hccs77
 int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int n1, int n2, int n3,
                     const double *gL, const double *gR, int bf, double *out)
 {
@@ -52,6 +53,7 @@ int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int 
     nuc3d::weno3js s5;
     nuc3d::upwind1st s6;
     nuc3d::hccs78 s7;
+    nuc3d::hccs77 s8;
     switch (scheme) {
     case 0: s0.interpolationInner(fin, upwind, fo, tilesize); s0.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
             s0.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
@@ -67,6 +69,8 @@ int ref_reconstruct(int scheme, int upwind, int tilesize, const double *in, int 
             s6.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
     case 7: s7.interpolationInner(fin, upwind, fo, tilesize); s7.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
             s7.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
+    case 8: s8.interpolationInner(fin, upwind, fo, tilesize); s8.interpolationBoundaryL(fin, fl, upwind, fo, tilesize);
+            s8.interpolationBoundaryR(fin, fr, upwind, fo, tilesize); break;
     default: return -1;
     }
     (void)op;

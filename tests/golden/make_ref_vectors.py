@@ -1,6 +1,6 @@
 """Write tests/golden/ref_vectors.npz: input/output vectors of the reference's
 own scheme objects (weno5js, weno5z, weno7js, crweno5, weno3js, upwind1st,
-hccs78, centraldifference2nd/6th/8th, tvdrk3rd), computed by oracle/_ref/libnucref.so -- the reference's sources
+hccs78, hccs77, centraldifference2nd/6th/8th, tvdrk3rd), computed by oracle/_ref/libnucref.so -- the reference's sources
 compiled unmodified from /root/reference/src by oracle/Makefile.
 
 The fixture lets the oracle be checked against the reference where oracle/_ref
@@ -77,6 +77,16 @@ def main():
             out[f"dif_{j}_in"], out[f"dif_{j}_gL"], out[f"dif_{j}_gR"] = fin, gL, gR
             out[f"dif_{j}_out"] = O.field_diff(scheme, ts, fin, 0.173, gL, gR, use_ref=True)
             j += 1
+    for scheme in (O.HCCS77,):
+        for upwind in (1, -1):
+            for kind in ("smooth", "random", "step"):
+                for (n1, n2, n3) in ((16, 2, 1), (27, 1, 2)):
+                    fin, gL, gR = lines(rng, n1, n2, n3, 4, kind)
+                    ref = O.field_reconstruct(scheme, upwind, O.halo(scheme), fin, gL, gR, use_ref=True)
+                    out[f"rec_{k}_meta"] = np.array([scheme, upwind, O.halo(scheme)])
+                    out[f"rec_{k}_in"], out[f"rec_{k}_gL"], out[f"rec_{k}_gR"] = fin, gL, gR
+                    out[f"rec_{k}_out"] = ref
+                    k += 1
     np.savez_compressed(os.path.join(HERE, "ref_vectors.npz"), **out)
     print(f"{k} reconstruction, {j} difference, 3 rk3 vectors")
 

@@ -38,7 +38,7 @@ def _close(a, b):
 
 
 @pytest.mark.parametrize("scheme", [G.WENO5JS, G.WENO5Z, G.WENO7JS, G.CRWENO5, G.HCCS, G.WENO3JS, G.UPWIND1ST,
-                                    G.HCCS78])
+                                    G.HCCS78, G.HCCS77])
 @pytest.mark.parametrize("upwind", [1, -1])
 @pytest.mark.parametrize("kind", ["smooth", "random"])
 def test_reconstruct_matches_oracle(scheme, upwind, kind):
@@ -52,7 +52,7 @@ def test_reconstruct_matches_oracle(scheme, upwind, kind):
         assert _close(got, ref) <= TOL, (n1, _close(got, ref))
 
 
-@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5, G.HCCS, G.WENO3JS, G.HCCS78])
+@pytest.mark.parametrize("scheme", [G.WENO5JS, G.CRWENO5, G.HCCS, G.WENO3JS, G.HCCS78, G.HCCS77])
 def test_reconstruct_discontinuous(scheme):
     """Step data drives the WENO weights and the hccs theta switch far from ideal."""
     rng = np.random.default_rng(5 + scheme)

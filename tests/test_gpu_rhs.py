@@ -39,7 +39,7 @@ def _perturbed_tgv(oc, seed, amp=0.2):
     return q.ravel().copy()
 
 
-ILL_CONDITIONED = (G.WENO7JS, G.HCCS, G.HCCS78)
+ILL_CONDITIONED = (G.WENO7JS, G.HCCS, G.HCCS78, G.HCCS77)
 
 
 def _ulp_floor(oc, q, steps):
@@ -88,6 +88,9 @@ CASES = [
     ((24, 20, 18), (1, 1, 1), G.WENO5JS, G.CD2, G.LF, G.NAVIERSTOKES3D),
     ((20, 20, 20), (2, 2, 1), G.WENO3JS, G.CD2, G.AUSM, G.NAVIERSTOKES3D),
     ((32, 24, 16), (1, 1, 1), G.UPWIND1ST, G.CD6, G.LF, G.EULER3D),
+    # hccs77 (src/hccs77.cpp)
+    ((16, 16, 16), (1, 1, 1), G.HCCS77, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((32, 32, 32), (2, 1, 1), G.HCCS77, G.CD8, G.AUSMP, G.NAVIERSTOKES3D),
     # AUSM+ splitting (src/physicsModel.cpp:932-1179)
     ((16, 16, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSMP, G.NAVIERSTOKES3D),
     ((24, 20, 18), (2, 2, 2), G.CRWENO5, G.CD8, G.AUSMP, G.NAVIERSTOKES3D),
@@ -277,6 +280,7 @@ LONG_CASES = [
     ((256, 12, 12), (1, 1, 1), G.WENO3JS, G.CD2, G.LF, G.NAVIERSTOKES3D),
     ((12, 160, 12), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
     ((12, 12, 100), (1, 1, 2), G.UPWIND1ST, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 12, 256), (1, 1, 1), G.HCCS77, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
 ]
 
 

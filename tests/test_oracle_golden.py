@@ -18,7 +18,7 @@ def _load():
 def test_reconstruction_vectors_bitexact():
     z = _load()
     keys = sorted({k.split("_")[1] for k in z.files if k.startswith("rec_")}, key=int)
-    assert len(keys) == 84
+    assert len(keys) == 96
     for k in keys:
         scheme, upwind, ts = (int(v) for v in z[f"rec_{k}_meta"])
         got = O.field_reconstruct(scheme, upwind, ts, z[f"rec_{k}_in"], z[f"rec_{k}_gL"], z[f"rec_{k}_gR"])

@@ -28,7 +28,8 @@ def _data(rng, n1, n2, n3, bf, kind):
             np.ascontiguousarray(line[:, :, bf + n1:]))
 
 
-@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.WENO7JS, O.CRWENO5, O.WENO3JS, O.UPWIND1ST, O.HCCS78])
+@pytest.mark.parametrize("scheme", [O.WENO5JS, O.WENO5Z, O.WENO7JS, O.CRWENO5, O.WENO3JS, O.UPWIND1ST, O.HCCS78,
+                                    O.HCCS77])
 @pytest.mark.parametrize("upwind", [1, -1])
 @pytest.mark.parametrize("kind", ["smooth", "random", "step"])
 def test_reconstruct_bitexact(scheme, upwind, kind):
