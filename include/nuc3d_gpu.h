@@ -180,6 +180,34 @@ long long nuc_average_steps(const nuc_ctx *ctx);
 int nuc_average_field(nuc_ctx *ctx, int group, int index, double *out);
 int nuc_average_means(nuc_ctx *ctx, int group, double *out, int nout);
 int nuc_average_end(nuc_ctx *ctx);
+/* nuc_average_restore: start an average of `type` from saved fields instead of
+ * zeros, as postproc's constructor does from an averaged-statistics file
+ * (src/postproc.cpp:157-256): `fields` holds every field of the type's groups
+ * in file order (primary, grads[, derived[, TKE, budget]]), N doubles each,
+ * and `steps` the averaging steps they hold (averStep). */
+int nuc_average_restore(nuc_ctx *ctx, int type, long long steps, const double *fields);
+
+/* ---- the reference's files, per rank of the config's rank grid -----------
+ * Rank (cx, cy, cz) is rank (cz*npy + cy)*npx + cx (the reference's Cartesian
+ * communicator, src/MPICommunicator.cpp:26-33) and holds its block of the
+ * global field, x fastest.  Files go to / come from directory `dir`.
+ * nuc_write_restart / nuc_read_restart: block::outputQ_binary / inputQ_binary
+ *   (src/block.cpp:609-671), NUC3d_step_<step>_ID_<rank>.bin = [double time]
+ *   [5 fields]; q is the global state (5N doubles); *time gets rank 0's time.
+ * nuc_write_tecplot: block::outputQ_tecplot (src/block.cpp:553-607),
+ *   NUC3d_step_<step>_id_<rank>.dat: node coordinates (origin subtracted; NULL
+ *   = 0) and rho u v w pressure temperature e sound_speed cell-centred.
+ * nuc_average_write: postproc::OutputAveraged_bin (src/postproc.cpp:412-522),
+ *   Averaged_step_<averStep>_id_<rank>.bin = [int averStep][groups of `type`];
+ *   type must be the context's running averaging type.
+ * nuc_average_read: the reader of src/postproc.cpp:157-256 for
+ *   Averaged_step_<step>_ID_<rank>.bin (or _id_, as the writer names it), then
+ *   nuc_average_restore. */
+int nuc_write_restart(const nuc_config *cfg, const char *dir, int step, double time, const double *q);
+int nuc_read_restart(const nuc_config *cfg, const char *dir, int step, double *time, double *q);
+int nuc_write_tecplot(const nuc_config *cfg, const char *dir, int step, const double *origin, const double *q);
+int nuc_average_write(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir);
+int nuc_average_read(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir, int step);
 
 /* ---- shell spectra ----------------------------------------------------------
  * postproc::solveFFT_* (src/postproc.cpp:931-1184) with fft::solveShell

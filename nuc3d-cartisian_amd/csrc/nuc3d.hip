@@ -2300,6 +2300,27 @@ extern "C" int nuc_average_update(nuc_ctx *x)
     return NUC_OK;
 }
 
+extern "C" int nuc_average_restore(nuc_ctx *x, int type, long long steps, const double *fields)
+{
+    if (!x || !fields || steps < 0) return NUC_EINVAL;
+    int rc = nuc_average_begin(x, type);
+    if (rc != NUC_OK) return rc;
+    static const int cnt[5] = {NPRIM, NGRAD, NDER, NTKE, NBUD};
+    const PostLayout L = post_layout(type, x->N);
+    const long seg[5] = {L.ap, L.ag, L.ad, L.at, L.bu};
+    const int ng = type == 1 ? 2 : (type == 2 ? 3 : 5);
+    size_t off = 0;
+    for (int g = 0; g < ng; g++) {
+        const size_t n = (size_t)cnt[g] * x->N;
+        NUC_CHECK(x, hipMemcpyAsync(x->post_buf + seg[g], fields + off, n * sizeof(double), hipMemcpyHostToDevice,
+                                    x->stream));
+        off += n;
+    }
+    NUC_CHECK(x, hipStreamSynchronize(x->stream));
+    x->post_step = steps;
+    return NUC_OK;
+}
+
 static long post_group(const nuc_ctx *x, int group, int *count)
 {
     static const int cnt[5] = {NPRIM, NGRAD, NDER, NTKE, NBUD};

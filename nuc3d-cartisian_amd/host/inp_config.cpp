@@ -139,6 +139,7 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
     std::map<std::string, int> io = {{"startStep", 0}, {"endStep", 100000}, {"postStep", 1},
                                      {"saveStep", 1000}, {"postSaveStep", 100}, {"Benchmark", 0}};
     std::map<std::string, double> tc = {{"cfl", 0.5}, {"tl", 0.0}, {"dt", 0.01}};
+    std::map<std::string, std::string> ft = {{"Binary", "yes"}, {"Tecplot", "yes"}, {"PostProc", "yes"}};
     {
         std::vector<std::pair<std::string, std::string>> kv;
         if (!read_pairs(dir + "/IOController.in", kv)) { err = "inp/IOController.in does not exist"; return NUC_EINVAL; }
@@ -146,11 +147,14 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
             std::istringstream vs(p.second);
             if (io.count(p.first)) { int v = 0; vs >> v; io[p.first] = v; }
             else if (tc.count(p.first)) { double v = 0; vs >> v; tc[p.first] = v; }
+            else if (ft.count(p.first)) ft[p.first] = p.second;
         }
     }
     c.cfl = tc["cfl"];
     rc.start_step = io["startStep"]; rc.end_step = io["endStep"]; rc.post_step = io["postStep"];
     rc.save_step = io["saveStep"]; rc.benchmark = io["Benchmark"]; rc.tl = tc["tl"];
+    // a type is written when its value is "yes" (src/block.cpp:245-263)
+    rc.binary = ft["Binary"] == "yes"; rc.tecplot = ft["Tecplot"] == "yes"; rc.postproc = ft["PostProc"] == "yes";
     c.precision = NUC_FP64;
     return NUC_OK;
 }

@@ -1,5 +1,5 @@
 // nuc3d_host.cpp -- host classes of nuc3d_host.hpp and host utilities
-// (initial conditions, restart files) exported through the C ABI.
+// (initial conditions) exported through the C ABI.
 #include "nuc3d_host.hpp"
 #include "hit_spectrum.hpp"
 
@@ -399,24 +399,6 @@ void initial_shu_osher(const nuc_config &c, double pert, unsigned long long seed
                 q[3 * N + id] = rho * ww;
                 q[4 * N + id] = p / (g - 1.0) + 0.5 * rho * (uu * uu + vv * vv + ww * ww);
             }
-}
-
-bool write_restart(const std::string &path, double time, const std::vector<double> &q)
-{
-    std::ofstream f(path, std::ios::binary);
-    if (!f) return false;
-    f.write(reinterpret_cast<const char *>(&time), sizeof(double));
-    f.write(reinterpret_cast<const char *>(q.data()), q.size() * sizeof(double));
-    return (bool)f;
-}
-
-bool read_restart(const std::string &path, double &time, std::vector<double> &q)
-{
-    std::ifstream f(path, std::ios::binary);
-    if (!f) return false;
-    f.read(reinterpret_cast<char *>(&time), sizeof(double));
-    f.read(reinterpret_cast<char *>(q.data()), q.size() * sizeof(double));
-    return (bool)f;
 }
 
 }  // namespace nuc3d

@@ -118,6 +118,7 @@ public:
 // run control read from inp/*.in
 struct RunControl {
     int start_step = 0, end_step = 100000, post_step = 1, save_step = 1000, benchmark = 0;
+    bool binary = true, tecplot = true, postproc = true;   // IOController.in file types
     double tl = 0.0;
     double origin[3] = {0, 0, 0};
     double mach_t = 0.3, k_peak = 8.0;
@@ -184,8 +185,5 @@ void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vecto
 void initial_hit(const nuc_config &c, double k0, double u_rms, unsigned long long seed, std::vector<double> &q);
 // Shu-Osher profile along x with a seeded solenoidal velocity perturbation
 void initial_shu_osher(const nuc_config &c, double pert, unsigned long long seed, std::vector<double> &q);
-// restart file: [double time][5 fields x-fastest] (src/block.cpp:609-671)
-bool write_restart(const std::string &path, double time, const std::vector<double> &q);
-bool read_restart(const std::string &path, double &time, std::vector<double> &q);
 
 }  // namespace nuc3d

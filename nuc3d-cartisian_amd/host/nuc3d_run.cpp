@@ -3,8 +3,9 @@
 // condition (Benchmark 0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = seeded HIT) or read the restart
 // file when startStep != 0, then step while ifsolve() holds
 // (src/IOcontroller.cpp:182-192), printing the status line of
-// block::printStatus (src/block.cpp:217-220) and writing restart files every
-// saveStep (src/block.cpp:609-638).
+// block::printStatus (src/block.cpp:217-220) and writing the restart and
+// Tecplot files IOController.in selects every saveStep and at the end
+// (src/singleBlock.cpp:50-68, src/block.cpp:238-266).
 //
 // usage: nuc3d_run [inp_dir] [--seed S]
 #include "nuc3d_host.hpp"
@@ -36,10 +37,8 @@ int main(int argc, char **argv)
     double time = 0.0;
     int step = rc.start_step;
     if (rc.start_step != 0) {
-        char path[512];
-        snprintf(path, sizeof(path), "flowData/NUC3d_step_%d_ID_0.bin", rc.start_step);
-        if (!nuc3d::read_restart(path, time, q)) {
-            fprintf(stderr, "cannot read restart file %s\n", path);
+        if (nuc_read_restart(&cfg, "flowData", rc.start_step, &time, q.data()) != NUC_OK) {
+            fprintf(stderr, "cannot read the restart files flowData/NUC3d_step_%d_ID_*.bin\n", rc.start_step);
             return 1;
         }
     } else if (rc.benchmark == 0) {
@@ -56,6 +55,14 @@ int main(int argc, char **argv)
         fprintf(stderr, "Benchmark %d is not available (0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = HIT, or a restart)\n", rc.benchmark);
         return 1;
     }
+    // block::Output (src/block.cpp:238-266): the files IOController.in asks for
+    auto output = [&](nuc3d::PDEData3d &pde) {
+        nuc3d::check(nuc_download_state(pde.ctx(), q.data()), "nuc_download_state");
+        if (rc.binary && nuc_write_restart(&cfg, "flowData", step, time, q.data()) != NUC_OK)
+            fprintf(stderr, "cannot write flowData/NUC3d_step_%d_ID_*.bin\n", step);
+        if (rc.tecplot && nuc_write_tecplot(&cfg, "flowData", step, rc.origin, q.data()) != NUC_OK)
+            fprintf(stderr, "cannot write flowData/NUC3d_step_%d_id_*.dat\n", step);
+    };
     try {
         nuc3d::PDEData3d pde(cfg);
         nuc3d::check(nuc_upload_state(pde.ctx(), q.data()), "nuc_upload_state");
@@ -67,13 +74,9 @@ int main(int argc, char **argv)
             time += info.dt;
             printf("========step = %d\n time = %g, dt = %g, residual = %g, GPU time = %g(s)\n", step, time, info.dt,
                    info.res, wall);
-            if (rc.save_step > 0 && step % rc.save_step == 0) {
-                nuc3d::check(nuc_download_state(pde.ctx(), q.data()), "nuc_download_state");
-                char path[512];
-                snprintf(path, sizeof(path), "flowData/NUC3d_step_%d_ID_0.bin", step);
-                if (!nuc3d::write_restart(path, time, q)) fprintf(stderr, "cannot write %s\n", path);
-            }
+            if (rc.save_step > 0 && step % rc.save_step == 0) output(pde);
         }
+        output(pde);   // and once more after the loop (src/singleBlock.cpp:67)
     } catch (const std::exception &e) {
         fprintf(stderr, "%s\n", e.what());
         return 2;

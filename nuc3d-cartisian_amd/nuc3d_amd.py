@@ -42,7 +42,8 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_average_begin", "nuc_average_update", "nuc_average_steps", "nuc_average_field",
            "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device",
            "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar", "nuc_init_default",
-           "nuc_init_ivc", "nuc_init_default_device", "nuc_init_ivc_device"]
+           "nuc_init_ivc", "nuc_init_default_device", "nuc_init_ivc_device", "nuc_average_restore",
+           "nuc_write_restart", "nuc_read_restart", "nuc_write_tecplot", "nuc_average_write", "nuc_average_read"]
 
 
 class NucConfig(ctypes.Structure):
@@ -162,6 +163,12 @@ def lib():
         L.nuc_average_field.argtypes = [vp, ctypes.c_int, ctypes.c_int, _dp]
         L.nuc_average_means.argtypes = [vp, ctypes.c_int, _dp, ctypes.c_int]
         L.nuc_average_end.argtypes = [vp]
+        L.nuc_average_restore.argtypes = [vp, ctypes.c_int, ctypes.c_longlong, _dp]
+        L.nuc_write_restart.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_char_p, ctypes.c_int, ctypes.c_double, _dp]
+        L.nuc_read_restart.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        L.nuc_write_tecplot.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_char_p, ctypes.c_int, _dp, _dp]
+        L.nuc_average_write.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p]
+        L.nuc_average_read.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.nuc_rk3_stage.argtypes = [vp, ctypes.c_int]
         L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
@@ -229,6 +236,30 @@ def init_tgv(cfg):
     q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
     _check(lib().nuc_init_tgv(ctypes.byref(cfg), _ptr(q)), "nuc_init_tgv")
     return q
+
+
+def write_restart(cfg, directory, step, time, q):
+    """Per-rank restart files NUC3d_step_<step>_ID_<rank>.bin (nuc_write_restart)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    _check(lib().nuc_write_restart(ctypes.byref(cfg), str(directory).encode(), int(step), float(time), _ptr(q)),
+           "nuc_write_restart")
+
+
+def read_restart(cfg, directory, step):
+    """(time, q) assembled from the per-rank restart files (nuc_read_restart)."""
+    q = np.zeros(5 * cfg.nx * cfg.ny * cfg.nz)
+    t = ctypes.c_double()
+    _check(lib().nuc_read_restart(ctypes.byref(cfg), str(directory).encode(), int(step), ctypes.byref(t), _ptr(q)),
+           "nuc_read_restart")
+    return t.value, q
+
+
+def write_tecplot(cfg, directory, step, q, origin=None):
+    """Per-rank Tecplot files NUC3d_step_<step>_id_<rank>.dat (nuc_write_tecplot)."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    o = None if origin is None else np.ascontiguousarray(origin, dtype=np.float64)
+    _check(lib().nuc_write_tecplot(ctypes.byref(cfg), str(directory).encode(), int(step),
+                                   None if o is None else _ptr(o), _ptr(q)), "nuc_write_tecplot")
 
 
 def init_default(cfg):
@@ -365,6 +396,21 @@ class Context:
         out = np.zeros(self.AVG_COUNTS[group])
         _check(lib().nuc_average_means(self.h, int(group), _ptr(out), out.size), "nuc_average_means", self.h)
         return out
+
+    def average_restore(self, avg_type, steps, fields):
+        """Start an average from saved fields (every field of the type's groups, in order)."""
+        f = np.ascontiguousarray(fields, dtype=np.float64)
+        _check(lib().nuc_average_restore(self.h, int(avg_type), int(steps), _ptr(f)), "nuc_average_restore", self.h)
+
+    def average_write(self, avg_type, directory):
+        """Per-rank averaged-statistics files Averaged_step_<n>_id_<rank>.bin."""
+        _check(lib().nuc_average_write(self.h, ctypes.byref(self.cfg), int(avg_type), str(directory).encode()),
+               "nuc_average_write", self.h)
+
+    def average_read(self, avg_type, directory, step):
+        """Restore an average from the per-rank files of averaging step `step`."""
+        _check(lib().nuc_average_read(self.h, ctypes.byref(self.cfg), int(avg_type), str(directory).encode(),
+                                      int(step)), "nuc_average_read", self.h)
 
     def average_end(self):
         _check(lib().nuc_average_end(self.h), "nuc_average_end", self.h)

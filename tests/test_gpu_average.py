@@ -132,3 +132,42 @@ def test_mixed_path_budget_error():
         print(f"budget {t:20s} fp64 {b64[i]: .6e} mixed {b32[i]: .6e} rel {e:.2e}", flush=True)
         assert e <= 1e-5, t
     assert abs(b32[0] - b64[0]) <= 1e-5 * abs(b64[0])   # tke itself
+
+
+def test_averaged_files_round_trip(tmp_path):
+    """postproc's averaged-statistics files (src/postproc.cpp:157-256,412-522):
+    write per rank, read back into a new context, and the continued average
+    equals the uninterrupted one bit for bit."""
+    gc, _ = _cfgs((16, 16, 16), (2, 1, 1), G.WENO5JS, G.CD6, G.LF)
+    q = G.init_hit(gc, k0=3.0, u_rms=0.5, seed=4)
+    with G.Context(gc) as a:
+        a.upload(q)
+        a.average_begin(3)
+        for _ in range(3):
+            a.step(1)
+            a.average_update()
+        a.average_write(3, tmp_path)
+        names = sorted(p.name for p in tmp_path.iterdir())
+        assert names == ["Averaged_step_3_id_0.bin", "Averaged_step_3_id_1.bin"]
+        raw = np.fromfile(tmp_path / names[1], dtype=np.uint8)
+        assert raw.size == 4 + 153 * 8 * 16 * 16 * 8                  # int averStep + 153 fields of 8x16x16
+        assert int(raw[:4].view(np.int32)[0]) == 3
+        state = a.download()
+        a.step(1)
+        a.average_update()
+        ref = [a.average_group(g) for g in range(5)]
+    # the reader's name (_ID_) as well as the writer's (_id_)
+    (tmp_path / names[0]).rename(tmp_path / "Averaged_step_3_ID_0.bin")
+    with G.Context(gc) as b:
+        b.upload(state)
+        b.average_read(3, tmp_path, 3)
+        assert b.average_steps() == 3
+        b.step(1)
+        b.average_update()
+        got = [b.average_group(g) for g in range(5)]
+    for g in range(5):
+        assert np.array_equal(got[g], ref[g]), g
+    with G.Context(gc) as c:
+        with pytest.raises(G.NucError):
+            c.average_read(3, tmp_path, 4)                             # no such averaging step
+
