@@ -92,6 +92,10 @@ MIXED_CASES = [
     (G.WENO7JS, G.CD8, G.LF, G.FORCING_NONE),
     (G.HCCS, G.CD6, G.AUSM, G.FORCING_NONE),
     (G.CRWENO5, G.CD8, G.LF, G.FORCING_CONST),
+    (G.WENO3JS, G.CD2, G.AUSMP, G.FORCING_NONE),
+    (G.UPWIND1ST, G.CD6, G.LF, G.FORCING_NONE),
+    (G.HCCS78, G.CD8, G.AUSMP, G.FORCING_NONE),
+    (G.HCCS77, G.CD6, G.LF, G.FORCING_ABC),
 ]
 
 
