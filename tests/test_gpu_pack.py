@@ -63,6 +63,19 @@ MEMBER_SETS = {
         (_cfg((64, 64, 64), scheme=G.HCCS, model=G.EULER3D, riemann=G.AUSM), "tgv", 7),
         (_cfg((64, 64, 64), scheme=G.HCCS, model=G.EULER3D, riemann=G.AUSM, mach=0.4), "hit", 8),
     ],
+    # the remaining operators: hccs77 with AUSM+ and cd2; WENO3-JS; hccs78
+    "hccs77-ausmp-cd2": [
+        (_cfg((40, 36, 32), scheme=G.HCCS77, riemann=G.AUSMP, vis=G.CD2), "hit", 9),
+        (_cfg((48, 48, 40), scheme=G.HCCS77, riemann=G.AUSMP, vis=G.CD2, mach=0.3), "tgv", 10),
+    ],
+    "weno3js-upwind-mixed": [
+        (_cfg((32, 32, 32), scheme=G.WENO3JS, precision=G.MIXED), "hit", 11),
+        (_cfg((40, 34, 36), scheme=G.WENO3JS, precision=G.MIXED, mach=0.35), "hit", 12),
+    ],
+    "hccs78-r4": [
+        (_cfg((72, 66, 80), scheme=G.HCCS78, vis=G.CD8), "hit", 13),
+        (_cfg((96, 70, 68), scheme=G.HCCS78, vis=G.CD8, npr=(1, 2, 1)), "hit", 14),
+    ],
 }
 
 
