@@ -155,6 +155,22 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
     rc.save_step = io["saveStep"]; rc.benchmark = io["Benchmark"]; rc.tl = tc["tl"];
     // a type is written when its value is "yes" (src/block.cpp:245-263)
     rc.binary = ft["Binary"] == "yes"; rc.tecplot = ft["Tecplot"] == "yes"; rc.postproc = ft["PostProc"] == "yes";
+    rc.post_save_step = io["postSaveStep"];
+    // ---- PostProc.in: values in order, the names ignored (src/postproc.cpp:62-77)
+    {
+        std::vector<std::pair<std::string, std::string>> kv;
+        if (read_pairs(dir + "/PostProc.in", kv)) {
+            std::vector<int> vals;
+            for (auto &p : kv) { std::istringstream vs(p.second); int v = 0; vs >> v; vals.push_back(v); }
+            if (vals.size() > 0) rc.average_type = vals[0];
+            if (vals.size() > 1) rc.average_start = vals[1];
+            if (vals.size() > 2) rc.simultaneous = vals[2];
+            if (rc.average_type < 0 || rc.average_type > 3) {
+                err = "PostProc.in: no such postproc for " + std::to_string(rc.average_type);
+                return NUC_EINVAL;
+            }
+        }
+    }
     c.precision = NUC_FP64;
     return NUC_OK;
 }

@@ -119,6 +119,11 @@ public:
 struct RunControl {
     int start_step = 0, end_step = 100000, post_step = 1, save_step = 1000, benchmark = 0;
     bool binary = true, tecplot = true, postproc = true;   // IOController.in file types
+    int post_save_step = 100;
+    // PostProc.in, six positional values (src/postproc.cpp:36-82): TimeAverageType,
+    // TimeAverageStartStep, SimutaneousData, AveragedSaveType, SimutaneousSaveType,
+    // fft post type; absent file: no post-processing
+    int average_type = 0, average_start = 0, simultaneous = 0;
     double tl = 0.0;
     double origin[3] = {0, 0, 0};
     double mach_t = 0.3, k_peak = 8.0;

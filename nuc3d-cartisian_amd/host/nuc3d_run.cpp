@@ -3,9 +3,13 @@
 // condition (Benchmark 0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = seeded HIT) or read the restart
 // file when startStep != 0, then step while ifsolve() holds
 // (src/IOcontroller.cpp:182-192), printing the status line of
-// block::printStatus (src/block.cpp:217-220) and writing the restart and
-// Tecplot files IOController.in selects every saveStep and at the end
-// (src/singleBlock.cpp:50-68, src/block.cpp:238-266).
+// block::printStatus (src/block.cpp:217-220), post-processing every postStep
+// as PostProc.in selects (running averages; enstrophy.dat and turb_para.dat),
+// and writing the restart, Tecplot and averaged-statistics files
+// IOController.in selects every saveStep and at the end (src/singleBlock.cpp:50-68,
+// src/block.cpp:238-266).  Not written: the averaged and instantaneous Tecplot
+// files and the spectra files of postproc (AveragedSaveType 1,
+// SimutaneousSaveType, fft post type).
 //
 // usage: nuc3d_run [inp_dir] [--seed S]
 #include "nuc3d_host.hpp"
@@ -55,17 +59,52 @@ int main(int argc, char **argv)
         fprintf(stderr, "Benchmark %d is not available (0 = at rest, 1 = isentropic vortex, 2 = Taylor-Green, 3 = HIT, or a restart)\n", rc.benchmark);
         return 1;
     }
-    // block::Output (src/block.cpp:238-266): the files IOController.in asks for
+    // block::Output (src/block.cpp:238-266): the files IOController.in asks for,
+    // and postproc::OutputPost's averaged-statistics files (src/postproc.cpp:310-343)
     auto output = [&](nuc3d::PDEData3d &pde) {
         nuc3d::check(nuc_download_state(pde.ctx(), q.data()), "nuc_download_state");
         if (rc.binary && nuc_write_restart(&cfg, "flowData", step, time, q.data()) != NUC_OK)
             fprintf(stderr, "cannot write flowData/NUC3d_step_%d_ID_*.bin\n", step);
         if (rc.tecplot && nuc_write_tecplot(&cfg, "flowData", step, rc.origin, q.data()) != NUC_OK)
             fprintf(stderr, "cannot write flowData/NUC3d_step_%d_id_*.dat\n", step);
+        if (rc.postproc && rc.average_type != 0 &&
+            nuc_average_write(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData") != NUC_OK)
+            fprintf(stderr, "cannot write AveragedFlowData/Averaged_step_*_id_*.bin\n");
+    };
+    // postproc::solvePost (src/postproc.cpp:281-308): the running averages, and
+    // solveQ's global statistics appended to enstrophy.dat and turb_para.dat
+    // (src/postproc.cpp:880-920)
+    auto post = [&](nuc3d::PDEData3d &pde) {
+        if (rc.average_type != 0) nuc3d::check(nuc_average_update(pde.ctx()), "nuc_average_update");
+        if (rc.simultaneous != 0) {
+            double st[25];
+            nuc3d::check(nuc_stats(pde.ctx(), st), "nuc_stats");
+            FILE *e = fopen("enstrophy.dat", "a"), *t = fopen("turb_para.dat", "a");
+            if (e) {
+                fprintf(e, "%d %.12g", step, time);
+                for (int i = 0; i < 16; i++) fprintf(e, " %.12g", st[i]);
+                fprintf(e, "\n");
+                fclose(e);
+            }
+            if (t) {
+                fprintf(t, "%d %.12g", step, time);
+                for (int i = 16; i < 25; i++) fprintf(t, " %.12g", st[i]);
+                fprintf(t, "\n");
+                fclose(t);
+            }
+        }
     };
     try {
         nuc3d::PDEData3d pde(cfg);
         nuc3d::check(nuc_upload_state(pde.ctx(), q.data()), "nuc_upload_state");
+        if (rc.average_type != 0) {
+            // a new average, or the one saved at TimeAverageStartStep (src/postproc.cpp:157-256)
+            if (rc.average_start != 0)
+                nuc3d::check(nuc_average_read(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData", rc.average_start),
+                             "nuc_average_read");
+            else
+                nuc3d::check(nuc_average_begin(pde.ctx(), rc.average_type), "nuc_average_begin");
+        }
         while (time <= rc.tl && step < rc.end_step) {
             auto t0 = std::chrono::steady_clock::now();
             nuc_step_info info = pde.step(1);
@@ -74,6 +113,8 @@ int main(int argc, char **argv)
             time += info.dt;
             printf("========step = %d\n time = %g, dt = %g, residual = %g, GPU time = %g(s)\n", step, time, info.dt,
                    info.res, wall);
+            // singleBlock::solvePDE's cadence (src/singleBlock.cpp:52-66, src/IOcontroller.cpp:194-213)
+            if (rc.post_step > 0 && step % rc.post_step == 0) post(pde);
             if (rc.save_step > 0 && step % rc.save_step == 0) output(pde);
         }
         output(pde);   // and once more after the loop (src/singleBlock.cpp:67)

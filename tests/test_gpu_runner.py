@@ -1,8 +1,9 @@
 """The reference-shaped executable (nuc3d_run: src/NUC3d.cpp:3-14,
-src/singleBlock.cpp:31-71) end to end: inp/*.in in, per-rank restart and
-Tecplot files out every saveStep, and a restart from an intermediate step that
-reproduces the uninterrupted run bit for bit; the final state equals the same
-steps taken through the C ABI."""
+src/singleBlock.cpp:31-71) end to end: inp/*.in in; per-rank restart, Tecplot
+and averaged-statistics files out every saveStep; enstrophy.dat and
+turb_para.dat every postStep; a restart (state and running average) from an
+intermediate step that reproduces the uninterrupted run bit for bit; the final
+state, statistics and averages equal the same steps taken through the C ABI."""
 import os
 import subprocess
 
@@ -17,8 +18,11 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 RUNNER = os.path.join(ROOT, "nuc3d-cartisian_amd", "nuc3d_run")
 
 
-def _write_inp(d, start, end):
+def _write_inp(d, start, end, average_start=0):
     os.makedirs(d, exist_ok=True)
+    with open(os.path.join(d, "PostProc.in"), "w") as f:
+        f.write(f"AverageType 1\nAverageStartStep {average_start}\nSimutaneousData 1\nAveragedSaveType 0\n"
+                "SimutaneousSaveType 0\n")
     with open(os.path.join(d, "block.in"), "w") as f:
         f.write("1 2 1\n24 24 24\n6.283185307179586 6.283185307179586 6.283185307179586\n")
     with open(os.path.join(d, "Method.in"), "w") as f:
@@ -28,8 +32,8 @@ def _write_inp(d, start, end):
         f.write("Neqs 5\nEosName IdealGas\nRiemannSolver LF\nPDEModel NavierStokes3d\nViscousModel Sutherland\n"
                 "Forcing None\nReynolds 400.0\nMach 0.2\n")
     with open(os.path.join(d, "IOController.in"), "w") as f:
-        f.write(f"startStep {start}\nendStep {end}\nsaveStep 2\npostStep 100\npostSaveStep 100\nBenchmark 2\n"
-                "cfl 0.5\ntl 100.0\ndt 0.01\nBinary yes\nTecplot yes\nPostProc no\n")
+        f.write(f"startStep {start}\nendStep {end}\nsaveStep 2\npostStep 1\npostSaveStep 100\nBenchmark 2\n"
+                "cfl 0.5\ntl 100.0\ndt 0.01\nBinary yes\nTecplot yes\nPostProc yes\n")
 
 
 def _run(cwd, inp):
@@ -44,6 +48,8 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
         pytest.fail("nuc3d_run is not built (make -C nuc3d-cartisian_amd)")
     flow = tmp_path / "flowData"
     flow.mkdir()
+    aver = tmp_path / "AveragedFlowData"
+    aver.mkdir()
     _write_inp(tmp_path / "inp", 0, 4)
     out = _run(tmp_path, str(tmp_path / "inp"))
     assert out.count("========step =") == 4
@@ -51,20 +57,44 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
     for step in (2, 4):
         for r in range(2):
             assert f"NUC3d_step_{step}_ID_{r}.bin" in names and f"NUC3d_step_{step}_id_{r}.dat" in names
+    assert sorted(os.listdir(aver)) == [f"Averaged_step_{a}_id_{r}.bin" for a in (2, 4) for r in range(2)]
     cfg, _, _ = G.config_from_inp(str(tmp_path / "inp"))
     t4, q4 = G.read_restart(cfg, flow, 4)
-    # the same 4 steps through the C ABI
+    # the same 4 steps through the C ABI, with the statistics and averages of each
+    stats = []
     with G.Context(cfg) as ctx:
         ctx.upload(G.init_tgv(cfg))
-        ctx.step(4)
+        ctx.average_begin(1)
+        for _ in range(4):
+            ctx.step(1)
+            ctx.average_update()
+            stats.append(ctx.stats())
         ref = ctx.download()
+        rho_avg = ctx.average_field(0, 0)
     assert np.array_equal(q4, ref)
+    ens = [list(map(float, l.split())) for l in open(tmp_path / "enstrophy.dat")]
+    tur = [list(map(float, l.split())) for l in open(tmp_path / "turb_para.dat")]
+    assert [int(r[0]) for r in ens] == [1, 2, 3, 4] and all(len(r) == 18 for r in ens)
+    assert all(len(r) == 11 for r in tur)
+    for k in range(4):   # 12 significant digits, as written
+        assert np.allclose(ens[k][2:], stats[k][:16], rtol=1e-11, atol=1e-300)
+        assert np.allclose(tur[k][2:], stats[k][16:25], rtol=1e-11, atol=1e-300)
+    # the averaged file at step 4: [int averStep][10 primaries][21 gradients] per rank block
+    raw = np.fromfile(aver / "Averaged_step_4_id_1.bin", dtype=np.uint8)
+    assert int(raw[:4].view(np.int32)[0]) == 4
+    blk = raw[4:].view(np.float64).reshape(31, 24, 12, 24)[0]           # rho of rank 1 (cy = 1)
+    assert np.array_equal(blk, rho_avg.reshape(24, 24, 24)[:, 12:, :])
+    keep_avg = {r: (aver / f"Averaged_step_4_id_{r}.bin").read_bytes() for r in range(2)}
     # restart at step 2 (inputQ_binary, src/block.cpp:640-671) and run to 4 again
     keep = {r: (flow / f"NUC3d_step_4_ID_{r}.bin").read_bytes() for r in range(2)}
     for r in range(2):
         os.remove(flow / f"NUC3d_step_4_ID_{r}.bin")
-    _write_inp(tmp_path / "inp2", 2, 4)
+        os.remove(aver / f"Averaged_step_4_id_{r}.bin")
+    _write_inp(tmp_path / "inp2", 2, 4, average_start=2)
     out = _run(tmp_path, str(tmp_path / "inp2"))
     assert out.count("========step =") == 2
     for r in range(2):
         assert (flow / f"NUC3d_step_4_ID_{r}.bin").read_bytes() == keep[r]
+        assert (aver / f"Averaged_step_4_id_{r}.bin").read_bytes() == keep_avg[r]
+    lines = open(tmp_path / "enstrophy.dat").read().splitlines()
+    assert lines[4:] == lines[2:4]   # steps 3 and 4 again, the same text
