@@ -208,6 +208,12 @@ int nuc_read_restart(const nuc_config *cfg, const char *dir, int step, double *t
 int nuc_write_tecplot(const nuc_config *cfg, const char *dir, int step, const double *origin, const double *q);
 int nuc_average_write(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir);
 int nuc_average_read(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir, int step);
+/* nuc_average_write_tecplot: postproc::OutputAveraged_tec (src/postproc.cpp:523-720),
+ *   Averaged_tec_step_<averStep>_id_<rank>.dat: title with the time, node
+ *   coordinates (origin subtracted; NULL = 0) and the type's averaged fields
+ *   cell-centred, 12 significant digits. */
+int nuc_average_write_tecplot(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir, double time,
+                              const double *origin);
 
 /* ---- shell spectra ----------------------------------------------------------
  * postproc::solveFFT_* (src/postproc.cpp:931-1184) with fft::solveShell

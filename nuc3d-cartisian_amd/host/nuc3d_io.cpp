@@ -10,7 +10,9 @@
 //     [the averaged groups of the averaging type], written as
 //     Averaged_step_<averStep>_id_<rank>.bin and read as
 //     Averaged_step_<step>_ID_<rank>.bin (the reference's writer and reader
-//     differ in the case of "id"; the reader here accepts both).
+//     differ in the case of "id"; the reader here accepts both);
+//   * Tecplot files of the averaged fields, postproc::OutputAveraged_tec
+//     (src/postproc.cpp:523-720), Averaged_tec_step_<averStep>_id_<rank>.dat.
 //
 // Rank numbering: the reference's Cartesian communicator has dims
 // (npz, npy, npx) (src/MPICommunicator.cpp:26-33) and MPI numbers Cartesian
@@ -81,6 +83,54 @@ std::string file_name(const char *dir, const char *stem, long step, const char *
 
 const int kGroupCount[5] = {10, 21, 28, 49, 45};
 
+// Tecplot variable names of the averaged groups, in field order
+// (src/postproc.hpp:49-121): part of the file format
+const char *const kAvgNames[153] = {
+    // primary
+    "rho", "u", "v", "w", "p", "T", "e", "alpha", "miu", "coeff",
+    // gradients, vorticity and its squares
+    "dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz", "dpdx", "dpdy", "dpdz",
+    "dTdx", "dTdy", "dTdz", "omega0", "omega1", "omega2", "omega0^2", "omega1^2", "omega2^2",
+    // derived
+    "rhou", "rhov", "rhow", "rhoe", "uu", "uv", "uw", "vv", "vw", "ww",
+    "rhouu", "rhouv", "rhouw", "rhovv", "rhovw", "rhoww",
+    "tau_xx", "tau_xy", "tau_xz", "tau_yy", "tau_yz", "tau_zz",
+    "rhotau_xx", "rhotau_xy", "rhotau_xz", "rhotau_yy", "rhotau_yz", "rhotau_zz",
+    // TKE fields
+    "up", "vp", "wp", "pdudx", "pdvdy", "pdwdz",
+    "dtau_xxdx", "dtau_yxdx", "dtau_zxdx", "dtau_xydy", "dtau_yydy", "dtau_zydy",
+    "dtau_xzdz", "dtau_yzdz", "dtau_zzdz",
+    "utau_xx", "vtau_yx", "wtau_zx", "uktau_kx", "duktau_kxdx",
+    "utau_xy", "vtau_yy", "wtau_zy", "uktau_ky", "duktau_kydy",
+    "utau_xz", "vtau_yz", "wtau_zz", "uktau_kz", "duktau_kzdz",
+    "tau_xxdudx", "tau_yxdvdx", "tau_zxdwdx", "tau_xydudy", "tau_yydvdy", "tau_zydwdy",
+    "tau_xzdudz", "tau_yzdvdz", "tau_zzdwdz",
+    "rhouuu", "rhouuv", "rhouuw", "rhovuv", "rhouvw", "rhouww", "rhovvv", "rhovvw", "rhovww", "rhowww",
+    // TKE budget
+    "tke", "production", "turbulen_transportation", "pressure_diffusion", "p1", "p2", "p3",
+    "viscous_diffusion", "v1", "v2", "v3", "convection",
+    "drhoudx", "drhoudy", "drhoudz", "drhovdx", "drhovdy", "drhovdz", "drhowdx", "drhowdy", "drhowdz",
+    "up_t", "vp_t", "wp_t", "dupdx_t", "dvpdy_t", "dwpdz_t",
+    "uktau_kx_t", "uktau_ky_t", "uktau_kz_t", "duktau_kxdx_t", "duktau_kydy_t", "duktau_kzdz_t",
+    "rhouk", "rhovk", "rhowk", "drhoukdx", "drhovkdy", "drhowkdz",
+    "u_tke", "v_tke", "w_tke", "du_tkedx", "dv_tkedy", "dw_tkedz"};
+
+// node coordinates of a rank's block, x then y then z (src/block.cpp:124-138)
+void write_nodes(std::ofstream &f, const RankGrid &g, const double L[3], const double o[3], int rank)
+{
+    int c[3];
+    rank_coords(g, rank, c);
+    for (int d = 0; d < 3; d++) {
+        const double h = L[d] / g.n[d], base = L[d] / g.np[d] * c[d];
+        for (int k = 0; k <= g.nl[2]; k++)
+            for (int j = 0; j <= g.nl[1]; j++)
+                for (int i = 0; i <= g.nl[0]; i++) {
+                    const int ix = d == 0 ? i : (d == 1 ? j : k);
+                    f << base + h * ix - o[d] << "\n";
+                }
+    }
+}
+
 int groups_of(int type) { return type == 1 ? 2 : (type == 2 ? 3 : 5); }
 
 }  // namespace
@@ -143,16 +193,7 @@ extern "C" int nuc_write_tecplot(const nuc_config *cfg, const char *dir, int ste
         f << "\n Zone I = " << g.nl[0] + 1 << ", J= " << g.nl[1] + 1 << ", K=" << g.nl[2] + 1
           << "\n DATAPACKING=BLOCK, VARLOCATION=([4-11]=CELLCENTERED)\n";
         f << std::setprecision(12);
-        // node coordinates of the rank's block (src/block.cpp:124-138)
-        for (int d = 0; d < 3; d++) {
-            const double h = L[d] / g.n[d], base = L[d] / g.np[d] * c[d];
-            for (int k = 0; k <= g.nl[2]; k++)
-                for (int j = 0; j <= g.nl[1]; j++)
-                    for (int i = 0; i <= g.nl[0]; i++) {
-                        const int ix = d == 0 ? i : (d == 1 ? j : k);
-                        f << base + h * ix - o[d] << "\n";
-                    }
-        }
+        write_nodes(f, g, L, o, r);
         // con2prim's primitive and acoustic fields (src/physicsModel.cpp:331-344)
         for (int v = 0; v < 8; v++)
             for (long id = 0; id < g.Nl; id++) {
@@ -229,4 +270,45 @@ extern "C" int nuc_average_read(nuc_ctx *ctx, const nuc_config *cfg, int type, c
         }
     }
     return nuc_average_restore(ctx, type, averStep, all.data());
+}
+
+extern "C" int nuc_average_write_tecplot(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir, double time,
+                                         const double *origin)
+{
+    RankGrid g;
+    if (!ctx || !cfg || !dir || type < 1 || type > 3 || !rank_grid(cfg, g)) return NUC_EINVAL;
+    const long long steps = nuc_average_steps(ctx);
+    if (steps < 0) return NUC_EINVAL;
+    const int ng = groups_of(type);
+    int nf = 0;
+    for (int gr = 0; gr < ng; gr++) nf += kGroupCount[gr];
+    const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+    const double L[3] = {cfg->lx, cfg->ly, cfg->lz};
+    std::vector<std::ofstream> files(g.ranks);
+    for (int r = 0; r < g.ranks; r++) {
+        files[r].open(file_name(dir, "Averaged_tec_step_", steps, "_id_", r, ".dat"));
+        if (!files[r]) return NUC_EINVAL;
+        std::ofstream &f = files[r];
+        // header of postproc::OutputAveraged_tec (src/postproc.cpp:554-641); the
+        // time at the stream's default 6 digits, the values at 12
+        f << "title=time_" << time << "\n" << "variables=x,y,z";
+        for (int i = 0; i < nf; i++) f << "," << kAvgNames[i];
+        f << "\n Zone I = " << g.nl[0] + 1 << ", J= " << g.nl[1] + 1 << ", K=" << g.nl[2] + 1
+          << "\n DATAPACKING=BLOCK, VARLOCATION=([4-" << 3 + nf << "]=CELLCENTERED)\n";
+        f << std::setprecision(12);
+        write_nodes(f, g, L, o, r);
+    }
+    std::vector<double> field(g.N), loc(g.Nl);
+    for (int gr = 0; gr < ng; gr++)
+        for (int i = 0; i < kGroupCount[gr]; i++) {
+            int rc = nuc_average_field(ctx, gr, i, field.data());
+            if (rc != NUC_OK) return rc;
+            for (int r = 0; r < g.ranks; r++) {
+                block_copy(g, r, field.data(), loc.data(), true);
+                for (long id = 0; id < g.Nl; id++) files[r] << loc[id] << "\n";
+            }
+        }
+    for (auto &f : files)
+        if (!f) return NUC_EINVAL;
+    return NUC_OK;
 }

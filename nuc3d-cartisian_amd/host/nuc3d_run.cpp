@@ -7,9 +7,9 @@
 // as PostProc.in selects (running averages; enstrophy.dat and turb_para.dat),
 // and writing the restart, Tecplot and averaged-statistics files
 // IOController.in selects every saveStep and at the end (src/singleBlock.cpp:50-68,
-// src/block.cpp:238-266).  Not written: the averaged and instantaneous Tecplot
-// files and the spectra files of postproc (AveragedSaveType 1,
-// SimutaneousSaveType, fft post type).
+// src/block.cpp:238-266), with the averaged Tecplot files when AveragedSaveType
+// is 1.  Not written: postproc's instantaneous Tecplot files and spectra files
+// (SimutaneousSaveType, fft post type).
 //
 // usage: nuc3d_run [inp_dir] [--seed S]
 #include "nuc3d_host.hpp"
@@ -67,9 +67,14 @@ int main(int argc, char **argv)
             fprintf(stderr, "cannot write flowData/NUC3d_step_%d_ID_*.bin\n", step);
         if (rc.tecplot && nuc_write_tecplot(&cfg, "flowData", step, rc.origin, q.data()) != NUC_OK)
             fprintf(stderr, "cannot write flowData/NUC3d_step_%d_id_*.dat\n", step);
-        if (rc.postproc && rc.average_type != 0 &&
-            nuc_average_write(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData") != NUC_OK)
-            fprintf(stderr, "cannot write AveragedFlowData/Averaged_step_*_id_*.bin\n");
+        if (rc.postproc && rc.average_type != 0) {
+            if (nuc_average_write(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData") != NUC_OK)
+                fprintf(stderr, "cannot write AveragedFlowData/Averaged_step_*_id_*.bin\n");
+            // AveragedSaveType 1: binary and Tecplot (src/postproc.cpp:322-334)
+            if (rc.averaged_save_type == 1 &&
+                nuc_average_write_tecplot(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData", time, rc.origin) != NUC_OK)
+                fprintf(stderr, "cannot write AveragedFlowData/Averaged_tec_step_*_id_*.dat\n");
+        }
     };
     // postproc::solvePost (src/postproc.cpp:281-308): the running averages, and
     // solveQ's global statistics appended to enstrophy.dat and turb_para.dat

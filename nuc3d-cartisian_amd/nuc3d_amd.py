@@ -43,7 +43,8 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_average_means", "nuc_average_end", "nuc_init_tgv_device", "nuc_init_hit_device",
            "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar", "nuc_init_default",
            "nuc_init_ivc", "nuc_init_default_device", "nuc_init_ivc_device", "nuc_average_restore",
-           "nuc_write_restart", "nuc_read_restart", "nuc_write_tecplot", "nuc_average_write", "nuc_average_read"]
+           "nuc_write_restart", "nuc_read_restart", "nuc_write_tecplot", "nuc_average_write", "nuc_average_read",
+           "nuc_average_write_tecplot"]
 
 
 class NucConfig(ctypes.Structure):
@@ -169,6 +170,8 @@ def lib():
         L.nuc_write_tecplot.argtypes = [ctypes.POINTER(NucConfig), ctypes.c_char_p, ctypes.c_int, _dp, _dp]
         L.nuc_average_write.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p]
         L.nuc_average_read.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.nuc_average_write_tecplot.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p,
+                                                ctypes.c_double, _dp]
         L.nuc_rk3_stage.argtypes = [vp, ctypes.c_int]
         L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
@@ -406,6 +409,13 @@ class Context:
         """Per-rank averaged-statistics files Averaged_step_<n>_id_<rank>.bin."""
         _check(lib().nuc_average_write(self.h, ctypes.byref(self.cfg), int(avg_type), str(directory).encode()),
                "nuc_average_write", self.h)
+
+    def average_write_tecplot(self, avg_type, directory, time, origin=None):
+        """Per-rank Tecplot files of the averaged fields, Averaged_tec_step_<n>_id_<rank>.dat."""
+        o = None if origin is None else np.ascontiguousarray(origin, dtype=np.float64)
+        _check(lib().nuc_average_write_tecplot(self.h, ctypes.byref(self.cfg), int(avg_type), str(directory).encode(),
+                                               float(time), None if o is None else _ptr(o)),
+               "nuc_average_write_tecplot", self.h)
 
     def average_read(self, avg_type, directory, step):
         """Restore an average from the per-rank files of averaging step `step`."""

@@ -134,6 +134,27 @@ def test_mixed_path_budget_error():
     assert abs(b32[0] - b64[0]) <= 1e-5 * abs(b64[0])   # tke itself
 
 
+def _check_average_tecplot(path, ctx, n, npr, rank):
+    """postproc::OutputAveraged_tec's layout (src/postproc.cpp:523-720)."""
+    lines = open(path).read().split("\n")
+    assert lines[0] == "title=time_0.125"
+    names = lines[1].split(",")
+    assert names[:3] == ["variables=x", "y", "z"] and len(names) == 3 + 153
+    assert names[3] == "rho" and names[3 + 31] == "rhou" and names[3 + 108] == "tke" and names[-1] == "dw_tkedz"
+    nl = [n[d] // npr[d] for d in range(3)]
+    assert lines[2] == f" Zone I = {nl[0] + 1}, J= {nl[1] + 1}, K={nl[2] + 1}"
+    assert lines[3] == " DATAPACKING=BLOCK, VARLOCATION=([4-156]=CELLCENTERED)"
+    vals = np.array([float(x) for x in lines[4:] if x.strip()])
+    nn = (nl[0] + 1) * (nl[1] + 1) * (nl[2] + 1)
+    nc = nl[0] * nl[1] * nl[2]
+    assert vals.size == 3 * nn + 153 * nc
+    cx = rank % npr[0]
+    for g, i, k in ((0, 0, 0), (1, 15, 10 + 15), (4, 0, 108)):   # rho, omega0, tke
+        f = ctx.average_field(g, i).reshape(n[2], n[1], n[0])[:, :, cx * nl[0]:(cx + 1) * nl[0]].ravel()
+        got = vals[3 * nn + k * nc:3 * nn + (k + 1) * nc]
+        assert np.allclose(got, f, rtol=1e-11, atol=1e-12 * np.abs(f).max()), (g, i)
+
+
 def test_averaged_files_round_trip(tmp_path):
     """postproc's averaged-statistics files (src/postproc.cpp:157-256,412-522):
     write per rank, read back into a new context, and the continued average
@@ -147,7 +168,11 @@ def test_averaged_files_round_trip(tmp_path):
             a.step(1)
             a.average_update()
         a.average_write(3, tmp_path)
-        names = sorted(p.name for p in tmp_path.iterdir())
+        tec_dir = tmp_path / "tec"
+        tec_dir.mkdir()
+        a.average_write_tecplot(3, tec_dir, 0.125)
+        _check_average_tecplot(tec_dir / "Averaged_tec_step_3_id_1.dat", a, (16, 16, 16), (2, 1, 1), 1)
+        names = sorted(p.name for p in tmp_path.iterdir() if p.is_file())
         assert names == ["Averaged_step_3_id_0.bin", "Averaged_step_3_id_1.bin"]
         raw = np.fromfile(tmp_path / names[1], dtype=np.uint8)
         assert raw.size == 4 + 153 * 8 * 16 * 16 * 8                  # int averStep + 153 fields of 8x16x16

@@ -21,7 +21,7 @@ RUNNER = os.path.join(ROOT, "nuc3d-cartisian_amd", "nuc3d_run")
 def _write_inp(d, start, end, average_start=0):
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "PostProc.in"), "w") as f:
-        f.write(f"AverageType 1\nAverageStartStep {average_start}\nSimutaneousData 1\nAveragedSaveType 0\n"
+        f.write(f"AverageType 1\nAverageStartStep {average_start}\nSimutaneousData 1\nAveragedSaveType 1\n"
                 "SimutaneousSaveType 0\n")
     with open(os.path.join(d, "block.in"), "w") as f:
         f.write("1 2 1\n24 24 24\n6.283185307179586 6.283185307179586 6.283185307179586\n")
@@ -57,7 +57,8 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
     for step in (2, 4):
         for r in range(2):
             assert f"NUC3d_step_{step}_ID_{r}.bin" in names and f"NUC3d_step_{step}_id_{r}.dat" in names
-    assert sorted(os.listdir(aver)) == [f"Averaged_step_{a}_id_{r}.bin" for a in (2, 4) for r in range(2)]
+    assert sorted(os.listdir(aver)) == sorted([f"Averaged_step_{a}_id_{r}.bin" for a in (2, 4) for r in range(2)]
+                                              + [f"Averaged_tec_step_{a}_id_{r}.dat" for a in (2, 4) for r in range(2)])
     cfg, _, _ = G.config_from_inp(str(tmp_path / "inp"))
     t4, q4 = G.read_restart(cfg, flow, 4)
     # the same 4 steps through the C ABI, with the statistics and averages of each
