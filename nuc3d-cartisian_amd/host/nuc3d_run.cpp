@@ -4,16 +4,18 @@
 // file when startStep != 0, then step while ifsolve() holds
 // (src/IOcontroller.cpp:182-192), printing the status line of
 // block::printStatus (src/block.cpp:217-220), post-processing every postStep
-// as PostProc.in selects (running averages; enstrophy.dat and turb_para.dat),
+// as PostProc.in selects (running averages; enstrophy.dat, turb_para.dat and
+// the shell spectra files),
 // and writing the restart, Tecplot and averaged-statistics files
 // IOController.in selects every saveStep and at the end (src/singleBlock.cpp:50-68,
 // src/block.cpp:238-266), with the averaged Tecplot files when AveragedSaveType
-// is 1.  Not written: postproc's instantaneous Tecplot files and spectra files
-// (SimutaneousSaveType, fft post type).
+// is 1.  Not written: postproc's instantaneous Tecplot files
+// (SimutaneousSaveType).
 //
 // usage: nuc3d_run [inp_dir] [--seed S]
 #include "nuc3d_host.hpp"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -96,6 +98,31 @@ int main(int argc, char **argv)
                 for (int i = 16; i < 25; i++) fprintf(t, " %.12g", st[i]);
                 fprintf(t, "\n");
                 fclose(t);
+            }
+            // postproc::solveFFT's shell spectra (src/postproc.cpp:931-1184),
+            // written as fft::solveShell does (src/fft.cpp:449-470):
+            // flowDataTemp/power_shell_<step>_<name>.dat, "k E E/sum(E)" per shell
+            const int kmax = std::min(cfg.nx, std::min(cfg.ny, cfg.nz)) / 2;
+            std::vector<double> kin(5 * (size_t)kmax), sc((size_t)kmax);
+            nuc3d::check(nuc_spectra_kinetic(pde.ctx(), kin.data(), kmax), "nuc_spectra_kinetic");
+            auto shell_file = [&](const char *name, const double *e) {
+                char path[256];
+                snprintf(path, sizeof(path), "flowDataTemp/power_shell_%d_%s.dat", step, name);
+                FILE *f = fopen(path, "w");
+                if (!f) { fprintf(stderr, "cannot write %s\n", path); return; }
+                double sum = 0.0;
+                for (int k = 0; k < kmax; k++) sum += e[k];
+                for (int k = 0; k < kmax; k++) fprintf(f, "%d %.12g %.12g\n", k, e[k], e[k] / sum);
+                fprintf(f, "\n");
+                fclose(f);
+            };
+            const char *kn[5] = {"kin", "kinc", "kins", "kinl", "kinr"};
+            for (int s5 = 0; s5 < 5; s5++) shell_file(kn[s5], kin.data() + (size_t)s5 * kmax);
+            const int sf[4] = {NUC_SPEC_HELICITY, NUC_SPEC_ENSTROPHY, NUC_SPEC_DENSITY, NUC_SPEC_PRESSURE};
+            const char *sn[4] = {"hel", "ens", "den", "pre"};
+            for (int i = 0; i < 4; i++) {
+                nuc3d::check(nuc_spectrum_scalar(pde.ctx(), sf[i], sc.data(), kmax), "nuc_spectrum_scalar");
+                shell_file(sn[i], sc.data());
             }
         }
     };

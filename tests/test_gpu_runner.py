@@ -1,7 +1,7 @@
 """The reference-shaped executable (nuc3d_run: src/NUC3d.cpp:3-14,
 src/singleBlock.cpp:31-71) end to end: inp/*.in in; per-rank restart, Tecplot
-and averaged-statistics files out every saveStep; enstrophy.dat and
-turb_para.dat every postStep; a restart (state and running average) from an
+and averaged-statistics files out every saveStep; enstrophy.dat,
+turb_para.dat and the shell spectra files every postStep; a restart (state and running average) from an
 intermediate step that reproduces the uninterrupted run bit for bit; the final
 state, statistics and averages equal the same steps taken through the C ABI."""
 import os
@@ -50,6 +50,7 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
     flow.mkdir()
     aver = tmp_path / "AveragedFlowData"
     aver.mkdir()
+    (tmp_path / "flowDataTemp").mkdir()
     _write_inp(tmp_path / "inp", 0, 4)
     out = _run(tmp_path, str(tmp_path / "inp"))
     assert out.count("========step =") == 4
@@ -72,6 +73,8 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
             stats.append(ctx.stats())
         ref = ctx.download()
         rho_avg = ctx.average_field(0, 0)
+        kin = ctx.spectra_kinetic()
+        ens_sp = ctx.spectrum_scalar(G.SPEC_ENSTROPHY)
     assert np.array_equal(q4, ref)
     ens = [list(map(float, l.split())) for l in open(tmp_path / "enstrophy.dat")]
     tur = [list(map(float, l.split())) for l in open(tmp_path / "turb_para.dat")]
@@ -80,6 +83,16 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
     for k in range(4):   # 12 significant digits, as written
         assert np.allclose(ens[k][2:], stats[k][:16], rtol=1e-11, atol=1e-300)
         assert np.allclose(tur[k][2:], stats[k][16:25], rtol=1e-11, atol=1e-300)
+    # shell spectra of every post step (src/fft.cpp:449-470): "k E E/sum"
+    temp = tmp_path / "flowDataTemp"
+    for step in range(1, 5):
+        for name in ("kin", "kinc", "kins", "kinl", "kinr", "hel", "ens", "den", "pre"):
+            assert (temp / f"power_shell_{step}_{name}.dat").exists(), (step, name)
+    for name, e in (("kin", kin[0]), ("kins", kin[2]), ("kinr", kin[4]), ("ens", ens_sp)):
+        rows = np.array([list(map(float, l.split())) for l in open(temp / f"power_shell_4_{name}.dat") if l.strip()])
+        assert rows.shape == (12, 3) and np.array_equal(rows[:, 0], np.arange(12))
+        assert np.allclose(rows[:, 1], e, rtol=1e-11, atol=1e-300), name
+        assert np.allclose(rows[:, 2], e / e.sum(), rtol=1e-11, atol=1e-300), name
     # the averaged file at step 4: [int averStep][10 primaries][21 gradients] per rank block
     raw = np.fromfile(aver / "Averaged_step_4_id_1.bin", dtype=np.uint8)
     assert int(raw[:4].view(np.int32)[0]) == 4
