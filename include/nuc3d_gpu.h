@@ -214,6 +214,11 @@ int nuc_average_read(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *
  *   cell-centred, 12 significant digits. */
 int nuc_average_write_tecplot(nuc_ctx *ctx, const nuc_config *cfg, int type, const char *dir, double time,
                               const double *origin);
+/* nuc_write_q_tecplot: postproc::OutputTemporal (src/postproc.cpp:346-405),
+ *   Q_tec_step_<step>_id_<rank>.dat: title with the time, node coordinates and
+ *   the 16 fields of nuc_q_fields of the current state, cell-centred. */
+int nuc_write_q_tecplot(nuc_ctx *ctx, const nuc_config *cfg, const char *dir, int step, double time,
+                        const double *origin);
 
 /* ---- shell spectra ----------------------------------------------------------
  * postproc::solveFFT_* (src/postproc.cpp:931-1184) with fft::solveShell
@@ -227,6 +232,12 @@ int nuc_average_write_tecplot(nuc_ctx *ctx, const nuc_config *cfg, int type, con
  *   reference's solveShell counts it.
  * Shell sums are FP64; their order within a workgroup is not fixed. */
 enum nuc_spectrum_field { NUC_SPEC_DENSITY = 0, NUC_SPEC_PRESSURE = 1, NUC_SPEC_ENSTROPHY = 2, NUC_SPEC_HELICITY = 3 };
+/* nuc_q_fields: the 16 per-point fields of postproc::solveQ
+ * (src/postproc.cpp:736-835) of the current state into out (16N host doubles,
+ * field after field): du/dx du/dy du/dz dv/dx ... dw/dz with the viscous
+ * schemes, vorticity (3), enstrophy |omega|^2/2, Q, helicity omega.u, kinetic
+ * energy rho|u|^2/2. */
+int nuc_q_fields(nuc_ctx *ctx, double *out);
 int nuc_spectra_kinetic(nuc_ctx *ctx, double *out, int kmax);
 int nuc_spectrum_scalar(nuc_ctx *ctx, int field, double *out, int kmax);
 

@@ -2761,6 +2761,73 @@ __global__ __launch_bounds__(256) void k_spec_prep(SpecPrep A)
     }
 }
 
+// the per-point fields of postproc::solveQ after the gradients
+// (src/postproc.cpp:815-835): vorticity, enstrophy, Q, helicity, kinetic energy
+struct QArgs {
+    const double GMEM *q;
+    const double GMEM *uvw;
+    double GMEM *f;                // 16N: 9 gradients (input), then the 7 outputs
+    long N;
+};
+
+__global__ __launch_bounds__(256) void k_qfields(QArgs A)
+{
+    const long id = (long)blockIdx.x * 256 + threadIdx.x;
+    if (id >= A.N) return;
+    const long N = A.N;
+    const double ux = A.f[id], uy = A.f[N + id], uz = A.f[2 * N + id];
+    const double vx = A.f[3 * N + id], vy = A.f[4 * N + id], vz = A.f[5 * N + id];
+    const double wx = A.f[6 * N + id], wy = A.f[7 * N + id], wz = A.f[8 * N + id];
+    const double u0 = A.uvw[id], u1 = A.uvw[N + id], u2 = A.uvw[2 * N + id];
+    const double w0 = wy - vz, w1 = uz - wx, w2 = vx - uy;
+    const double q0 = ux * vy + ux * wz + vy * wz - uy * vx - uz * wx - vz * wy;
+    A.f[9 * N + id] = w0;
+    A.f[10 * N + id] = w1;
+    A.f[11 * N + id] = w2;
+    A.f[12 * N + id] = (w0 * w0 + w1 * w1 + w2 * w2) * 0.5;
+    A.f[13 * N + id] = q0;
+    A.f[14 * N + id] = w0 * u0 + w1 * u1 + w2 * u2;
+    A.f[15 * N + id] = 0.5 * A.q[id] * (u0 * u0 + u1 * u1 + u2 * u2);
+}
+
+extern "C" int nuc_q_fields(nuc_ctx *x, double *out)
+{
+    if (!x || !out) return NUC_EINVAL;
+    const long N = x->N;
+    double *buf = nullptr;
+    if (hipMallocAsync(reinterpret_cast<void **>(&buf), 19 * (size_t)N * sizeof(double), x->stream) != hipSuccess) {
+        x->err = "device allocation (Q fields)";
+        return NUC_ENOMEM;
+    }
+    double *uvw = buf, *f = buf + 3 * N;
+    const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
+    SpecPrep P{};
+    P.q = gp((const double *)x->q[x->next_stage]);
+    P.uvw = gp(uvw);
+    P.what = 3;
+    P.N = N;
+    P.gamma = x->c.gamma;
+    hipLaunchKernelGGL(k_spec_prep, grid, blk, 0, x->stream, P);
+    // the gradients with the viscous scheme (solveGrad_xyz, src/postproc.cpp:759-768)
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) {
+            PostDiffArgs D{gp((const double *)(uvw + (size_t)c * N)), gp(f + (size_t)(3 * c + d) * N), N, x->n[0],
+                           x->n[1], d, x->n[d], x->nloc[d], x->tsv[d], vis_id(x->c.vis_scheme[d]), 1.0 / x->h[d]};
+            hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
+        }
+    QArgs Q{gp((const double *)x->q[x->next_stage]), gp((const double *)uvw), gp(f), N};
+    hipLaunchKernelGGL(k_qfields, grid, blk, 0, x->stream, Q);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, f, 16 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, x->stream);
+    (void)hipFreeAsync(buf, x->stream);
+    hipError_t e2 = hipStreamSynchronize(x->stream);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        x->err = "Q fields";
+        return NUC_EHIP;
+    }
+    return NUC_OK;
+}
+
 static int spectra_run(nuc_ctx *x, int what, double *out, int kmax_out)
 {
     const long N = x->N;

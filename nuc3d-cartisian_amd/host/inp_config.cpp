@@ -166,6 +166,7 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
             if (vals.size() > 1) rc.average_start = vals[1];
             if (vals.size() > 2) rc.simultaneous = vals[2];
             if (vals.size() > 3) rc.averaged_save_type = vals[3];
+            if (vals.size() > 4) rc.simultaneous_save = vals[4];
             if (rc.average_type < 0 || rc.average_type > 3) {
                 err = "PostProc.in: no such postproc for " + std::to_string(rc.average_type);
                 return NUC_EINVAL;

@@ -123,7 +123,7 @@ struct RunControl {
     // PostProc.in, six positional values (src/postproc.cpp:36-82): TimeAverageType,
     // TimeAverageStartStep, SimutaneousData, AveragedSaveType, SimutaneousSaveType,
     // fft post type; absent file: no post-processing
-    int average_type = 0, average_start = 0, simultaneous = 0, averaged_save_type = 0;
+    int average_type = 0, average_start = 0, simultaneous = 0, averaged_save_type = 0, simultaneous_save = 0;
     double tl = 0.0;
     double origin[3] = {0, 0, 0};
     double mach_t = 0.3, k_peak = 8.0;

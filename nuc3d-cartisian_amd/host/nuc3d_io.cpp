@@ -12,7 +12,9 @@
 //     Averaged_step_<step>_ID_<rank>.bin (the reference's writer and reader
 //     differ in the case of "id"; the reader here accepts both);
 //   * Tecplot files of the averaged fields, postproc::OutputAveraged_tec
-//     (src/postproc.cpp:523-720), Averaged_tec_step_<averStep>_id_<rank>.dat.
+//     (src/postproc.cpp:523-720), Averaged_tec_step_<averStep>_id_<rank>.dat;
+//   * Tecplot files of solveQ's per-point fields, postproc::OutputTemporal
+//     (src/postproc.cpp:346-405), Q_tec_step_<step>_id_<rank>.dat.
 //
 // Rank numbering: the reference's Cartesian communicator has dims
 // (npz, npy, npx) (src/MPICommunicator.cpp:26-33) and MPI numbers Cartesian
@@ -310,5 +312,35 @@ extern "C" int nuc_average_write_tecplot(nuc_ctx *ctx, const nuc_config *cfg, in
         }
     for (auto &f : files)
         if (!f) return NUC_EINVAL;
+    return NUC_OK;
+}
+
+extern "C" int nuc_write_q_tecplot(nuc_ctx *ctx, const nuc_config *cfg, const char *dir, int step, double time,
+                                   const double *origin)
+{
+    RankGrid g;
+    if (!ctx || !cfg || !dir || !rank_grid(cfg, g)) return NUC_EINVAL;
+    std::vector<double> f((size_t)16 * g.N), loc(g.Nl);
+    int rc = nuc_q_fields(ctx, f.data());
+    if (rc != NUC_OK) return rc;
+    static const char *const names[16] = {"dudx", "dudy", "dudz", "dvdx", "dvdy", "dvdz", "dwdx", "dwdy", "dwdz",
+                                          "omega0", "omega1", "omega2", "enstrophy", "Q", "helicity", "kinetic"};
+    const double o[3] = {origin ? origin[0] : 0.0, origin ? origin[1] : 0.0, origin ? origin[2] : 0.0};
+    const double L[3] = {cfg->lx, cfg->ly, cfg->lz};
+    for (int r = 0; r < g.ranks; r++) {
+        std::ofstream out(file_name(dir, "Q_tec_step_", step, "_id_", r, ".dat"));
+        if (!out) return NUC_EINVAL;
+        out << "title=time_" << time << "\n" << "variables=x,y,z";
+        for (const char *nm : names) out << "," << nm;
+        out << "\n Zone I = " << g.nl[0] + 1 << ", J= " << g.nl[1] + 1 << ", K=" << g.nl[2] + 1
+            << "\n DATAPACKING=BLOCK, VARLOCATION=([4-19]=CELLCENTERED)\n";
+        out << std::setprecision(12);
+        write_nodes(out, g, L, o, r);
+        for (int v = 0; v < 16; v++) {
+            block_copy(g, r, f.data() + (size_t)v * g.N, loc.data(), true);
+            for (long id = 0; id < g.Nl; id++) out << loc[id] << "\n";
+        }
+        if (!out) return NUC_EINVAL;
+    }
     return NUC_OK;
 }

@@ -9,8 +9,7 @@
 // and writing the restart, Tecplot and averaged-statistics files
 // IOController.in selects every saveStep and at the end (src/singleBlock.cpp:50-68,
 // src/block.cpp:238-266), with the averaged Tecplot files when AveragedSaveType
-// is 1.  Not written: postproc's instantaneous Tecplot files
-// (SimutaneousSaveType).
+// is 1 and of solveQ's per-point fields when SimutaneousSaveType is set.
 //
 // usage: nuc3d_run [inp_dir] [--seed S]
 #include "nuc3d_host.hpp"
@@ -77,6 +76,12 @@ int main(int argc, char **argv)
                 nuc_average_write_tecplot(pde.ctx(), &cfg, rc.average_type, "AveragedFlowData", time, rc.origin) != NUC_OK)
                 fprintf(stderr, "cannot write AveragedFlowData/Averaged_tec_step_*_id_*.dat\n");
         }
+        // SimutaneousSaveType: the Q fields (src/postproc.cpp:336-339), here of the
+        // current state (the reference keeps those of its last post step, the
+        // same state when postStep divides saveStep)
+        if (rc.postproc && rc.simultaneous_save != 0 &&
+            nuc_write_q_tecplot(pde.ctx(), &cfg, "flowDataTemp", step, time, rc.origin) != NUC_OK)
+            fprintf(stderr, "cannot write flowDataTemp/Q_tec_step_%d_id_*.dat\n", step);
     };
     // postproc::solvePost (src/postproc.cpp:281-308): the running averages, and
     // solveQ's global statistics appended to enstrophy.dat and turb_para.dat

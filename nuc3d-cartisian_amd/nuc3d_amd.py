@@ -44,7 +44,7 @@ EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_
            "nuc_server_submit_ic", "nuc_spectra_kinetic", "nuc_spectrum_scalar", "nuc_init_default",
            "nuc_init_ivc", "nuc_init_default_device", "nuc_init_ivc_device", "nuc_average_restore",
            "nuc_write_restart", "nuc_read_restart", "nuc_write_tecplot", "nuc_average_write", "nuc_average_read",
-           "nuc_average_write_tecplot"]
+           "nuc_average_write_tecplot", "nuc_q_fields", "nuc_write_q_tecplot"]
 
 
 class NucConfig(ctypes.Structure):
@@ -172,6 +172,9 @@ def lib():
         L.nuc_average_read.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.nuc_average_write_tecplot.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_int, ctypes.c_char_p,
                                                 ctypes.c_double, _dp]
+        L.nuc_q_fields.argtypes = [vp, _dp]
+        L.nuc_write_q_tecplot.argtypes = [vp, ctypes.POINTER(NucConfig), ctypes.c_char_p, ctypes.c_int,
+                                          ctypes.c_double, _dp]
         L.nuc_rk3_stage.argtypes = [vp, ctypes.c_int]
         L.nuc_tile_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.nuc_set_tile_shape.argtypes = [vp, ctypes.c_int]
@@ -344,6 +347,18 @@ class Context:
 
     def step_async(self, n):
         _check(lib().nuc_step_async(self.h, n), "nuc_step_async", self.h)
+
+    def q_fields(self):
+        """(16, N): solveQ's per-point fields of the current state (nuc_q_fields)."""
+        out = np.zeros(16 * self.cfg.nx * self.cfg.ny * self.cfg.nz)
+        _check(lib().nuc_q_fields(self.h, _ptr(out)), "nuc_q_fields", self.h)
+        return out.reshape(16, -1)
+
+    def write_q_tecplot(self, directory, step, time, origin=None):
+        """Per-rank Tecplot files Q_tec_step_<step>_id_<rank>.dat of the 16 Q fields."""
+        o = None if origin is None else np.ascontiguousarray(origin, dtype=np.float64)
+        _check(lib().nuc_write_q_tecplot(self.h, ctypes.byref(self.cfg), str(directory).encode(), int(step),
+                                         float(time), None if o is None else _ptr(o)), "nuc_write_q_tecplot", self.h)
 
     def spectra_kinetic(self):
         """(5, kmax): total, compressive, solenoidal, left, right kinetic energy per shell."""

@@ -22,7 +22,7 @@ def _write_inp(d, start, end, average_start=0):
     os.makedirs(d, exist_ok=True)
     with open(os.path.join(d, "PostProc.in"), "w") as f:
         f.write(f"AverageType 1\nAverageStartStep {average_start}\nSimutaneousData 1\nAveragedSaveType 1\n"
-                "SimutaneousSaveType 0\n")
+                "SimutaneousSaveType 1\n")
     with open(os.path.join(d, "block.in"), "w") as f:
         f.write("1 2 1\n24 24 24\n6.283185307179586 6.283185307179586 6.283185307179586\n")
     with open(os.path.join(d, "Method.in"), "w") as f:
@@ -93,6 +93,11 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
         assert rows.shape == (12, 3) and np.array_equal(rows[:, 0], np.arange(12))
         assert np.allclose(rows[:, 1], e, rtol=1e-11, atol=1e-300), name
         assert np.allclose(rows[:, 2], e / e.sum(), rtol=1e-11, atol=1e-300), name
+    # solveQ's per-point fields as Tecplot (postproc::OutputTemporal) at every save
+    for step in (2, 4):
+        for r in range(2):
+            head = open(temp / f"Q_tec_step_{step}_id_{r}.dat").readline()
+            assert head.startswith("title=time_")
     # the averaged file at step 4: [int averStep][10 primaries][21 gradients] per rank block
     raw = np.fromfile(aver / "Averaged_step_4_id_1.bin", dtype=np.uint8)
     assert int(raw[:4].view(np.int32)[0]) == 4
