@@ -18,6 +18,11 @@ traffic of 106.7 B per cell (320 B per cell per RK3 step: read u^n and u^s,
 write u^{s+1}) split evenly over the three direction sweeps that make up a
 stage: 35.6 B per cell per sweep launch.
 
+The sweeps are bound by FP64 VALU issue rather than HBM; roofline also
+reports valu_issue_frac: the kernel's VALU wave-instructions per launch (from
+the committed SQ_INSTS_VALU summary) at one per 4 cycles per SIMD, over its
+measured launch time.
+
 cpu_baseline: the oracle (a C restatement of the reference algorithm) timed on
 this host's cores on one RHS evaluation of the same 256^3 workload.
 
@@ -86,6 +91,23 @@ def measured_traffic(kernel, n):
             rec = json.loads(line[len("machine-readable: "):])
             if rec.get("kernel") == kernel:
                 return float(rec["traffic_bytes"]), os.path.relpath(path, ROOT)
+    return None, None
+
+
+VALU_WAVE_INSTS_PER_S = 256 * 4 * 2.4e9 / 4   # one 64-lane FP64 VALU wave-instruction per 4 cycles per SIMD
+
+
+def measured_valu(kernel, n):
+    """VALU wave-instructions per launch of `kernel` at n^3 from the committed
+    rocprofv3 --pmc SQ_INSTS_VALU summary, or None."""
+    path = os.path.join(ROOT, "profiles", "k_sweep_crweno5_256_fp64_valu.md")
+    if n != 256 or not os.path.exists(path):
+        return None, None
+    for line in open(path):
+        if line.startswith("machine-readable: "):
+            rec = json.loads(line[len("machine-readable: "):])
+            if rec.get("kernel") == kernel:
+                return float(rec["valu_wave_insts"]), os.path.relpath(path, ROOT)
     return None, None
 
 
@@ -328,6 +350,11 @@ def main():
     if traffic:
         roofline["traffic_source"] = traffic_src
         roofline["traffic_gbs"] = round(traffic / (avg_ms * 1e-3) / 1e9, 2)
+    # the sweeps' binding resource: FP64 VALU issue against its floor
+    valu, valu_src = measured_valu(name, n)
+    if valu:
+        roofline["valu_issue_frac"] = round(valu / VALU_WAVE_INSTS_PER_S / (avg_ms * 1e-3), 4)
+        roofline["valu_source"] = valu_src
     stage_ms = ms_per_step / 3.0
     if args.profile_slots:
         for k, (t, c) in prof.items():
