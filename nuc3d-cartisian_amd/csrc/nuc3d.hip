@@ -75,7 +75,8 @@ struct SweepArgs {
     int fv_f32;
     double GMEM *rhs[5];
     int stage, epilogue;
-    int accum;                // x sweep adds to rhs (forcing initialised it)
+    int first;                // the stage's first sweep: writes rhs without reading it
+    int accum;                // the first sweep adds to rhs (forcing initialised it)
     int mixed;                // FP32 stencils (host-side dispatch only)
     const double GMEM *fmean; // z sweep: <f.u> added back to the energy RHS
     const double GMEM *qn[5];
@@ -83,6 +84,11 @@ struct SweepArgs {
     const double GMEM *dtp;
     double GMEM *res_part;
 };
+
+// Sweep order within a stage: z, y, then x.  The last sweep applies the RK3
+// update (reads u^n and the accumulated rhs, writes the new state), and x is
+// the direction whose loads are contiguous.
+constexpr int kSweepOrder[3] = {2, 1, 0};
 
 struct PrimArgs {
     const double GMEM *q[5];
@@ -575,7 +581,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             unv[k] = 0.0;
             if (FULL || (lv && c < n)) {
                 long id = base + (long)c * stride;
-                if (d > 0 || A.accum) rin[k] = A.rhs[v][id];
+                if (!A.first || A.accum) rin[k] = A.rhs[v][id];
                 if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
             }
         }
@@ -1238,8 +1244,10 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     }
     if (!dalloc(&x->lam, 3 * MAXRANK * sizeof(unsigned long long))) return fail("device allocation (lam)");
     if (!dalloc(&x->scal, 8 * sizeof(double))) return fail("device allocation (scal)");
-    long lines_z = x->N / x->n[2];
-    x->res_nb = (lines_z + sweep_lpb(x->cpt[2]) - 1) / sweep_lpb(x->cpt[2]);
+    // residual partials: one per workgroup of the last sweep
+    const int dl = kSweepOrder[2];
+    long lines_last = x->N / x->n[dl];
+    x->res_nb = (lines_last + sweep_lpb(x->cpt[dl]) - 1) / sweep_lpb(x->cpt[dl]);
     if (!dalloc(&x->res_part, x->res_nb * sizeof(double))) return fail("device allocation (residual)");
     (void)hipMemsetAsync(x->lam, 0, 3 * MAXRANK * sizeof(unsigned long long), x->stream);
     (void)hipMemsetAsync(x->scal, 0, 8 * sizeof(double), x->stream);
@@ -1356,11 +1364,13 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         for (int v = 0; v < 5; v++) { S.q[v] = gp(qi + v * N); S.rhs[v] = gp(x->rhs + v * N); }
         if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(x->fv + (long)fvmap[d][o] * N);
         S.stage = stage;
-        S.accum = (d == 0 && (x->forcing || x->sources)) ? 1 : 0;
+        const bool first = (d == kSweepOrder[0]), last = (d == kSweepOrder[2]);
+        S.first = first ? 1 : 0;
+        S.accum = (first && (x->forcing || x->sources)) ? 1 : 0;
         S.mixed = x->mixed ? 1 : 0;
         S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
-        S.fmean = (d == 2 && x->forcing) ? gp(x->scal + 4) : nullptr;
-        S.epilogue = (d == 2 && epilogue) ? 1 : 0;
+        S.fmean = (last && x->forcing) ? gp(x->scal + 4) : nullptr;
+        S.epilogue = (last && epilogue) ? 1 : 0;
         if (S.epilogue) {
             for (int v = 0; v < 5; v++) { S.qn[v] = gp(x->q[0] + v * N); S.qout[v] = gp(x->q[outb] + v * N); }
             S.dtp = gp(x->scal);
@@ -1464,7 +1474,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     }
     if (x->forcing) {
         // Const forcing (src/physicsModel.cpp:1368-1677) initialises rhs; the
-        // x sweep accumulates onto it and the z sweep adds <f.u> back
+        // first sweep accumulates onto it and the last adds <f.u> back
         double *r[5];
         for (int v = 0; v < 5; v++) r[v] = x->rhs + v * N;
         ProfScope ps(x, 7);
@@ -1472,12 +1482,13 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     }
     if (x->sources) {
         // ABC/TG forcing and the rotating-frame sources initialise rhs (or add
-        // to the Const forcing's initialisation); the x sweep accumulates onto it
+        // to the Const forcing's initialisation); the first sweep accumulates onto it
         SourceArgs SA = source_args(x, in, x->forcing);
         ProfScope ps(x, 7);
         hipLaunchKernelGGL(k_source, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, x->stream, SA);
     }
-    for (int d = 0; d < 3; d++) {
+    for (int i = 0; i < 3; i++) {
+        const int d = kSweepOrder[i];
         ProfScope ps(x, 3 + d);
         launch_sweep(x->c.scheme[d], x->cpt[d], sa.sw[d], x->stream);
     }
@@ -3121,7 +3132,8 @@ static void pack_enqueue_step(nuc_pack *p)
                 hipLaunchKernelGGL(k_gradq_pack<double>, dim3((unsigned)p->grad_total), dim3(GTX, GTY, 1), 0, p->stream,
                                    p->grad[st], p->grad_off, nj);
         }
-        for (int d = 0; d < 3; d++) {
+        for (int i = 0; i < 3; i++) {
+            const int d = kSweepOrder[i];
             PackLaunch L{p->sw[st][d], p->sw_off[d], nj, p->sw_total[d], p->mixed, p->full[d]};
             launch_sweep_pack(p->scheme[d], p->riemann, p->cpt[d], L, p->stream);
         }
