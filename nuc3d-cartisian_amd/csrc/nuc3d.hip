@@ -13,8 +13,9 @@
 //              f^- (explicit WENO or compact tridiagonal solved per wave),
 //              differences the interface flux and the viscous flux and
 //              accumulates the RHS.  The periodic halo of MPIComunicator is an
-//              index wrap inside the line buffer.  The z sweep ends with the
-//              TVD-RK3 update (src/TVD-RK3.cpp:37-157) and the residual.
+//              index wrap inside the line buffer.  The sweeps run z, y, x;
+//              the x sweep ends with the TVD-RK3 update
+//              (src/TVD-RK3.cpp:37-157) and the residual.
 #include "nuc3d_gpu.h"
 #include "nuc_device.h"
 #include "nuc_lines.h"
@@ -78,7 +79,7 @@ struct SweepArgs {
     int first;                // the stage's first sweep: writes rhs without reading it
     int accum;                // the first sweep adds to rhs (forcing initialised it)
     int mixed;                // FP32 stencils (host-side dispatch only)
-    const double GMEM *fmean; // z sweep: <f.u> added back to the energy RHS
+    const double GMEM *fmean; // last sweep: <f.u> added back to the energy RHS
     const double GMEM *qn[5];
     double GMEM *qout[5];
     const double GMEM *dtp;
@@ -1294,7 +1295,7 @@ extern "C" int nuc_download_state(nuc_ctx *x, double *q)
 }
 
 // Arguments of every launch of one RHS evaluation of state buffer `in`; with
-// epilogue the z sweep performs RK stage `stage` from `in` (and q[0] = u^n)
+// epilogue the last sweep performs RK stage `stage` from `in` (and q[0] = u^n)
 // into `outb`.  The single-context launches and the packed launches are built
 // from the same arguments.
 struct StageArgs {

@@ -42,7 +42,7 @@ void plan_destroy(Plan &p, hipStream_t s);
 long long plan_bytes(const nuc_config &c);
 
 // rhs[0] = 0, rhs[1..3] = -f, rhs[4] = -(f.u); mean_out = <f.u> (added back to
-// rhs[4] by the z sweep, src/physicsModel.cpp:1605-1667)
+// rhs[4] by the stage's last sweep, src/physicsModel.cpp:1605-1667)
 void enqueue(const Plan &p, const double *u, const double *v, const double *w, double *const rhs[5],
              double *mean_out, hipStream_t s);
 

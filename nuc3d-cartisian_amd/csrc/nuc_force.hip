@@ -8,7 +8,7 @@
 //   -> 1 substitution (:1483-1484,1534-1557), theta (:1559)          k_fband
 //   forcing spectrum (theta-1) v_s on the shell, unnormalised
 //   backward FFT, real part (:1560-1603)                              k_fg, k_fapply
-//   <f.u> correction and RHS update, rho taken as 1 (:1605-1667)     k_fapply, k_fmean, z sweep
+//   <f.u> correction and RHS update, rho taken as 1 (:1605-1667)     k_fapply, k_fmean, last sweep
 #include "nuc_force.h"
 
 #include <algorithm>
