@@ -92,6 +92,21 @@ def test_inp_missing_directory_is_an_error():
         G.config_from_inp(os.path.join(GOLD, "does_not_exist"))
 
 
+def test_inp_postproc_average_type_is_checked(tmp_path):
+    """PostProc.in's first value is TimeAverageType 0..3; any other is the
+    reference's "No such postproc" error (src/postproc.cpp:85-154).  Without
+    the file no post-processing is configured."""
+    import shutil
+    d = tmp_path / "inp"
+    shutil.copytree(os.path.join(GOLD, "inp_tgv64"), d)
+    G.config_from_inp(str(d))                                   # no PostProc.in
+    (d / "PostProc.in").write_text("AverageType 3\nAverageStartStep 0\nSimutaneousData 1\n")
+    G.config_from_inp(str(d))
+    (d / "PostProc.in").write_text("AverageType 4\nAverageStartStep 0\n")
+    with pytest.raises(G.NucError):
+        G.config_from_inp(str(d))
+
+
 def test_footprint_scales_with_cells_and_options():
     a = G.footprint(G.make_config(n=(64, 64, 64)))
     b = G.footprint(G.make_config(n=(128, 64, 64)))
