@@ -63,6 +63,10 @@ typedef struct nuc_config {
      * centre and radius of the centrifugal well */
     int coriolis, centrifugal;
     double rossby, x_center, y_center, well_rad;
+    /* x/y/z_origin of inp/block.in line 4 (src/block.cpp:61): subtracted from
+     * every cell centre (src/block.cpp:116-118), so it shifts the analytic
+     * forcings, the centrifugal well and the analytic initial conditions */
+    double origin[3];
 } nuc_config;
 
 typedef struct nuc_step_info {
@@ -80,6 +84,8 @@ typedef struct nuc_ctx nuc_ctx;
  * fields and the halo buffers.  All device memory is owned by the context. */
 int nuc_create(const nuc_config *cfg, nuc_ctx **out);
 int nuc_destroy(nuc_ctx *ctx);
+/* the message of the last failure on ctx; with ctx NULL, the message of the
+ * last failed nuc_create on the calling thread ("" when it succeeded) */
 const char *nuc_last_error(const nuc_ctx *ctx);
 /* bytes of device memory a context of this configuration allocates */
 long long nuc_footprint(const nuc_config *cfg);

@@ -1184,18 +1184,25 @@ extern "C" long long nuc_footprint(const nuc_config *c)
     return doubles * 8 + (N / 32 + 4096) * 8 + 4096 + fb;
 }
 
+// message of the last failed nuc_create on this thread (nuc_last_error(NULL))
+static thread_local std::string g_create_err;
+
 extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
 {
-    if (!cfg || !out) return NUC_EINVAL;
+    g_create_err.clear();
+    if (!cfg || !out) { g_create_err = "nuc_create: null config or output pointer"; return NUC_EINVAL; }
     *out = nullptr;
     std::string err;
     int rc = validate(cfg, err);
     if (rc != NUC_OK) {
-        fprintf(stderr, "nuc_create: %s\n", err.c_str());
+        g_create_err = "nuc_create: " + err;
         return rc;
     }
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return NUC_ENODEV;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_err = "nuc_create: no usable gfx950 device";
+        return NUC_ENODEV;
+    }
     nuc_ctx *x = new nuc_ctx();
     x->c = *cfg;
     x->n[0] = cfg->nx; x->n[1] = cfg->ny; x->n[2] = cfg->nz;
@@ -1212,14 +1219,18 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     }
     const size_t N = (size_t)x->N;
     auto fail = [&](const char *what) {
-        x->err = what;
+        g_create_err = std::string("nuc_create: ") + what;
         nuc_destroy(x);
         return NUC_ENOMEM;
     };
     // a non-blocking stream and stream-ordered allocations: contexts created
     // and destroyed while others run (the serving loop) neither wait for nor
     // stall the other contexts' streams
-    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) { delete x; return NUC_EHIP; }
+    if (hipStreamCreateWithFlags(&x->stream, hipStreamNonBlocking) != hipSuccess) {
+        g_create_err = "nuc_create: hipStreamCreateWithFlags failed";
+        delete x;
+        return NUC_EHIP;
+    }
     auto dalloc = [&](auto **ptr, size_t bytes) {
         return hipMallocAsync(reinterpret_cast<void **>(ptr), bytes, x->stream) == hipSuccess;
     };
@@ -1237,7 +1248,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
         std::string ferr;
         int frc = nucforce::plan_create(x->fplan, *cfg, x->stream, ferr);
         if (frc != NUC_OK) {
-            fprintf(stderr, "nuc_create: %s\n", ferr.c_str());
+            g_create_err = "nuc_create: " + ferr;
             x->forcing = false;
             nuc_destroy(x);
             return frc;
@@ -1275,7 +1286,7 @@ extern "C" int nuc_destroy(nuc_ctx *x)
     return NUC_OK;
 }
 
-extern "C" const char *nuc_last_error(const nuc_ctx *x) { return x ? x->err.c_str() : "null context"; }
+extern "C" const char *nuc_last_error(const nuc_ctx *x) { return x ? x->err.c_str() : g_create_err.c_str(); }
 
 extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
 {
@@ -1393,6 +1404,7 @@ struct SourceArgs {
     long N;
     int nx, ny, nl[3], forcing, coriolis, centrifugal, accumulate;
     double dx, dy, dz, lxr, lyr, lzr, A, kf, Ro, xc, yc, r0;
+    double ox, oy, oz;   // block.in origin, subtracted from the centres
 };
 
 __global__ __launch_bounds__(256) void k_source(SourceArgs A)
@@ -1402,9 +1414,9 @@ __global__ __launch_bounds__(256) void k_source(SourceArgs A)
     const int i = (int)(id % A.nx);
     const long rr = id / A.nx;
     const int j = (int)(rr % A.ny), k = (int)(rr / A.ny);
-    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]);
-    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]);
-    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]);
+    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]) - A.ox;
+    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]) - A.oy;
+    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]) - A.oz;
     const double rho = A.q[id];
     const double u0 = A.q[A.N + id] / rho, v0 = A.q[2 * A.N + id] / rho, w0 = A.q[3 * A.N + id] / rho;
     double r1 = 0.0, r2 = 0.0, r3 = 0.0, r4 = 0.0;
@@ -1452,6 +1464,7 @@ static SourceArgs source_args(nuc_ctx *x, int in, bool accumulate)
     A.lxr = c.lx / c.npx; A.lyr = c.ly / c.npy; A.lzr = c.lz / c.npz;
     A.A = c.forcing_amp; A.kf = c.forcing_k;
     A.Ro = c.rossby; A.xc = c.x_center; A.yc = c.y_center; A.r0 = c.well_rad;
+    A.ox = c.origin[0]; A.oy = c.origin[1]; A.oz = c.origin[2];
     return A;
 }
 
@@ -2492,6 +2505,7 @@ struct AnalyticIcArgs {
     int nx, ny, nz, nl[3], kind;
     long N;
     double dx, dy, dz, lxr, lyr, lzr, gamma, mach;   // cell sizes, rank box sizes
+    double ox, oy, oz;                               // block.in origin
 };
 
 // initial_default (src/block.cpp:285-314): at rest, p = 1/(gamma M^2)
@@ -2528,9 +2542,9 @@ __global__ __launch_bounds__(256) void k_ic_analytic(AnalyticIcArgs A)
     const long r = id / A.nx;
     const int j = (int)(r % A.ny), k = (int)(r / A.ny);
     // cell centres per rank, src/block.cpp:116-118
-    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]);
-    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]);
-    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]);
+    const double x0 = A.lxr * (i / A.nl[0]) + A.dx * (0.5 + i % A.nl[0]) - A.ox;
+    const double y0 = A.lyr * (j / A.nl[1]) + A.dy * (0.5 + j % A.nl[1]) - A.oy;
+    const double z0 = A.lzr * (k / A.nl[2]) + A.dz * (0.5 + k % A.nl[2]) - A.oz;
     if (A.kind == IC_DEFAULT) { ic_default(A, id); return; }
     if (A.kind == IC_IVC) { ic_ivc(A, id, x0, y0); return; }
     const double g = A.gamma, m = A.mach;
@@ -2557,6 +2571,7 @@ static int init_analytic(nuc_ctx *x, int kind)
     A.dx = c.lx / c.nx; A.dy = c.ly / c.ny; A.dz = c.lz / c.nz;
     A.lxr = c.lx / c.npx; A.lyr = c.ly / c.npy; A.lzr = c.lz / c.npz;
     A.gamma = c.gamma; A.mach = c.mach;
+    A.ox = c.origin[0]; A.oy = c.origin[1]; A.oz = c.origin[2];
     x->next_stage = 0;
     hipLaunchKernelGGL(k_ic_analytic, dim3((unsigned)((x->N + 255) / 256)), dim3(256), 0, x->stream, A);
     NUC_CHECK(x, hipGetLastError());

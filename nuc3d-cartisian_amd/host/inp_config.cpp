@@ -57,6 +57,7 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
         rc.origin[0] = f ? x0 : 0.0;
         rc.origin[1] = f ? y0 : 0.0;
         rc.origin[2] = f ? z0 : 0.0;
+        for (int d = 0; d < 3; d++) c.origin[d] = rc.origin[d];
     }
     // ---- Method.in (defaults src/fieldOperator.cpp:22-33)
     std::map<std::string, std::string> method = {

@@ -408,8 +408,7 @@ extern "C" int nuc_init_tgv(const nuc_config *cfg, double *q)
 {
     if (!cfg || !q) return NUC_EINVAL;
     std::vector<double> v;
-    double o[3] = {0, 0, 0};
-    nuc3d::initial_taylorgreen(*cfg, o, v);
+    nuc3d::initial_taylorgreen(*cfg, cfg->origin, v);
     std::memcpy(q, v.data(), v.size() * sizeof(double));
     return NUC_OK;
 }
@@ -428,8 +427,7 @@ extern "C" int nuc_init_ivc(const nuc_config *cfg, double *q)
     if (!cfg || !q) return NUC_EINVAL;
     if (cfg->npx < 1 || cfg->npy < 1 || cfg->nx % cfg->npx != 0 || cfg->ny % cfg->npy != 0) return NUC_EINVAL;
     std::vector<double> v;
-    double o[3] = {0, 0, 0};
-    nuc3d::initial_ivc(*cfg, o, v);
+    nuc3d::initial_ivc(*cfg, cfg->origin, v);
     std::memcpy(q, v.data(), v.size() * sizeof(double));
     return NUC_OK;
 }

@@ -440,8 +440,12 @@ extern "C" double nuc_server_now(nuc_server *s) { return s ? s->now() : 0.0; }
 
 extern "C" const char *nuc_server_job_error(nuc_server *s, long long id)
 {
+    // copied under the lock into this thread's buffer: a concurrent
+    // nuc_server_result may free the job right after the lock is released
+    static thread_local std::string msg;
     if (!s) return "null server";
     std::lock_guard<std::mutex> lk(s->mu);
     auto it = s->jobs.find(id);
-    return it == s->jobs.end() ? "unknown job" : it->second->err.c_str();
+    msg = (it == s->jobs.end()) ? std::string("unknown job") : it->second->err;
+    return msg.c_str();
 }

@@ -61,7 +61,7 @@ class NucConfig(ctypes.Structure):
         ("forcing_k", ctypes.c_double), ("cfl", ctypes.c_double),
         ("coriolis", ctypes.c_int), ("centrifugal", ctypes.c_int),
         ("rossby", ctypes.c_double), ("x_center", ctypes.c_double), ("y_center", ctypes.c_double),
-        ("well_rad", ctypes.c_double),
+        ("well_rad", ctypes.c_double), ("origin", ctypes.c_double * 3),
     ]
 
 
@@ -207,7 +207,7 @@ def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, 
                 model=NAVIERSTOKES3D, viscosity=SUTHERLAND, forcing=FORCING_NONE, precision=FP64,
                 gamma=1.4, mach=0.1, reynolds=1600.0, prandtl=0.72, t_ref=110.4, t_inf=288.0,
                 forcing_amp=0.01, forcing_amp_hel=0.01, forcing_k=1.0, cfl=0.5, coriolis=False, centrifugal=False,
-                rossby=0.0, x_center=3.1415926, y_center=3.1415926, well_rad=1.0):
+                rossby=0.0, x_center=3.1415926, y_center=3.1415926, well_rad=1.0, origin=(0.0, 0.0, 0.0)):
     c = NucConfig()
     c.nx, c.ny, c.nz = n
     c.npx, c.npy, c.npz = npr
@@ -226,6 +226,8 @@ def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, 
     c.cfl = cfl
     c.coriolis, c.centrifugal = int(bool(coriolis)), int(bool(centrifugal))
     c.rossby, c.x_center, c.y_center, c.well_rad = rossby, x_center, y_center, well_rad
+    for d in range(3):
+        c.origin[d] = origin[d]
     return c
 
 
@@ -307,7 +309,9 @@ class Context:
         self.cfg = cfg
         self.N = cfg.nx * cfg.ny * cfg.nz
         h = ctypes.c_void_p()
-        _check(lib().nuc_create(ctypes.byref(cfg), ctypes.byref(h)), "nuc_create")
+        rc = lib().nuc_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != 0:
+            raise NucError(rc, lib().nuc_last_error(None).decode())
         self.h = h
 
     def close(self):

@@ -15,6 +15,11 @@
  * copied once into a periodically extended buffer; the segment of rank r is
  * then exactly the rank-local line with its ghost cells, and the reference's
  * per-rank inner/boundary code is applied to it.
+ *
+ * ORC_PLANT_* macros: compiled in only by the tests that plant a known
+ * mistake (a dropped term, a swapped factor) into a throwaway copy of this
+ * library, to show that the closed-form pins of those functions fail on it.
+ * The normal build defines none of them.
  */
 #include "nuc_oracle.h"
 #include <math.h>
@@ -250,19 +255,29 @@ static void hccs_row(const double *f, double *lo, double *di, double *hi, double
     double omega1 = alpha1 / alphaSum;
     double omega2 = alpha2 / alphaSum;
     double theta = 1.0 / (1.0 + pow(alphaSum - 1.0, 2));
+    double blend = (1.0 - theta) / theta;
+#if defined(ORC_PLANT_HCCS_SWAP)
+    blend = theta / (1.0 - theta);
+#elif defined(ORC_PLANT_HCCS_INV)
+    blend = 1.0 / theta;
+#endif
     if (theta > 0.5) {
         *lo = 2.0 / 7.0;
         *di = 4.0 / 7.0;
         *hi = 1.0 / 7.0;
         *rhs = q7;
     } else {
-        *lo = (2.0 / 3.0 * omega0 + 1.0 / 3.0 * omega1) * (1.0 - theta) / theta + CUC[0];
-        *di = (1.0 / 3.0 * omega0 + 2.0 / 3.0 * (omega1 + omega2)) * (1.0 - theta) / theta + CUC[1];
-        *hi = (1.0 / 3.0 * omega2) * (1.0 - theta) / theta + CUC[2];
+        *lo = (2.0 / 3.0 * omega0 + 1.0 / 3.0 * omega1) * blend + CUC[0];
+        *di = (1.0 / 3.0 * omega0 + 2.0 / 3.0 * (omega1 + omega2)) * blend + CUC[1];
+        *hi = (1.0 / 3.0 * omega2) * blend + CUC[2];
         *rhs = (1.0 / 6.0 * omega0 * f[1] + (5.0 * (omega0 + omega1) + omega2) * f[2] / 6.0
-                + (omega1 + 5.0 * omega2) * f[3] / 6.0) * (1.0 - theta) / theta + q7;
+                + (omega1 + 5.0 * omega2) * f[3] / 6.0) * blend + q7;
     }
 }
+
+/* One interior hccs row from its 5-point upwind stencil f[0..4] (cells i-2..i+2
+ * of src/hccs.cpp:138-183): out = {lo, di, hi, rhs}. */
+void orc_hccs_row(const double *f, double *out) { hccs_row(f, out, out + 1, out + 2, out + 3); }
 
 /* gather a stencil z = -r .. r+extra for interface m in upwind coordinates */
 static inline void gather_ext(const double *f, int m, int upwind, int r, int extra, double *st)
@@ -920,9 +935,10 @@ static void add_sources(orc_solver *s)
         for (int j = 0; j < s->n[1]; j++)
             for (int i = 0; i < s->n[0]; i++) {
                 const long id = gidx(s, i, j, k);
-                const double x0 = c->lx / c->npx * (i / s->nloc[0]) + dx * (0.5 + i % s->nloc[0]);
-                const double y0 = c->ly / c->npy * (j / s->nloc[1]) + dy * (0.5 + j % s->nloc[1]);
-                const double z0 = c->lz / c->npz * (k / s->nloc[2]) + dz * (0.5 + k % s->nloc[2]);
+                /* cell centres per rank minus the origin, src/block.cpp:116-118 */
+                const double x0 = c->lx / c->npx * (i / s->nloc[0]) + dx * (0.5 + i % s->nloc[0]) - c->origin[0];
+                const double y0 = c->ly / c->npy * (j / s->nloc[1]) + dy * (0.5 + j % s->nloc[1]) - c->origin[1];
+                const double z0 = c->lz / c->npz * (k / s->nloc[2]) + dz * (0.5 + k % s->nloc[2]) - c->origin[2];
                 const double rho = s->prim[id], u0 = s->prim[N + id], v0 = s->prim[2 * N + id], w0 = s->prim[3 * N + id];
                 if (c->forcing == ORC_FORCE_ABC) {
                     const double rho0 = 1.0;
@@ -976,11 +992,17 @@ void orc_rhs(orc_solver *s, double *rhs_out, double *dt_local)
             double grad = ux + vy + wz;
             double tau_xx = miu0 * (2.0 * ux - 2.0 / 3.0 * grad);
             double tau_xy = miu0 * (uy + vx - 2.0 / 3.0 * grad);
+#ifdef ORC_PLANT_TAUXY
+            tau_xy = miu0 * (uy + vx);
+#endif
             double tau_xz = miu0 * (uz + wx - 2.0 / 3.0 * grad);
             double tau_yy = miu0 * (2.0 * vy - 2.0 / 3.0 * grad);
             double tau_yz = miu0 * (vz + wy - 2.0 / 3.0 * grad);
             double tau_zz = miu0 * (2.0 * wz - 2.0 / 3.0 * grad);
             double tau_tx = coeff0 * tx, tau_ty = coeff0 * ty, tau_tz = coeff0 * tz;
+#ifdef ORC_PLANT_HEAT
+            tau_ty = coeff0 * tz;
+#endif
             double u = s->prim[1 * N + id], v = s->prim[2 * N + id], w = s->prim[3 * N + id];
             double *F = s->fv;
             F[0 * N + id] = 0.0; F[1 * N + id] = tau_xx; F[2 * N + id] = tau_xy; F[3 * N + id] = tau_xz;
@@ -989,6 +1011,9 @@ void orc_rhs(orc_solver *s, double *rhs_out, double *dt_local)
             F[9 * N + id] = u * tau_xy + v * tau_yy + w * tau_yz + tau_ty;
             F[10 * N + id] = 0.0; F[11 * N + id] = tau_xz; F[12 * N + id] = tau_yz; F[13 * N + id] = tau_zz;
             F[14 * N + id] = u * tau_xz + v * tau_yz + w * tau_zz + tau_tz;
+#ifdef ORC_PLANT_EFLUX
+            F[4 * N + id] = u * tau_xx + v * tau_xy + tau_tx;
+#endif
         }
         for (int d = 0; d < 3; d++)
             for (int v = 0; v < 5; v++)
@@ -1071,12 +1096,12 @@ void orc_init_tgv(const orc_config *c, double *q)
     for (int k = 0; k < c->nz; k++)
         for (int j = 0; j < c->ny; j++)
             for (int i = 0; i < c->nx; i++) {
-                /* cell centres per rank, src/block.cpp:116-118 (origin 0) */
+                /* cell centres per rank, src/block.cpp:116-118 */
                 int cx = i / nl[0], cy = j / nl[1], cz = k / nl[2];
                 int li = i % nl[0], lj = j % nl[1], lk = k % nl[2];
-                double x0 = c->lx / c->npx * cx + dx * (0.5 + li) - 0.0;
-                double y0 = c->ly / c->npy * cy + dy * (0.5 + lj) - 0.0;
-                double z0 = c->lz / c->npz * cz + dz * (0.5 + lk) - 0.0;
+                double x0 = c->lx / c->npx * cx + dx * (0.5 + li) - c->origin[0];
+                double y0 = c->ly / c->npy * cy + dy * (0.5 + lj) - c->origin[1];
+                double z0 = c->lz / c->npz * cz + dz * (0.5 + lk) - c->origin[2];
                 double gamma = c->gamma, mach = c->mach;
                 /* src/block.cpp:377-391 */
                 double rho0 = 1.0;
@@ -1123,8 +1148,8 @@ void orc_init_ivc(const orc_config *c, double *q)
     for (int k = 0; k < c->nz; k++)
         for (int j = 0; j < c->ny; j++)
             for (int i = 0; i < c->nx; i++) {
-                double x0 = c->lx / c->npx * (i / nl[0]) + dx * (0.5 + i % nl[0]);
-                double y0 = c->ly / c->npy * (j / nl[1]) + dy * (0.5 + j % nl[1]);
+                double x0 = c->lx / c->npx * (i / nl[0]) + dx * (0.5 + i % nl[0]) - c->origin[0];
+                double y0 = c->ly / c->npy * (j / nl[1]) + dy * (0.5 + j % nl[1]) - c->origin[1];
                 double r = sqrt(pow(x0 - x_c, 2) + pow(y0 - y_c, 2));
                 double rho0 = pow((1 - (gamma - 1.0) * b * b * exp(1 - r * r) / (8.0 * gamma * pie * pie)), 2.5);
                 double u0 = 0.5 - b / (2.0 * pie) * exp((1 - r * r) / 2) * (y0 - y_c);
@@ -1175,7 +1200,13 @@ void orc_stats(orc_solver *s, double *out)
     for (int t = 0; t < 16; t++) out[t] = m[t] / Nd;
     u_rms_aver /= Nd; div_aver /= Nd; miu_aver /= Nd; rho_bar /= Nd; Skew3 /= Nd; Skew2 /= Nd; alpha_aver /= Nd;
     double skew = Skew3 / pow(Skew2, 1.5);
+#ifdef ORC_PLANT_SKEW
+    skew = Skew3 / pow(Skew2, 2.0);
+#endif
     double lambda = sqrt(u_rms_aver / div_aver);
+#ifdef ORC_PLANT_LAMBDA
+    lambda = u_rms_aver / div_aver;
+#endif
     double Re_lambda = sqrt(u_rms_aver) * lambda * rho_bar / miu_aver * s->c.reynolds;
     double tau_t = lambda / sqrt(u_rms_aver);
     out[16] = sqrt(u_rms_aver);

@@ -40,6 +40,7 @@ typedef struct {
     int nthreads;            /* OpenMP threads for line loops (CPU baseline) */
     int coriolis, centrifugal;                        /* source switches */
     double rossby, x_center, y_center, well_rad;      /* PhysModel.in */
+    double origin[3];                                 /* block.in x/y/z_origin (src/block.cpp:61) */
 } orc_config;
 
 /* --- operator level: one line of n cells, f valid on [-bf, n+bf) --------- */
@@ -59,6 +60,10 @@ void orc_field_reconstruct(int scheme, int upwind, int ts, const double *in, int
 void orc_field_diff(int scheme, int ts, const double *in, double h, int n1, int n2, int n3,
                     const double *gL, const double *gR, int bf, double *out);
 void orc_rk3(int stage, const double *rhs, const double *un, double *ui, double dt, long n);
+
+/* One interior row of hccs (src/hccs.cpp:138-183) from the 5-point upwind
+ * stencil f[0..4]: out = {lo, di, hi, rhs} (sub-, main, super-diagonal, rhs). */
+void orc_hccs_row(const double *f, double *out);
 
 /* --- pointwise physics (src/physicsModel.cpp) ----------------------------- */
 void orc_con2prim(const double *q, double gamma, double mach, double *prim /*rho,u,v,w,p,T,e,c*/);

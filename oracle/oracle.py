@@ -36,7 +36,7 @@ class OrcConfig(ctypes.Structure):
         ("forcing_k", ctypes.c_double), ("cfl", ctypes.c_double), ("nthreads", ctypes.c_int),
         ("coriolis", ctypes.c_int), ("centrifugal", ctypes.c_int),
         ("rossby", ctypes.c_double), ("x_center", ctypes.c_double), ("y_center", ctypes.c_double),
-        ("well_rad", ctypes.c_double),
+        ("well_rad", ctypes.c_double), ("origin", ctypes.c_double * 3),
     ]
 
 
@@ -52,7 +52,8 @@ _ref = None
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # NUC_ORACLE_LIB: a throwaway build with a planted mistake (tests only)
+        path = os.environ.get("NUC_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError("oracle/liboracle.so is not built (run: make -f oracle/Makefile oracle)")
         L = ctypes.CDLL(path)
@@ -78,6 +79,7 @@ def lib():
         L.orc_init_default.argtypes = [ctypes.POINTER(OrcConfig), _dp]
         L.orc_init_ivc.argtypes = [ctypes.POINTER(OrcConfig), _dp]
         L.orc_stats.argtypes = [ctypes.c_void_p, _dp]
+        L.orc_hccs_row.argtypes = [_dp, _dp]
         L.orc_post_create.restype = ctypes.c_void_p
         L.orc_post_create.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.orc_post_destroy.argtypes = [ctypes.c_void_p]
@@ -169,6 +171,14 @@ def rk3(stage, rhs, un, ui, dt, use_ref=False):
     return ui
 
 
+def hccs_row(f5):
+    """(lo, di, hi, rhs) of one interior hccs row from its 5-point stencil."""
+    f5 = np.ascontiguousarray(f5, dtype=np.float64)
+    out = np.zeros(4)
+    lib().orc_hccs_row(_ptr(f5), _ptr(out))
+    return out
+
+
 def con2prim(q, gamma, mach):
     q = np.ascontiguousarray(q, dtype=np.float64)
     w = np.zeros(8)
@@ -196,7 +206,7 @@ def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, 
                 viscosity=SUTHERLAND, forcing=FORCE_NONE, gamma=1.4, mach=0.1, reynolds=1600.0,
                 prandtl=0.72, t_ref=110.4, t_inf=288.0, forcing_amp=0.01, forcing_amp_hel=0.01,
                 forcing_k=1.0, cfl=0.5, nthreads=1, coriolis=False, centrifugal=False, rossby=0.0,
-                x_center=3.1415926, y_center=3.1415926, well_rad=1.0):
+                x_center=3.1415926, y_center=3.1415926, well_rad=1.0, origin=(0.0, 0.0, 0.0)):
     c = OrcConfig()
     c.nx, c.ny, c.nz = n
     c.npx, c.npy, c.npz = npr
@@ -216,6 +226,8 @@ def make_config(n=(16, 16, 16), npr=(1, 1, 1), L=None, scheme=WENO5JS, vis=CD6, 
     c.nthreads = nthreads
     c.coriolis, c.centrifugal = int(bool(coriolis)), int(bool(centrifugal))
     c.rossby, c.x_center, c.y_center, c.well_rad = rossby, x_center, y_center, well_rad
+    for d in range(3):
+        c.origin[d] = origin[d]
     return c
 
 

@@ -20,10 +20,16 @@ def _download_after(cfg, init):
         return ctx.download()
 
 
-@pytest.mark.parametrize("n,npr", [((32, 24, 16), (2, 2, 1)), ((20, 20, 20), (1, 1, 1))], ids=str)
-def test_tgv_device_equals_host(n, npr):
-    cfg = G.make_config(n=n, npr=npr, mach=0.1)
+@pytest.mark.parametrize("n,npr,origin", [((32, 24, 16), (2, 2, 1), (0.0, 0.0, 0.0)),
+                                          ((20, 20, 20), (1, 1, 1), (0.0, 0.0, 0.0)),
+                                          ((20, 16, 12), (2, 1, 1), (0.4, -1.1, 2.5))], ids=str)
+def test_tgv_device_equals_host(n, npr, origin):
+    cfg = G.make_config(n=n, npr=npr, mach=0.1, origin=origin)
     host = G.init_tgv(cfg)
+    if any(origin):
+        oc = O.make_config(n=n, npr=npr, mach=0.1, origin=origin)
+        np.testing.assert_array_equal(host, O.init_tgv(oc))
+        assert not np.array_equal(host, G.init_tgv(G.make_config(n=n, npr=npr, mach=0.1)))
     dev = _download_after(cfg, lambda c: c.init_tgv_device())
     N = n[0] * n[1] * n[2]
     a, b = dev.reshape(5, N), host.reshape(5, N)
@@ -47,9 +53,10 @@ def test_hit_device_equals_host(n, L, k0):
     assert np.all(err <= 1e-13), err
 
 
-@pytest.mark.parametrize("npr", [(1, 1, 1), (2, 2, 1)], ids=str)
-def test_ivc_and_default_device_equal_host_and_oracle(npr):
-    kw = dict(n=(40, 32, 8), L=(10.0, 10.0, 2.5), npr=npr, mach=0.3)
+@pytest.mark.parametrize("npr,origin", [((1, 1, 1), (0.0, 0.0, 0.0)), ((2, 2, 1), (0.0, 0.0, 0.0)),
+                                        ((2, 1, 1), (-1.5, 0.75, 0.0))], ids=str)
+def test_ivc_and_default_device_equal_host_and_oracle(npr, origin):
+    kw = dict(n=(40, 32, 8), L=(10.0, 10.0, 2.5), npr=npr, mach=0.3, origin=origin)
     cfg, oc = G.make_config(**kw), O.make_config(**kw)
     N = 40 * 32 * 8
     for name, host, orc, init in (("ivc", G.init_ivc, O.init_ivc, lambda c: c.init_ivc_device()),

@@ -364,12 +364,25 @@ SOURCE_CASES = [
 def test_sources_rhs_and_ten_steps_match_oracle(case):
     """ABC/TG forcing and the Coriolis / centrifugal sources
     (src/physicsModel.cpp:1280-1319,1679-1913) against the oracle."""
+    _source_case(case, (0.0, 0.0, 0.0))
+
+
+@pytest.mark.parametrize("case", [SOURCE_CASES[0], SOURCE_CASES[1], SOURCE_CASES[3]],
+                         ids=lambda c: f"f{c[0]}-cor{int(c[1])}-cen{int(c[2])}-s{c[3]}")
+def test_sources_with_block_origin_match_oracle(case):
+    """A nonzero block.in origin is subtracted from every cell centre
+    (src/block.cpp:116-118), which shifts the ABC/TG fields and the centrifugal
+    well: device and oracle must agree with it."""
+    _source_case(case, (0.7, -1.3, 2.1))
+
+
+def _source_case(case, origin):
     forcing, cor, cen, scheme, rie, model, npr = case
-    print(f"source case {case}", flush=True)
+    print(f"source case {case} origin {origin}", flush=True)
     kw = dict(n=(24, 20, 16), npr=npr, scheme=scheme, vis=G.CD6, riemann=rie, model=model, mach=0.2,
               reynolds=800.0, forcing_amp=0.05, forcing_k=1.0 if forcing != G.FORCING_CONST else 2.0,
               forcing_amp_hel=0.0, coriolis=cor, centrifugal=cen, rossby=0.6, x_center=3.0, y_center=3.3,
-              well_rad=1.4)
+              well_rad=1.4, origin=origin)
     gc = G.make_config(forcing=forcing, **kw)
     oc = O.make_config(forcing=forcing, **kw)
     q = _perturbed_tgv(oc, 7)

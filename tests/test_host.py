@@ -7,6 +7,7 @@ block.in (src/block.cpp:49-61), Method.in (src/fieldOperator.cpp:21-125),
 PhysModel.in (six positional pairs, then key/value; src/physicsModel.cpp:72-156)
 and IOController.in (src/IOcontroller.cpp:12-110).
 """
+import ctypes
 import os
 import re
 import time
@@ -186,3 +187,20 @@ def test_inp_abc_forcing_and_rotating_frame_sources():
     assert (s, e, c.cfl) == (0, 50, 0.4)
     c2, _, _ = G.config_from_inp(os.path.join(GOLD, "inp_tgv64"))
     assert (c2.coriolis, c2.centrifugal, c2.rossby, c2.well_rad) == (0, 0, 0.0, 1.0)
+
+
+def test_create_failure_message_without_a_context():
+    """nuc_create reports why it refused a configuration through
+    nuc_last_error(NULL) on the calling thread (validation runs before any
+    device call), and a successful-looking empty message otherwise."""
+    L = G.lib()
+    bad = G.make_config(n=(18, 16, 16), npr=(4, 1, 1))
+    h = ctypes.c_void_p()
+    rc = L.nuc_create(ctypes.byref(bad), ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert "divide evenly by the rank grid" in L.nuc_last_error(None).decode()
+    bad2 = G.make_config(n=(16, 16, 16), scheme=42)
+    assert L.nuc_create(ctypes.byref(bad2), ctypes.byref(h)) == 1
+    assert "unknown inviscid scheme" in L.nuc_last_error(None).decode()
+    with pytest.raises(G.NucError, match="divide evenly"):
+        G.Context(bad)

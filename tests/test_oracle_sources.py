@@ -103,3 +103,22 @@ def test_centrifugal_well_at_rest_is_radial():
     assert np.abs(-d[1] - fx).max() <= 1e-13 * np.abs(fx).max()
     assert np.abs(-d[2] - fy).max() <= 1e-13 * np.abs(fy).max()
     assert np.all(d[4] == 0) and np.all(d[3] == 0)   # the energy term goes to y-momentum (zero at rest)
+
+
+def test_block_origin_shifts_the_source_fields():
+    """The block.in origin is subtracted from the cell centres
+    (src/block.cpp:116-118).  With an origin of whole cells, (2h, -3h, h), every
+    source field on a state at rest is the field without origin rolled by
+    (2, -3, 1) cells, whatever the formula; the origin must also change it.
+    The well (not periodic) is narrow enough to vanish to 1e-12 at the box
+    faces that the roll wraps around (1e-11 of its peak)."""
+    h = 2 * np.pi / N1
+    kw = dict(forcing=O.FORCE_ABC, forcing_amp=0.05, centrifugal=True, rossby=0.6, x_center=3.0,
+              y_center=3.3, well_rad=0.7)
+    q = _state(_cfg(), 0, rest=True)
+    base = _delta(q, **kw)
+    shifted = _delta(q, origin=(2 * h, -3 * h, h), **kw)
+    # array axes are (z, y, x); a field f(x - o) sampled at cell i equals f at cell i - o/h
+    rolled = np.roll(base, shift=(1, -3, 2), axis=(1, 2, 3))
+    np.testing.assert_allclose(shifted, rolled, rtol=0, atol=1e-11 * np.abs(base).max())
+    assert np.abs(shifted - base).max() > 1e-3 * np.abs(base).max()
