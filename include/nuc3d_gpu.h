@@ -102,6 +102,22 @@ int nuc_download_state(nuc_ctx *ctx, double *q);
  * MPIComunicator.  rhs_host (5N, may be NULL) receives RHS with dQ/dt = -RHS;
  * dt_local (may be NULL) the min over ranks of getDt (src/euler3d.cpp:459-466). */
 int nuc_rhs(nuc_ctx *ctx, double *rhs_host, double *dt_local);
+/* nuc_rhs_noalloc: nuc_rhs without a context and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under hipGraph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  q_dev (5N device doubles, SoA, x
+ * fastest) is read, rhs_dev (5N device doubles) receives RHS with
+ * dQ/dt = -RHS, dt_local_dev (one device double, may be NULL) the min over
+ * ranks of getDt; all work is enqueued on `stream` (a hipStream_t, NULL for
+ * the null stream) and the call returns without waiting.  The intermediates
+ * (viscous fluxes, eigenvalue maxima) live in one fixed device array of the
+ * library sized for NUC_NOALLOC_MAX_CELLS cells (72 MiB, held on every GPU
+ * the library runs on until the process ends); larger grids and Const forcing
+ * (whose transform plan owns buffers) are refused with NUC_EINVAL.  Calls that
+ * overlap on one GPU share that array and race: serialise them (one stream,
+ * or events).  Errors: nuc_last_error(NULL) on the calling thread. */
+#define NUC_NOALLOC_MAX_CELLS 1048576L
+int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, double *rhs_dev, double *dt_local_dev, void *stream);
 /* nsteps TVD-RK3 steps: block::solve (src/block.cpp:191-215), i.e. three
  * (RHS + PDEData3d::solve) stages each.  info may be NULL. */
 int nuc_step(nuc_ctx *ctx, int nsteps, nuc_step_info *info);

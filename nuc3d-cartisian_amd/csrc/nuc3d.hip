@@ -1184,8 +1184,27 @@ extern "C" long long nuc_footprint(const nuc_config *c)
     return doubles * 8 + (N / 32 + 4096) * 8 + 4096 + fb;
 }
 
-// message of the last failed nuc_create on this thread (nuc_last_error(NULL))
+// message of the last failed call without a context on this thread
+// (nuc_create, nuc_rhs_noalloc), returned by nuc_last_error(NULL)
 static thread_local std::string g_create_err;
+
+// configuration, grid geometry and precision flags of a context (no device work)
+static void ctx_geometry(nuc_ctx *x, const nuc_config *cfg)
+{
+    x->c = *cfg;
+    x->n[0] = cfg->nx; x->n[1] = cfg->ny; x->n[2] = cfg->nz;
+    x->np[0] = cfg->npx; x->np[1] = cfg->npy; x->np[2] = cfg->npz;
+    double Lb[3] = {cfg->lx, cfg->ly, cfg->lz};
+    x->N = (long)cfg->nx * cfg->ny * cfg->nz;
+    x->P = cfg->npx * cfg->npy * cfg->npz;
+    for (int d = 0; d < 3; d++) {
+        x->nloc[d] = x->n[d] / x->np[d];
+        x->h[d] = Lb[d] / x->n[d];
+        x->ts[d] = scheme_halo(cfg->scheme[d]);
+        x->tsv[d] = vis_tilesize(cfg->scheme[d], cfg->vis_scheme[d]);
+        x->cpt[d] = rows_for(x->n[d]);
+    }
+}
 
 extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
 {
@@ -1204,19 +1223,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
         return NUC_ENODEV;
     }
     nuc_ctx *x = new nuc_ctx();
-    x->c = *cfg;
-    x->n[0] = cfg->nx; x->n[1] = cfg->ny; x->n[2] = cfg->nz;
-    x->np[0] = cfg->npx; x->np[1] = cfg->npy; x->np[2] = cfg->npz;
-    double Lb[3] = {cfg->lx, cfg->ly, cfg->lz};
-    x->N = (long)cfg->nx * cfg->ny * cfg->nz;
-    x->P = cfg->npx * cfg->npy * cfg->npz;
-    for (int d = 0; d < 3; d++) {
-        x->nloc[d] = x->n[d] / x->np[d];
-        x->h[d] = Lb[d] / x->n[d];
-        x->ts[d] = scheme_halo(cfg->scheme[d]);
-        x->tsv[d] = vis_tilesize(cfg->scheme[d], cfg->vis_scheme[d]);
-        x->cpt[d] = rows_for(x->n[d]);
-    }
+    ctx_geometry(x, cfg);
     const size_t N = (size_t)x->N;
     auto fail = [&](const char *what) {
         g_create_err = std::string("nuc_create: ") + what;
@@ -1729,6 +1736,91 @@ extern "C" int nuc_rhs(nuc_ctx *x, double *rhs_host, double *dt_local)
     NUC_CHECK(x, hipMemcpyAsync(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     if (dt_local) *dt_local = sc[3];
+    return NUC_OK;
+}
+
+// ---- nuc_rhs_noalloc: one RHS evaluation without any device allocation ----
+// The intermediates of an RHS evaluation that nuc_rhs keeps in its context
+// (the 9 viscous flux fields, the per-rank eigenvalue maxima, the scalar
+// slots) live in one fixed array of this code object instead.  Every GPU the
+// library runs on holds its own copy until the process ends; its address is
+// looked up once per device.  Calls that overlap on one GPU share the array
+// and race.
+constexpr long NOALLOC_MAX_CELLS = NUC_NOALLOC_MAX_CELLS;
+constexpr long NOALLOC_FV = 9 * NOALLOC_MAX_CELLS;              // doubles (float in the mixed path)
+constexpr long NOALLOC_LAM = 3 * MAXRANK;                       // uint64 slots
+constexpr long NOALLOC_DOUBLES = NOALLOC_FV + NOALLOC_LAM + 8;  // + 8 scalar slots
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;   // device ordinal -> device address of scratch
+
+__global__ void k_clear_u64(unsigned long long *p, int n)
+{
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0ull;
+}
+
+__global__ void k_copy_dt(const double *scal, double *dt_out) { *dt_out = scal[3]; }
+
+extern "C" int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, double *rhs_dev, double *dt_local_dev,
+                               void *stream)
+{
+    g_create_err.clear();
+    auto fail = [](int rc, const std::string &m) {
+        g_create_err = "nuc_rhs_noalloc: " + m;
+        return rc;
+    };
+    if (!cfg || !q_dev || !rhs_dev) return fail(NUC_EINVAL, "null config, state or rhs pointer");
+    std::string err;
+    int rc = validate(cfg, err);
+    if (rc != NUC_OK) return fail(rc, err);
+    const long N = (long)cfg->nx * cfg->ny * cfg->nz;
+    if (N > NOALLOC_MAX_CELLS)
+        return fail(NUC_EINVAL, std::to_string(N) + " cells exceed the scratch capacity of " +
+                                    std::to_string(NOALLOC_MAX_CELLS) + " cells (NUC_NOALLOC_MAX_CELLS)");
+    if (cfg->forcing == NUC_FORCING_CONST)
+        return fail(NUC_EINVAL, "Const forcing needs its transform plan's buffers: use a context");
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(NUC_ENODEV, "no current device");
+    double *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        auto it = g_scratch_ptr.find(dev);
+        if (it == g_scratch_ptr.end()) {
+            // set-up, once per device with that device current: the host-side
+            // name of a __device__ array is not a device address
+            void *p = nullptr;
+            hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return fail(NUC_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+            it = g_scratch_ptr.emplace(dev, static_cast<double *>(p)).first;
+        }
+        base = it->second;
+    }
+    // a context on the host stack that borrows the caller's buffers and the
+    // scratch array: the same launches as nuc_rhs, no device allocation
+    nuc_ctx x;
+    ctx_geometry(&x, cfg);
+    x.stream = static_cast<hipStream_t>(stream);
+    x.sources = cfg->forcing == NUC_FORCING_ABC || cfg->forcing == NUC_FORCING_TG || cfg->coriolis || cfg->centrifugal;
+    x.mixed = (cfg->precision == NUC_MIXED);
+    x.q[0] = const_cast<double *>(q_dev);
+    x.rhs = rhs_dev;
+    x.fv = base;
+    x.lam = reinterpret_cast<unsigned long long *>(base + NOALLOC_FV);
+    x.scal = base + NOALLOC_FV + NOALLOC_LAM;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        auto it = g_tune_gkz.find(std::make_tuple(x.n[0], x.n[1], x.n[2], x.mixed ? 1 : 0));
+        if (it != g_tune_gkz.end()) x.gkz = it->second;
+    }
+    // the array keeps the last call's contents: the eigenvalue maxima start
+    // from zero (a fill kernel, so that the call can be captured in a graph)
+    hipLaunchKernelGGL(k_clear_u64, dim3(1), dim3(256), 0, x.stream, x.lam, (int)NOALLOC_LAM);
+    enqueue_stage(&x, 0, 0, 0, false);
+    if (dt_local_dev) hipLaunchKernelGGL(k_copy_dt, dim3(1), dim3(1), 0, x.stream, x.scal, dt_local_dev);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(NUC_EHIP, std::string("launch: ") + hipGetErrorString(e));
     return NUC_OK;
 }
 

@@ -32,7 +32,7 @@ SPEC_DENSITY, SPEC_PRESSURE, SPEC_ENSTROPHY, SPEC_HELICITY = range(4)
 
 # every entry point include/nuc3d_gpu.h and include/nuc3d_serve.h declare
 EXPORTS = ["nuc_create", "nuc_destroy", "nuc_last_error", "nuc_footprint", "nuc_upload_state",
-           "nuc_download_state", "nuc_rhs", "nuc_step", "nuc_step_async", "nuc_set_graph", "nuc_query", "nuc_sync",
+           "nuc_download_state", "nuc_rhs", "nuc_rhs_noalloc", "nuc_step", "nuc_step_async", "nuc_set_graph", "nuc_query", "nuc_sync",
            "nuc_stats", "nuc_stats_ex", "nuc_op_reconstruct", "nuc_op_diff", "nuc_op_rk3", "nuc_config_from_inp",
            "nuc_init_tgv", "nuc_init_hit", "nuc_init_shu_osher", "nuc_profile", "nuc_profile_read",
            "nuc_server_create", "nuc_server_destroy", "nuc_server_submit", "nuc_server_result", "nuc_server_drain",
@@ -113,6 +113,7 @@ def lib():
         L.nuc_upload_state.argtypes = [vp, _dp]
         L.nuc_download_state.argtypes = [vp, _dp]
         L.nuc_rhs.argtypes = [vp, _dp, _dp]
+        L.nuc_rhs_noalloc.argtypes = [ctypes.POINTER(NucConfig), vp, vp, vp, vp]
         L.nuc_step.argtypes = [vp, ctypes.c_int, ctypes.POINTER(StepInfo)]
         L.nuc_step_async.argtypes = [vp, ctypes.c_int]
         L.nuc_set_graph.argtypes = [vp, ctypes.c_int]
