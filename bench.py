@@ -24,7 +24,10 @@ the committed SQ_INSTS_VALU summary) at one per 4 cycles per SIMD, over its
 measured launch time.
 
 cpu_baseline: the oracle (a C restatement of the reference algorithm) timed on
-this host's cores on one RHS evaluation of the same 256^3 workload.
+this host's CPU share (OMP_NUM_THREADS, else every core) on two full TVD-RK3
+steps of the same 256^3 workload, run as that many emulated MPI ranks (the
+box cut into the reference's rank blocks, one thread per rank's share of the
+line loops).
 
 mixed: the same workload with NUC_MIXED (FP32 split fluxes, reconstruction and
 viscous stencils; FP64 weights, tridiagonal solves, RK3 and reductions), timed
@@ -32,12 +35,14 @@ the same way, with the relative error of every global statistic (kinetic
 energy, dissipation, enstrophy, u_rms, Re_lambda, ...) against the FP64 run
 from the same initial state after the same steps.
 
-serving: a stream of independent boxes (Taylor-Green or decaying turbulence,
-N log-uniform per side, Mach U(0.1,0.5), Re U(400,3200), scheme uniform over
-WENO5-JS/WENO5-Z/CRWENO5, LF, cd6, CFL 0.5) through the serving loop:
-throughput with every job submitted at once against the same jobs run one at a
-time on plain contexts, and p50/p99 latency (arrival -> result) under Poisson
-arrivals at 20 jobs/s.
+serving: the serving stream -- 256 independent boxes (Taylor-Green or decaying
+turbulence, N log-uniform in [32, 192] per side, Mach U(0.1,0.5),
+Re U(400,3200), scheme uniform over WENO5-JS/WENO5-Z/CRWENO5, LF, cd6,
+CFL 0.5, 300 RK3 steps each, initial states generated on the device) arriving
+as a Poisson stream at 20 jobs/s -- through the serving loop, as an FP64 and a
+mixed stream: jobs/s and p50/p99 latency (arrival -> result), against the same
+jobs run one at a time on plain contexts, and the ceiling any schedule could
+reach (the stream's cell-updates at the 256^3 box's rate).
 """
 import argparse
 import json
@@ -59,23 +64,48 @@ def workload_config(G, n=256, precision=None):
                          prandtl=0.72, gamma=1.4, cfl=0.5, precision=G.FP64 if precision is None else precision)
 
 
-def cpu_baseline(q, n, threads):
-    """Time the oracle's RHS evaluation of the same state on the host cores."""
+def rank_grid(P, n):
+    """P emulated ranks as an (npx, npy, npz) grid whose sizes divide n (the
+    reference's MPI Cartesian grid, src/MPICommunicator.cpp:26-56)."""
+    dims = [1, 1, 1]
+    f, rest = 2, P
+    primes = []
+    while rest > 1:
+        while rest % f == 0:
+            primes.append(f)
+            rest //= f
+        f += 1
+    for pr in sorted(primes, reverse=True):
+        d = min((i for i in range(3) if n % (dims[i] * pr) == 0), key=lambda i: dims[i], default=None)
+        if d is None:
+            continue   # a factor that does not divide the grid: fewer ranks than threads
+        dims[d] *= pr
+    return tuple(dims)
+
+
+def cpu_baseline(q, n, threads, steps=2):
+    """Time the oracle on `steps` full TVD-RK3 steps of the same state on the
+    host cores, run as P = threads emulated MPI ranks: the box is cut into the
+    reference's P rank blocks (compact-scheme closures per block, as a P-rank
+    reference run places them) and every line loop runs on P threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
-    oc = O.make_config(n=(n, n, n), npr=(1, 1, 1), scheme=O.CRWENO5, vis=O.CD8, riemann=O.LF, model=O.NS,
+    npr = rank_grid(threads, n)
+    oc = O.make_config(n=(n, n, n), npr=npr, scheme=O.CRWENO5, vis=O.CD8, riemann=O.LF, model=O.NS,
                        mach=0.3 / np.sqrt(3.0), reynolds=200.0, cfl=0.5, nthreads=threads)
     s = O.Solver(oc)
     s.set_state(q)
     t0 = time.perf_counter()
-    s.rhs()
+    s.step(steps)
     t1 = time.perf_counter()
     del s
     cells = n ** 3
-    return {"value": cells / (t1 - t0) / 1e6, "unit": "Mcell-updates/s per RK3 stage", "cores": threads,
-            "kind": "port",
-            "sample": f"one RHS evaluation (one RK stage) of the same {n}^3 CRWENO5/LF/cd8 state, "
+    return {"value": cells * 3 * steps / (t1 - t0) / 1e6, "unit": "Mcell-updates/s per RK3 stage",
+            "cores": threads, "kind": "port", "ranks": list(npr),
+            "sample": f"{steps} full TVD-RK3 steps ({3 * steps} RHS evaluations + stage updates) of the same "
+                      f"{n}^3 CRWENO5/LF/cd8 state as {npr[0] * npr[1] * npr[2]} emulated ranks "
+                      f"{npr[0]}x{npr[1]}x{npr[2]}, "
                       f"oracle C restatement, gcc -O3 -ffp-contract=off, OpenMP {threads} threads, "
                       f"{t1 - t0:.1f} s"}
 
@@ -175,13 +205,22 @@ def budget_error(G, n, q, steps=3):
                                 for i, t in enumerate(BUDGET_TERMS)}}
 
 
-# compatible jobs packed into one launch up to 65536 cells per pack (pairs of
-# 32^3 boxes): packing beats concurrent streams only for such tiny boxes
-SERVE_PACK_CELLS = 1 << 16
+# The serving stream: 256 independent jobs, Poisson arrivals at 20 jobs/s,
+# Taylor-Green or decaying turbulence (50/50), N log-uniform in [32, 192] per
+# side, Mach U(0.1, 0.5), Re U(400, 3200), scheme uniform over
+# WENO5-JS/WENO5-Z/CRWENO5, LF, cd6, CFL 0.5, 300 RK3 steps each; FP64 and
+# mixed precision as separate streams admitted against a fixed HBM budget.
+SERVE_JOBS, SERVE_NMIN, SERVE_NMAX, SERVE_STEPS, SERVE_RATE = 256, 32, 192, 300, 20.0
+SERVE_SEED = 20261202 + 500
+SERVE_HBM_GIB = 64
 SERVE_MAX_ACTIVE = 16
+# idle compatible jobs of together at most this many cells advance as one
+# pack (one launch per kernel): up to four 40^3 or two 48^3 boxes
+SERVE_PACK_CELLS = 4 * 40 ** 3
 
 
-def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
+def serving_jobs(G, njobs, nmin, nmax, rate, seed):
+    """Job specs (cfg, initial condition, u_rms, seed, n) and Poisson arrival times."""
     import numpy as np
     rng = np.random.default_rng(seed)
     jobs = []
@@ -193,33 +232,41 @@ def serving_stream(G, njobs, nmin, nmax, steps, rate, seed):
         scheme = [G.WENO5JS, G.WENO5Z, G.CRWENO5][rng.integers(3)]
         cfg = G.make_config(n=(n, n, n), scheme=scheme, vis=G.CD6, riemann=G.LF, model=G.NAVIERSTOKES3D,
                             mach=mach, reynolds=re, cfl=0.5)
-        if rng.random() < 0.5:
-            q = G.init_tgv(cfg)
-        else:
-            q = G.init_hit(cfg, k0=4.0, u_rms=min(1.0, 0.3 / (np.sqrt(3.0) * mach)), seed=int(rng.integers(1 << 30)))
-        jobs.append((cfg, q, n))
+        hit = rng.random() >= 0.5
+        u_rms = min(1.0, 0.3 / (np.sqrt(3.0) * mach))
+        jobs.append((cfg, G.IC_HIT if hit else G.IC_TGV, u_rms, int(rng.integers(1 << 30)), n))
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=njobs))
     return jobs, arrivals
 
 
-def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=SERVE_MAX_ACTIVE, stats=None,
-                steps=None, hbm_budget=96 << 30, progress=0):
-    """All jobs through one server; arrivals None = all at once. Returns (wall s, latencies s)."""
+def with_precision(jobs, prec):
+    import copy
+    out = []
+    for cfg, ic, u_rms, seed, n in jobs:
+        c = copy.copy(cfg)
+        c.precision = prec
+        out.append((c, ic, u_rms, seed, n))
+    return out
+
+
+def run_stream(G, jobs, arrivals, steps, pack_cells=SERVE_PACK_CELLS, stats=None, label=""):
+    """Jobs submitted at their arrival times (None: all at once), initial
+    states generated on the device at admission; returns (wall s, latencies s)."""
     import numpy as np
-    steps = SERVE_STEPS if steps is None else steps
-    with G.Server(hbm_budget=hbm_budget, max_active=max_active, pack_cells=pack_cells, autotune=True) as srv:
+    with G.Server(hbm_budget=SERVE_HBM_GIB << 30, max_active=SERVE_MAX_ACTIVE, pack_cells=pack_cells,
+                  autotune=True) as srv:
         ids = []
         t0 = time.perf_counter()
-        for k, (cfg, q, _) in enumerate(jobs):
+        for k, (cfg, ic, u_rms, seed, _) in enumerate(jobs):
             if arrivals is not None:
                 wait = arrivals[k] - (time.perf_counter() - t0)
                 if wait > 0:
                     time.sleep(wait)
-            ids.append(srv.submit(cfg, q, steps))
+            ids.append(srv.submit_ic(cfg, ic, steps, k0=4.0, u_rms=u_rms, seed=seed))
         lat = []
         for k, jid in enumerate(ids):
-            if progress and k % progress == 0:
-                print(f"bench: serving, {k}/{len(ids)} jobs done, {time.perf_counter() - t0:.1f} s",
+            if label and k % 32 == 0:
+                print(f"bench: {label} stream, {k}/{len(ids)} jobs done, {time.perf_counter() - t0:.1f} s",
                       file=sys.stderr, flush=True)
             while True:
                 r = srv.result(jid, want_state=False)
@@ -235,65 +282,63 @@ def run_serving(G, jobs, arrivals=None, pack_cells=SERVE_PACK_CELLS, max_active=
     return wall, np.array(lat)
 
 
-def run_sequential(G, jobs, steps=None, progress=0):
-    steps = SERVE_STEPS if steps is None else steps
+def run_sequential(G, jobs, steps, label=""):
+    """The same jobs one at a time on plain contexts (same device initial
+    conditions, nuc_step); returns the wall time."""
     t0 = time.perf_counter()
-    for k, (cfg, q, _) in enumerate(jobs):
-        if progress and k % progress == 0:
-            print(f"bench: sequential, {k}/{len(jobs)} jobs done, {time.perf_counter() - t0:.1f} s",
+    for k, (cfg, ic, u_rms, seed, _) in enumerate(jobs):
+        if label and k % 32 == 0:
+            print(f"bench: {label} sequential, {k}/{len(jobs)} jobs done, {time.perf_counter() - t0:.1f} s",
                   file=sys.stderr, flush=True)
         with G.Context(cfg) as ctx:
-            ctx.upload(q)
+            if ic == G.IC_HIT:
+                ctx.init_hit_device(k0=4.0, u_rms=u_rms, seed=seed)
+            else:
+                ctx.init_tgv_device()
             ctx.step(steps)
-            ctx.download()
     return time.perf_counter() - t0
 
 
-def serve_full(G):
-    """The full serving stream: 256 jobs (TGV or decaying turbulence, N
-    log-uniform in [32, 192], Mach U(0.1, 0.5), Re U(400, 3200), scheme uniform
-    over WENO5-JS/WENO5-Z/CRWENO5, 300 RK3 steps each), Poisson arrivals at 20
-    jobs/s, admitted against a 64 GiB HBM budget, as an FP64 stream and a mixed
-    stream; each against the same jobs run one at a time on plain contexts."""
-    import copy
+def serving_leg(G, rate_256, njobs=SERVE_JOBS, steps=SERVE_STEPS, nmax=SERVE_NMAX):
+    """The serving stream in FP64 and in mixed precision: served throughput and
+    latency against sequential plain solves of the same jobs, and the ceiling
+    any schedule could reach (the stream's cell-updates at the 256^3 box's rate)."""
     import numpy as np
-    njobs, steps = 256, 300
-    print("bench: generating the 256-job stream", file=sys.stderr, flush=True)
-    jobs, arrivals = serving_stream(G, njobs, 32, 192, steps, 20.0, 20261202 + 500)
-    out = {"metric": "serving stream: jobs/s and p50/p99 job latency (arrival to result)",
-           "jobs": njobs, "steps_per_job": steps, "grid_n": [32, 192], "poisson_rate_per_s": 20.0,
-           "hbm_budget_gib": 64, "max_active": SERVE_MAX_ACTIVE, "pack_cells": SERVE_PACK_CELLS,
-           "cells_per_job_mean": float(np.mean([nj ** 3 for _, _, nj in jobs]))}
+    jobs, arrivals = serving_jobs(G, njobs, SERVE_NMIN, nmax, SERVE_RATE, SERVE_SEED)
+    cell_updates = float(sum(n ** 3 for *_, n in jobs)) * 3 * steps
+    out = {"jobs": njobs, "steps_per_job": steps, "grid_n": [SERVE_NMIN, nmax], "poisson_rate_per_s": SERVE_RATE,
+           "hbm_budget_gib": SERVE_HBM_GIB, "max_active": SERVE_MAX_ACTIVE, "pack_cells": SERVE_PACK_CELLS,
+           "cell_updates": cell_updates}
     for label, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
-        stream = []
-        for cfg, q, nj in jobs:
-            c = copy.copy(cfg)
-            c.precision = prec
-            stream.append((c, q, nj))
-        print(f"bench: {label} stream, warm-up and tuner cache", file=sys.stderr, flush=True)
-        run_serving(G, stream[:4], steps=2)
+        stream = with_precision(jobs, prec)
+        # kernels, allocator pools and the autotuner's per-shape cache warmed
+        # as in a running server (one tuning per distinct box shape), untimed
+        run_stream(G, stream[:2], None, 2)
         seen = set()
-        for cfg, q, nj in stream:
-            if nj not in seen:
-                seen.add(nj)
+        for cfg, ic, u_rms, seed, n in stream:
+            if n not in seen:
+                seen.add(n)
                 with G.Context(cfg) as tctx:
-                    tctx.upload(q)
+                    tctx.init_tgv_device()
                     tctx.autotune()
         st = {}
-        wall, lat = run_serving(G, stream, arrivals, steps=steps, hbm_budget=64 << 30, progress=32, stats=st)
-        seq = run_sequential(G, stream, steps=steps, progress=32)
+        wall, lat = run_stream(G, stream, arrivals, steps, stats=st, label=label)
+        seq = run_sequential(G, stream, steps, label=label)
+        rate = rate_256[label]
+        ceil = njobs / (cell_updates / (rate * 1e6))
         out[label] = {"jobs_per_s": round(njobs / wall, 3), "wall_s": round(wall, 2),
                       "sequential_jobs_per_s": round(njobs / seq, 3), "sequential_wall_s": round(seq, 2),
                       "speedup_vs_sequential": round(seq / wall, 3),
                       "p50_latency_s": round(float(np.percentile(lat, 50)), 3),
                       "p99_latency_s": round(float(np.percentile(lat, 99)), 3),
+                      "ceiling_jobs_per_s": round(ceil, 3), "fraction_of_ceiling": round(njobs / wall / ceil, 3),
+                      "ceiling_speedup_vs_sequential": round(ceil / (njobs / seq), 3),
                       "peak_active": int(st.get("peak_active", 0)), "peak_bytes": int(st.get("peak_bytes", 0)),
-                      "pack_slices": int(st.get("pack_slices", 0))}
-        print(f"bench: {label} stream {out[label]}", file=sys.stderr, flush=True)
-    print(json.dumps(out))
+                      "single_slices": int(st.get("single_slices", 0)), "pack_slices": int(st.get("pack_slices", 0))}
+        print(f"bench: {label} serving {json.dumps(out[label])}", file=sys.stderr, flush=True)
+    return out
 
 
-SERVE_STEPS = 100
 
 
 def main():
@@ -307,9 +352,11 @@ def main():
     ap.add_argument("--profile-slots", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--serve-jobs", type=int, default=64)
-    ap.add_argument("--serve-full", action="store_true",
-                    help="only the full 256-job serving stream (N 32-192, 300 steps; several minutes)")
+    ap.add_argument("--serve-jobs", type=int, default=SERVE_JOBS,
+                    help="jobs of the serving stream (default: the full 256-job stream)")
+    ap.add_argument("--serve-steps", type=int, default=SERVE_STEPS)
+    ap.add_argument("--serve-nmax", type=int, default=SERVE_NMAX)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="full RK3 steps of the CPU baseline")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box per GPU and measures one MI355X; "
@@ -318,10 +365,6 @@ def main():
 
     import numpy as np
     import nuc3d_amd as G
-
-    if args.serve_full:
-        serve_full(G)
-        return
 
     n = args.n
     cfg = workload_config(G, n)
@@ -379,37 +422,16 @@ def main():
 
     serving = None
     if not args.no_serving:
-        print("bench: serving", file=sys.stderr, flush=True)
-        jobs, arrivals = serving_stream(G, args.serve_jobs, 32, 96, SERVE_STEPS, 20.0, 20261202 + 500)
-        run_serving(G, jobs[:4])                      # warm the kernels and the allocator
-        # the autotuner's per-shape cache, as a running server holds it for
-        # shapes it has seen: one tuning per distinct box shape, untimed
-        seen = set()
-        for jcfg, jq, nj in jobs:
-            if nj not in seen:
-                seen.add(nj)
-                with G.Context(jcfg) as tctx:
-                    tctx.upload(jq)
-                    tctx.autotune()
-        sst = {}
-        wall_sat, _ = run_serving(G, jobs, stats=sst)
-        wall_seq = run_sequential(G, jobs)
-        wall_poi, lat = run_serving(G, jobs, arrivals)
-        serving = {"jobs": len(jobs), "steps_per_job": SERVE_STEPS, "grid_n": [32, 96],
-                   "jobs_per_s": round(len(jobs) / wall_sat, 3),
-                   "sequential_jobs_per_s": round(len(jobs) / wall_seq, 3),
-                   "speedup_vs_sequential": round(wall_seq / wall_sat, 3),
-                   "poisson_rate_per_s": 20.0,
-                   "p50_latency_s": round(float(np.percentile(lat, 50)), 4),
-                   "p99_latency_s": round(float(np.percentile(lat, 99)), 4),
-                   "poisson_wall_s": round(wall_poi, 3),
-                   "max_active": SERVE_MAX_ACTIVE, "pack_cells": SERVE_PACK_CELLS,
-                   "single_slices": int(sst.get("single_slices", 0)), "pack_slices": int(sst.get("pack_slices", 0))}
+        print("bench: serving stream", file=sys.stderr, flush=True)
+        rates = {"fp64": value, "mixed": mixed["value"] if mixed else value}
+        serving = serving_leg(G, rates, njobs=args.serve_jobs, steps=args.serve_steps, nmax=args.serve_nmax)
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(q, n, threads)
+        # the GPU host's CPU share (OMP_NUM_THREADS is set to it), else every core
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        print(f"bench: CPU baseline, {threads} threads, {args.cpu_steps} steps", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(q, n, threads, steps=args.cpu_steps)
 
     out = {
         "metric": "Mcell-updates/s per RK3 stage at 256^3, FP64 vs mixed; serving jobs/s + p50/p99",

@@ -43,6 +43,37 @@ int scheme_from_name(const std::string &s, int *halo)
     return -1;
 }
 
+int read_method(const std::string &dir, int scheme[3], int vis_scheme[3], std::string &err)
+{
+    // defaults src/fieldOperator.cpp:22-33
+    std::map<std::string, std::string> method = {
+        {"scheme_time", "RK3"}, {"scheme_x_ivs", "weno5js"}, {"scheme_y_ivs", "weno5js"},
+        {"scheme_z_ivs", "weno5js"}, {"scheme_x_vis", "cd6"}, {"scheme_y_vis", "cd6"}, {"scheme_z_vis", "cd6"}};
+    std::vector<std::pair<std::string, std::string>> kv;
+    if (!read_pairs(dir + "/Method.in", kv)) { err = "inp/Method.in does not exist"; return NUC_EINVAL; }
+    for (auto &p : kv) {
+        if (method.count(p.first)) method[p.first] = p.second;
+        else fprintf(stderr, "Method.in: parameter '%s' is not supported\n", p.first.c_str());
+    }
+    const char *ivs[3] = {"scheme_x_ivs", "scheme_y_ivs", "scheme_z_ivs"};
+    const char *vis[3] = {"scheme_x_vis", "scheme_y_vis", "scheme_z_vis"};
+    for (int d = 0; d < 3; d++) {
+        int halo;
+        int s = scheme_from_name(method[ivs[d]], &halo);
+        if (s < 0) {
+            err = "Method.in: inviscid scheme '" + method[ivs[d]] + "' is not available";
+            return NUC_EINVAL;
+        }
+        scheme[d] = s;
+        const std::string &v = method[vis[d]];
+        if (v == "cd8") vis_scheme[d] = NUC_CD8;
+        else if (v == "cd6") vis_scheme[d] = NUC_CD6;
+        else if (v == "cd2") vis_scheme[d] = NUC_CD2;
+        else vis_scheme[d] = NUC_CD6;  // src/fieldOperator.cpp:309-312 default
+    }
+    return NUC_OK;
+}
+
 int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string &err)
 {
     std::memset(&c, 0, sizeof(c));
@@ -59,34 +90,8 @@ int read_inp(const std::string &dir, nuc_config &c, RunControl &rc, std::string 
         rc.origin[2] = f ? z0 : 0.0;
         for (int d = 0; d < 3; d++) c.origin[d] = rc.origin[d];
     }
-    // ---- Method.in (defaults src/fieldOperator.cpp:22-33)
-    std::map<std::string, std::string> method = {
-        {"scheme_time", "RK3"}, {"scheme_x_ivs", "weno5js"}, {"scheme_y_ivs", "weno5js"},
-        {"scheme_z_ivs", "weno5js"}, {"scheme_x_vis", "cd6"}, {"scheme_y_vis", "cd6"}, {"scheme_z_vis", "cd6"}};
-    {
-        std::vector<std::pair<std::string, std::string>> kv;
-        if (!read_pairs(dir + "/Method.in", kv)) { err = "inp/Method.in does not exist"; return NUC_EINVAL; }
-        for (auto &p : kv) {
-            if (method.count(p.first)) method[p.first] = p.second;
-            else fprintf(stderr, "Method.in: parameter '%s' is not supported\n", p.first.c_str());
-        }
-        const char *ivs[3] = {"scheme_x_ivs", "scheme_y_ivs", "scheme_z_ivs"};
-        const char *vis[3] = {"scheme_x_vis", "scheme_y_vis", "scheme_z_vis"};
-        for (int d = 0; d < 3; d++) {
-            int halo;
-            int s = scheme_from_name(method[ivs[d]], &halo);
-            if (s < 0) {
-                err = "Method.in: inviscid scheme '" + method[ivs[d]] + "' is not available";
-                return NUC_EINVAL;
-            }
-            c.scheme[d] = s;
-            const std::string &v = method[vis[d]];
-            if (v == "cd8") c.vis_scheme[d] = NUC_CD8;
-            else if (v == "cd6") c.vis_scheme[d] = NUC_CD6;
-            else if (v == "cd2") c.vis_scheme[d] = NUC_CD2;
-            else c.vis_scheme[d] = NUC_CD6;  // src/fieldOperator.cpp:309-312 default
-        }
-    }
+    // ---- Method.in
+    if (read_method(dir, c.scheme, c.vis_scheme, err) != NUC_OK) return NUC_EINVAL;
     // ---- PhysModel.in (defaults src/physicsModel.cpp:22-43)
     std::map<std::string, double> par = {
         {"Reynolds", 1000}, {"Mach", 0.5}, {"Pt", 0.72}, {"Rossby", 0.0}, {"x_center", 3.1415926},

@@ -73,30 +73,55 @@ void tvdrk3rd::integrationAll(const VectorField &rhs, const VectorField &u_n, Ve
               "nuc_op_rk3");
 }
 
-fieldOperator3d::fieldOperator3d(const nuc_config &c)
+// bufferSize of an inviscid scheme id (src/fieldOperator.cpp:128-220)
+static int scheme_buffer(int scheme)
+{
+    if (scheme == NUC_WENO7JS || scheme == NUC_HCCS || scheme == NUC_HCCS77 || scheme == NUC_HCCS78) return 4;
+    if (scheme == NUC_WENO3JS) return 2;
+    if (scheme == NUC_UPWIND1ST) return 1;
+    return 3;
+}
+
+void fieldOperator3d::build(const int scheme[3], const int vis[3])
 {
     int *bs[3] = {&bufferSize_xi, &bufferSize_eta, &bufferSize_zeta};
+    interp_.clear();
+    diff_.clear();
     for (int d = 0; d < 3; d++) {
-        *bs[d] = (c.scheme[d] == NUC_WENO7JS || c.scheme[d] == NUC_HCCS) ? 4 : 3;
-        interp_.push_back(std::make_shared<gpuInterpolation>(c.scheme[d]));
-        diff_.push_back(std::make_shared<gpuDifferential>(c.vis_scheme[d]));
-        vis_[d] = c.vis_scheme[d];
+        *bs[d] = scheme_buffer(scheme[d]);
+        interp_.push_back(std::make_shared<gpuInterpolation>(scheme[d]));
+        diff_.push_back(std::make_shared<gpuDifferential>(vis[d]));
+        scheme_[d] = scheme[d];
+        vis_[d] = vis[d];
     }
-    integ_ = std::make_shared<tvdrk3rd>();
+    integ_ = std::make_shared<tvdrk3rd>();   // scheme_time RK3, the only choice (src/fieldOperator.cpp:366-385)
 }
+
+fieldOperator3d::fieldOperator3d(const nuc_config &c) { build(c.scheme, c.vis_scheme); }
+
+fieldOperator3d::fieldOperator3d(const std::string &inp_dir)
+{
+    int sch[3], vis[3];
+    std::string err;
+    if (read_method(inp_dir, sch, vis, err) != NUC_OK) throw nuc_error(NUC_EINVAL, err);
+    build(sch, vis);
+}
+
+fieldOperator3d::fieldOperator3d() : fieldOperator3d(std::string("inp")) {}
 
 int fieldOperator3d::getBufferSize() const
 {
     int b = std::max(bufferSize_xi, std::max(bufferSize_eta, bufferSize_zeta));
-    for (int d = 0; d < 3; d++)
-        if (vis_[d] == NUC_CD8) b = std::max(b, 4);
+    for (int d = 0; d < 3; d++) b = std::max(b, tsv(d));
     return b;
 }
 
 int fieldOperator3d::tsv(int dir) const
 {
-    // cd8 runs with four ghost cells (the reference passes the inviscid halo, src/fieldOperator.cpp:520-526)
-    return (vis_[dir] == NUC_CD8 && ts(dir) < 4) ? 4 : ts(dir);
+    // the viscous stencil's reach (cd2 1, cd6 3, cd8 4) when the inviscid halo
+    // is narrower (the reference passes the inviscid halo, src/fieldOperator.cpp:520-526)
+    const int reach = vis_[dir] == NUC_CD8 ? 4 : (vis_[dir] == NUC_CD2 ? 1 : 3);
+    return std::max(ts(dir), reach);
 }
 
 void fieldOperator3d::reconstructionInner(const Field &in, int dir, int upwind, Field &out)
@@ -125,54 +150,91 @@ void fieldOperator3d::timeIntegral(const VectorField &rhs, const VectorField &u_
     integ_->integrationAll(rhs, u_n, u_i, dt, stage);
 }
 
-// ---- PDE facade ---------------------------------------------------------------
+// ---- PDE level ------------------------------------------------------------------
 PDEData3d::PDEData3d(const nuc_config &c) : cfg_(c) { check(nuc_create(&c, &ctx_), "nuc_create"); }
 PDEData3d::~PDEData3d() { nuc_destroy(ctx_); }
+
+static void to_fields(const nuc_config &c, const std::vector<double> &q, VectorField &F)
+{
+    const size_t N = (size_t)c.nx * c.ny * c.nz;
+    F.clear();
+    for (int v = 0; v < 5; v++) {
+        F.emplace_back(c.nx, c.ny, c.nz);
+        std::memcpy(F[v].getDataPtr(), q.data() + v * N, N * sizeof(double));
+    }
+}
 
 void PDEData3d::setQ(const VectorField &Q)
 {
     size_t N = (size_t)cfg_.nx * cfg_.ny * cfg_.nz;
+    if (Q.size() != 5) throw nuc_error(NUC_EINVAL, "setQ: five conserved variables expected");
     std::vector<double> q(5 * N);
-    for (int v = 0; v < 5; v++) std::memcpy(q.data() + v * N, Q[v].getDataPtr(), N * sizeof(double));
-    check(nuc_upload_state(ctx_, q.data()), "nuc_upload_state");
-}
-
-void PDEData3d::getQ(VectorField &Q)
-{
-    size_t N = (size_t)cfg_.nx * cfg_.ny * cfg_.nz;
-    std::vector<double> q(5 * N);
-    check(nuc_download_state(ctx_, q.data()), "nuc_download_state");
-    Q.clear();
     for (int v = 0; v < 5; v++) {
-        Q.emplace_back(cfg_.nx, cfg_.ny, cfg_.nz);
-        std::memcpy(Q[v].getDataPtr(), q.data() + v * N, N * sizeof(double));
+        if ((size_t)Q[v].getSize() != N) throw nuc_error(NUC_EINVAL, "setQ: field size differs from the grid");
+        std::memcpy(q.data() + v * N, Q[v].getDataPtr(), N * sizeof(double));
     }
+    check(nuc_upload_state(ctx_, q.data()), "nuc_upload_state");
+    rhs_pending_ = false;
 }
 
-double PDEData3d::solveRHS(VectorField *rhs)
+VectorField &PDEData3d::getQ()
 {
-    size_t N = (size_t)cfg_.nx * cfg_.ny * cfg_.nz;
-    std::vector<double> r(rhs ? 5 * N : 0);
-    double dt = 0.0;
-    check(nuc_rhs(ctx_, rhs ? r.data() : nullptr, &dt), "nuc_rhs");
-    if (rhs) {
-        rhs->clear();
-        for (int v = 0; v < 5; v++) {
-            rhs->emplace_back(cfg_.nx, cfg_.ny, cfg_.nz);
-            std::memcpy((*rhs)[v].getDataPtr(), r.data() + v * N, N * sizeof(double));
-        }
-    }
-    return dt;
+    std::vector<double> q(5 * (size_t)cfg_.nx * cfg_.ny * cfg_.nz);
+    check(nuc_download_state(ctx_, q.data()), "nuc_download_state");
+    to_fields(cfg_, q, Q_);
+    return Q_;
+}
+
+VectorField &PDEData3d::getRHS()
+{
+    std::vector<double> r(5 * (size_t)cfg_.nx * cfg_.ny * cfg_.nz);
+    check(nuc_rhs(ctx_, r.data(), &dt_local), "nuc_rhs");
+    to_fields(cfg_, r, RHS_);
+    return RHS_;
+}
+
+void PDEData3d::solve(fieldOperator3d &, MPIComunicator3d_nonblocking &, double cfl, int step)
+{
+    if (!rhs_pending_)
+        throw nuc_error(NUC_EINVAL, "PDEData3d::solve: call the fluid model's solve for this stage first "
+                                    "(src/block.cpp:198-202)");
+    if (cfl != cfg_.cfl)
+        throw nuc_error(NUC_EINVAL, "PDEData3d::solve: cfl differs from the context's configuration");
+    rhs_pending_ = false;
+    check(nuc_rk3_stage(ctx_, step), "nuc_rk3_stage");
+    // dt_global set at stage 0, the residual at stage 2 (PDEData3d::setRES,
+    // src/PDEData3d.cpp:93-128); both live on the device
+    nuc_step_info info{};
+    check(nuc_sync(ctx_, &info), "nuc_sync");
+    dt_global = info.dt;
+    if (step == 2) res_global = info.res;
 }
 
 nuc_step_info PDEData3d::step(int n)
 {
     nuc_step_info info{};
     check(nuc_step(ctx_, n, &info), "nuc_step");
+    dt_global = info.dt;
+    res_global = info.res;
+    rhs_pending_ = false;
     return info;
 }
 
-void PDEData3d::solve(int stage) { check(nuc_rk3_stage(ctx_, stage), "nuc_rk3_stage"); }
+void EulerData3D::solve(PDEData3d &pde, fieldOperator3d &op, VectorBuffer &, physicsModel &phys,
+                        MPIComunicator3d_nonblocking &mpi, fft &, const VectorField &)
+{
+    const nuc_config &c = pde.config();
+    if (c.model != model_)
+        throw nuc_error(NUC_EINVAL, "EulerData3D::solve: the PDE's context runs the other model");
+    for (int d = 0; d < 3; d++)
+        if (op.schemeId(d) != c.scheme[d] || op.visSchemeId(d) != c.vis_scheme[d])
+            throw nuc_error(NUC_EINVAL, "EulerData3D::solve: fieldOperator3d schemes differ from the context's");
+    if (phys.getGamma() != c.gamma || phys.getMach() != c.mach || phys.getReynolds() != c.reynolds)
+        throw nuc_error(NUC_EINVAL, "EulerData3D::solve: physicsModel constants differ from the context's");
+    if (mpi.getMyDim(0) != c.npx || mpi.getMyDim(1) != c.npy || mpi.getMyDim(2) != c.npz)
+        throw nuc_error(NUC_EINVAL, "EulerData3D::solve: rank grid differs from the context's");
+    pde.rhs_pending_ = true;
+}
 
 // ---- initial conditions -------------------------------------------------------
 void initial_taylorgreen(const nuc_config &c, const double origin[3], std::vector<double> &q)
