@@ -71,20 +71,17 @@ struct SweepArgs {
     int nx, ny, nz;
     long nlines;
     double inv_h, gamma;
-    // the 5 conserved fields of the stage input, the RHS, u^n and the stage
-    // output each lie fstride doubles apart (one allocation per quantity)
-    const double GMEM *q;
+    const double GMEM *q[5];
     const void GMEM *fv[4];   // viscous flux components (float when fv_f32, else double)
     int fv_f32;
-    double GMEM *rhs;
-    long fstride;
+    double GMEM *rhs[5];
     int stage, epilogue;
     int first;                // the stage's first sweep: writes rhs without reading it
     int accum;                // the first sweep adds to rhs (forcing initialised it)
     int mixed;                // FP32 stencils (host-side dispatch only)
     const double GMEM *fmean; // last sweep: <f.u> added back to the energy RHS
-    const double GMEM *qn;
-    double GMEM *qout;
+    const double GMEM *qn[5];
+    double GMEM *qout[5];
     const double GMEM *dtp;
     double GMEM *res_part;
 };
@@ -280,9 +277,25 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
             con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
             qz[0][8] = T(u); qz[1][8] = T(v); qz[2][8] = T(w); qz[3][8] = T(Tt);
         }
-        // tile of plane k with wrapped x/y halos
-        for (int c = t; c < PX * PY; c += GTX * GTY) {
-            int cx = c % PX, cy = c / PX;
+        // tile of plane k with wrapped x/y halos: the thread's own (wrapped)
+        // cell is the centre of its register queue; only the 4-cell ring
+        // around the tile is read from the state
+#pragma unroll
+        for (int f = 0; f < 4; f++) tile[f][ty + 4][tx + 4] = qz[f][4];
+        constexpr int RING_ROWS = 8 * PX;                  // 4 rows above and below the tile
+        constexpr int RING = RING_ROWS + GTY * 8;          // + 4 columns left and right
+        for (int c = t; c < RING; c += GTX * GTY) {
+            int cx, cy;
+            if (c < RING_ROWS) {
+                cx = c % PX;
+                cy = c / PX;
+                cy = cy < 4 ? cy : cy + GTY;
+            } else {
+                const int u8 = c - RING_ROWS;
+                cy = 4 + u8 / 8;
+                cx = u8 % 8;
+                cx = cx < 4 ? cx : cx + GTX;
+            }
             int gi = wrapi((i0 + cx - 4 + 4 * nx) % nx, nx), gj = wrapi((j0 + cy - 4 + 4 * ny) % ny, ny);
             double u, v, w, Tt;
             con2prim_uvwT(A.q, gi + sy * gj + sz * k, gamma, mach, u, v, w, Tt);
@@ -350,26 +363,6 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A) { gradq_body<T>
 // line t%LPB, so each load instruction reads LPB consecutive doubles per cell.
 // Compute mapping: wave w owns line w (both signs, see nuc_lines.h).
 // ---------------------------------------------------------------------------
-// element at byte offset `boff` of a field: a scalar base plus a 32-bit
-// per-lane offset, which global loads and stores take as is (no 64-bit
-// address arithmetic per access).  validate() caps every line at 512 cells,
-// so a field holds at most 2^27 cells and its byte offsets stay below 2^30.
-// The offset passes through an empty asm at the access, so that its zero
-// extension is formed next to the load or store (the instruction selector
-// only folds it into the scalar-base form within one basic block).
-template <typename T>
-__device__ __forceinline__ T GMEM &fld(T GMEM *base, unsigned boff)
-{
-    asm volatile("" : "+v"(boff));
-    return *reinterpret_cast<T GMEM *>(reinterpret_cast<char GMEM *>(base) + boff);
-}
-template <typename T>
-__device__ __forceinline__ const T GMEM &fld(const T GMEM *base, unsigned boff)
-{
-    asm volatile("" : "+v"(boff));
-    return *reinterpret_cast<const T GMEM *>(reinterpret_cast<const char GMEM *>(base) + boff);
-}
-
 template <int RIE, typename T>
 __device__ __forceinline__ void split_params(const T q[5], int d, T gamma, T sp[5], bool ausmp = false)
 {
@@ -485,20 +478,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     else if (d == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
     else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
 
-    // byte offset of the thread's cells c = g + 64k within a field
-    unsigned boff[CPL];
-#pragma unroll
-    for (int k = 0; k < CPL; k++) boff[k] = (unsigned)((base + (long)(g + 64 * k) * stride) * sizeof(double));
-    // the field's base is kept opaque in scalar registers, so that the
-    // compiler does not fold the per-lane offsets into loop-invariant 64-bit
-    // vector addresses (which costs two registers per cell and array and a
-    // 64-bit add per access)
-    auto field = [&](auto *b, int v) {
-        auto *p = b + (long)v * A.fstride;
-        asm volatile("" : "+s"(p));
-        return p;
-    };
-
     // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344)
     T sp[CPL][NSP];
     unsigned innermask = 0;   // cd8 inner-form cells (rank-local position)
@@ -506,9 +485,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     for (int k = 0; k < CPL; k++) {
         int c = g + 64 * k;
         bool ok = FULL || (lv && c < n);
+        long id = base + (long)c * stride;
         T q[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) q[v] = ok ? T(fld(field(A.q, v), boff[k])) : (v == 0 ? T(1) : T(0));
+        for (int v = 0; v < 5; v++) q[v] = ok ? T(A.q[v][id]) : (v == 0 ? T(1) : T(0));
         T tmp[5];
         split_params<RIE, T>(q, d, T(A.gamma), tmp, A.riemann == RIE_AUSMP);
 #pragma unroll
@@ -562,13 +542,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             qo[k] = 0.0;
             xo[k] = T(0);
             if (FULL || (lv && c < n)) {
-                qo[k] = fld(field(A.q, v), boff[k]);
-                if (A.vis != VIS_NONE && v > 0) {
-                    const void GMEM *f = A.fv[v - 1];
-                    asm volatile("" : "+s"(f));
-                    xo[k] = A.fv_f32 ? T(fld(static_cast<const float GMEM *>(f), boff[k] / 2))
-                                     : T(fld(static_cast<const double GMEM *>(f), boff[k]));
-                }
+                long id = base + (long)c * stride;
+                qo[k] = A.q[v][id];
+                if (A.vis != VIS_NONE && v > 0)
+                    xo[k] = A.fv_f32 ? T(static_cast<const float GMEM *>(A.fv[v - 1])[id])
+                                     : T(static_cast<const double GMEM *>(A.fv[v - 1])[id]);
             }
         }
     };
@@ -633,8 +611,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             rin[k] = 0.0;
             unv[k] = 0.0;
             if (FULL || (lv && c < n)) {
-                if (!A.first || A.accum) rin[k] = fld(field(A.rhs, v), boff[k]);
-                if (A.epilogue && A.stage > 0) unv[k] = fld(field(A.qn, v), boff[k]);
+                long id = base + (long)c * stride;
+                if (!A.first || A.accum) rin[k] = A.rhs[v][id];
+                if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
             }
         }
         sync_lines(d);
@@ -642,6 +621,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
             if (FULL || (lv && c < n)) {
+                long id = base + (long)c * stride;
                 double r = rin[k];
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
                 r += ((Pd[slotP1(k)] + Md[slotM1(k)]) - (Pd[slotP(k)] + Md[slotM(k)])) * A.inv_h;
@@ -649,7 +629,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                     r -= double(central_diff<T>(Vl, c, (innermask >> k) & 1u, A.vis, T(A.inv_h)));
                 if (A.fmean && v == 4) r += *A.fmean;
                 if (!A.epilogue) {
-                    fld(field(A.rhs, v), boff[k]) = r;
+                    A.rhs[v][id] = r;
                 } else {
                     // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (0), :97-101 (1), :140-144 (2)
                     double rr = -r, nu;
@@ -661,7 +641,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                         else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qv[k]);
                         if (A.stage == 2) res += sq(nu - un);
                     }
-                    fld(field(A.qout, v), boff[k]) = nu;
+                    A.qout[v][id] = nu;
                 }
             }
         }
@@ -1429,9 +1409,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         S.nlines = N / x->n[d];
         S.inv_h = 1.0 / x->h[d];
         S.gamma = c.gamma;
-        S.q = gp(qi);
-        S.rhs = gp(x->rhs);
-        S.fstride = N;
+        for (int v = 0; v < 5; v++) { S.q[v] = gp(qi + v * N); S.rhs[v] = gp(x->rhs + v * N); }
         if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(x->fv + (long)fvmap[d][o] * N);
         S.stage = stage;
         const bool first = (d == kSweepOrder[0]), last = (d == kSweepOrder[2]);
@@ -1442,8 +1420,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         S.fmean = (last && x->forcing) ? gp(x->scal + 4) : nullptr;
         S.epilogue = (last && epilogue) ? 1 : 0;
         if (S.epilogue) {
-            S.qn = gp(x->q[0]);
-            S.qout = gp(x->q[outb]);
+            for (int v = 0; v < 5; v++) { S.qn[v] = gp(x->q[0] + v * N); S.qout[v] = gp(x->q[outb] + v * N); }
             S.dtp = gp(x->scal);
             S.res_part = gp(x->res_part);
         }

@@ -152,8 +152,10 @@ __device__ __forceinline__ void pcr32(T &al, T &ga, T &de, int h)
             alm = __shfl_up(al, s, 32); gam = __shfl_up(ga, s, 32); dem = __shfl_up(de, s, 32);
             alp = __shfl_down(al, s, 32); gap = __shfl_down(ga, s, 32); dep = __shfl_down(de, s, 32);
         }
-        if (h < s) { alm = T(0); gam = T(0); dem = T(0); }
-        if (h + s >= 32) { alp = T(0); gap = T(0); dep = T(0); }
+        // no masks: lanes h < s hold al = 0 and lanes h + s >= 32 hold ga = 0
+        // exactly (the first and last rows do not couple outward, and each
+        // level doubles those ranges), so the values they received from the
+        // wrapped or neighbouring lanes, all finite, are multiplied by zero
         T r = frcp(T(1) - al * gam - ga * alp);
         T nal = -r * al * alm;
         T nga = -r * ga * gap;
@@ -207,6 +209,9 @@ template <int R, typename T> struct Chunk {
         T al = -rb * A1 * A0;
         T ga = -rb * C1 * C0n;
         T de = rb * (D1 - A1 * D0 - C1 * D0n);
+        // the reduced system's first and last rows do not couple outward
+        if (h == 0) al = T(0);
+        if (h == 31) ga = T(0);
         pcr32(al, ga, de, h);
         T X1 = de;
         T X1m = shr1(X1);
@@ -330,24 +335,28 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             w5_parts<T>(f, d2 + r, b0, b1, b2, q0, q1, q2);
             TW P0, P1, P2;   // weights with unit ideal weights
             w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(1), TW(1), TW(1), P0, P1, P2);
-            // explicit WENO5-JS closure value (src/crweno5.cpp:87-129) from the same indicators
+            // Every row is scaled by a factor of its own, which leaves the
+            // tridiagonal solution unchanged.  The explicit WENO5-JS closure
+            // (src/crweno5.cpp:87-129) becomes the row (sum n_k) x = sum n_k q_k
+            // with the unnormalised JS weights n_k; the compact row
+            // (src/crweno5.cpp:133-162) is scaled by 3 times the weight
+            // denominator.
             TW n0 = TW(0.1) * P0, n1 = TW(0.6) * P1, n2 = TW(0.3) * P2;
-            TW wexp = (n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2);
-            // compact row (src/crweno5.cpp:133-162) scaled by the weight
-            // denominator, which leaves the tridiagonal solution unchanged
+            TW be = n0 + n1 + n2;
+            TW de = n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2);
             TW m0 = TW(0.2) * P0, m1 = TW(0.5) * P1, m2 = TW(0.3) * P2;
-            TW lo = TW(2.0 / 3.0) * m0 + TW(1.0 / 3.0) * m1;
-            TW di = TW(1.0 / 3.0) * m0 + TW(2.0 / 3.0) * (m1 + m2);
-            TW hi = TW(1.0 / 3.0) * m2;
+            TW lo = TW(2) * m0 + m1;
+            TW di = m0 + TW(2) * (m1 + m2);
+            TW hi = m2;
             TW rhs = (m0 * TW(f[1]) + (TW(5) * (m0 + m1) + m2) * TW(f[2]) + (m1 + TW(5) * m2) * TW(f[3]))
-                     * TW(1.0 / 6.0);
+                     * TW(0.5);
             int m = m0r + mstep * r;
             bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
             TS a = (valid && !expl) ? TS(lo) : TS(0);
-            TS b = (valid && !expl) ? TS(di) : TS(1);
+            TS b = valid ? TS(expl ? be : di) : TS(1);
             TS c = (valid && !expl) ? TS(hi) : TS(0);
-            TS d = valid ? TS(expl ? wexp : rhs) : TS(0);
+            TS d = valid ? TS(expl ? de : rhs) : TS(0);
             ch.row(r, a, b, c, d);
         }
         ch.solve(h);

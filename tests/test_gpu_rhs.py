@@ -192,7 +192,9 @@ def test_stats_match_oracle():
     scale[:14] = np.maximum(scale[:14], ref[18])
     err = np.abs(got - ref) / scale
     print("stats rel err", err, flush=True)
-    assert np.all(err <= 1e-9), err
+    # FP64 sums in another order: 5e-14 at worst (the skewness, a ratio of
+    # third and second moments of the gradients), 3e-15 elsewhere
+    assert np.all(err <= 1e-12), err
 
 
 def test_state_roundtrip_and_errors():
@@ -329,7 +331,14 @@ def test_long_lines_ten_steps_match_oracle(case):
 # sides (32 cells; 16 for the 512-cell lines) still resolve the spectrum, so
 # the nonlinear weights sit near their smooth values and the compact rows stay
 # diagonally dominant.  CRWENO5 (src/crweno5.cpp:83-220) with LF and cd8 in
-# each sweep direction; WENO5-Z (src/weno5z.cpp) with AUSM on 512-cell lines.
+# each sweep direction, at CFL 0.1; WENO5-Z (src/weno5z.cpp) with AUSM on
+# 512-cell lines at CFL 0.5.  At CFL 0.5 the CRWENO5 boxes with the long side
+# in x or y amplify a 1-ulp change of the initial state to 4.7e-13 and 4.5e-12
+# after 10 steps in the oracle itself (the reference algorithm's own floor,
+# measured with _ulp_floor), so 1e-12 cannot be a bar there for any
+# implementation short of the reference's exact arithmetic; at CFL 0.1 that
+# floor is 2e-15 and the bar holds with three orders of margin.  The CFL 0.5
+# long lines are in LONG_CASES, held to their measured floor.
 STRICT_CASES = [
     ((256, 32, 32), G.CRWENO5, G.CD8, G.LF),
     ((32, 256, 32), G.CRWENO5, G.CD8, G.LF),
@@ -343,7 +352,7 @@ STRICT_CASES = [
 def test_strict_bar_long_lines_decaying_turbulence(case):
     n, scheme, vis, rie = case
     print(f"strict long-line case {case}", flush=True)
-    cfl = 0.5
+    cfl = 0.1 if scheme == G.CRWENO5 else 0.5
     if rie == G.LF and n[2] > min(n[0], n[1]):
         cfl = 0.4 * min(n[0], n[1]) / n[2]   # LF's dt omits the z wave speed (src/physicsModel.cpp:653-654)
     gc, oc = _cfgs(n, (1, 1, 1), scheme, vis, rie, G.NAVIERSTOKES3D, mach=0.3 / np.sqrt(3.0), reynolds=200.0,
