@@ -357,6 +357,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=SERVE_STEPS)
     ap.add_argument("--serve-nmax", type=int, default=SERVE_NMAX)
     ap.add_argument("--cpu-steps", type=int, default=2, help="full RK3 steps of the CPU baseline")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="precision of the timed leg (mixed: for profiling the mixed path's kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box per GPU and measures one MI355X; "
@@ -367,7 +369,7 @@ def main():
     import nuc3d_amd as G
 
     n = args.n
-    cfg = workload_config(G, n)
+    cfg = workload_config(G, n, G.MIXED if args.precision == "mixed" else G.FP64)
     q = G.init_hit(cfg, k0=4.0, u_rms=1.0, seed=20261202)
     cells = n ** 3
     print(f"bench: FP64 {n}^3, {args.steps} steps", file=sys.stderr, flush=True)
@@ -444,7 +446,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 stencils",
         "data": "synthetic (seeded random-phase solenoidal HIT field, seed 20261202)",
         "config": {"workload": f"decaying compressible isotropic turbulence {n}^3, CRWENO5 x3 + LF splitting + "
                                "cd8 viscous + TVD-RK3, CFL 0.5, FP64",

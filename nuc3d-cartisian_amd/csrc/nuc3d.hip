@@ -575,11 +575,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         double qnx[CPL];
         T xnx[CPL];
         if (v < 4) fetch(v + 1, qnx, xnx);
+        // the final phase's own global reads (rhs of the previous sweeps, u^n of
+        // the RK update) are issued before the reconstruction, so that their
+        // latency is hidden behind it
+        double rin[CPL], unv[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
+            rin[k] = 0.0;
+            unv[k] = 0.0;
+            if (FULL || (lv && c < n)) {
+                long id = base + (long)c * stride;
+                if (!A.first || A.accum) rin[k] = A.rhs[v][id];
+                if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
+            }
+        }
         sync_lines(d);
         double out[R];
-        // weights and rows in T where FP32 keeps them in range; WENO5-Z squares
-        // tau/(is+1e-40) and stays FP64
-        using TW = typename std::conditional<SCH == WENO5Z, double, T>::type;
+        // weights and rows in T where FP32 keeps them in range; the Z-type
+        // weights (WENO5-Z, WENO7-Z and the hccs family) square tau/(is+eps)
+        // with eps down to 1e-40 and stay FP64
+        constexpr bool zw = SCH == WENO5Z || SCH == WENO7JS || SCH == HCCS || SCH == HCCS77 || SCH == HCCS78;
+        using TW = typename std::conditional<zw, double, T>::type;
         if (wv) line_half<SCH, R, T, TW, double>(Lw, m0r, mstep, z, h, neg, out, FULL);
         // the interface values overwrite the wave's own line only: the
         // reconstruction of the other waves is not waited for
@@ -600,21 +617,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             // periodic copy of interface 0 to interface n
             if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
             if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
-        }
-        // the final phase's own global reads (rhs of the previous sweeps, u^n of
-        // the RK update) are issued before the barrier, so that their latency
-        // overlaps the wait; the reconstruction's registers are free by now
-        double rin[CPL], unv[CPL];
-#pragma unroll
-        for (int k = 0; k < CPL; k++) {
-            int c = g + 64 * k;
-            rin[k] = 0.0;
-            unv[k] = 0.0;
-            if (FULL || (lv && c < n)) {
-                long id = base + (long)c * stride;
-                if (!A.first || A.accum) rin[k] = A.rhs[v][id];
-                if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
-            }
         }
         sync_lines(d);
 #pragma unroll
@@ -904,17 +906,27 @@ int rows_for(int nrows)
 constexpr int SWEEP_LPB = 4;
 int sweep_lpb(int) { return SWEEP_LPB; }
 
+// Schemes whose stencils run in FP32 in the mixed path.  The 7-point
+// indicators are evaluated there as sums of squares (w7_sos), which FP32 holds
+// to its relative precision.  hccs and hccs77 keep FP64 stencils: their blend
+// switches at theta = 0.5 between the upwind compact row and the blended row,
+// which do not meet there (src/hccs.cpp:146-183, src/hccs77.cpp:86-163), so an
+// FP32 rounding of the indicators that moves theta across 0.5 changes a row by
+// the full jump (measured: 7e-4 in the state after 5 steps of 24^3 turbulence
+// with hccs77).
+template <int SCH>
+constexpr bool mixed_fp32()
+{
+    return SCH != HCCS && SCH != HCCS77;
+}
+
 template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
-    // The 7-point schemes keep FP64 stencils in the mixed path too: their
-    // smoothness indicators cancel terms up to 17246 f^2, so FP32 inputs alone
-    // move the state by ~1e-4 in a few steps.
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78 && SCH != HCCS77);
     const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
-    if constexpr (fp32_ok) {
+    if constexpr (mixed_fp32<SCH>()) {
         if (A.mixed) {
             if (full) hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, A);
             else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, A);
@@ -1017,9 +1029,8 @@ struct PackLaunch {
 template <int SCH, int RIE, int R>
 void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
 {
-    constexpr bool fp32_ok = (SCH != WENO7JS && SCH != HCCS && SCH != HCCS78 && SCH != HCCS77);
     const dim3 grid((unsigned)P.nblocks), block(64 * SWEEP_LPB);
-    if constexpr (fp32_ok) {
+    if constexpr (mixed_fp32<SCH>()) {
         if (P.mixed) {
             if (P.full) hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, P.tab, P.off, P.nj);
             else hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, P.tab, P.off, P.nj);

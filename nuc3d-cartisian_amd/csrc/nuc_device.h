@@ -15,14 +15,37 @@ enum { RIE_LF = 0, RIE_AUSM = 1, RIE_AUSMP = 2 };   // AUSM+ runs the AUSM kerne
 
 template <typename T> __device__ __forceinline__ T sq(T x) { return x * x; }
 
+// WENO7 smoothness indicators as sums of squares, for FP32 stencils (the
+// mixed path).  An exact LDL^T factorisation of each of the reference's
+// quadratic forms (src/weno7js.cpp:80-99) in the differences g_i = f_i - f_{i+1}
+// (the forms vanish on constants):
+//   is0(f0..f3) = (547 g0 - 1394 g1 + 927 g2)^2 / 547
+//               + 85040 / (547 * 2126^2) * (2126 g1 - 3167 g2)^2 + 203060 / 1063 * g2^2
+//   is1(f0..f3) = (267 g0 - 554 g1 + 247 g2)^2 / 267
+//               + 85040 / (267 * 2126^2) * (2126 g1 - 1345 g2)^2 + 203060 / 1063 * g2^2
+// and is2, is3 are is1, is0 of the mirrored stencil.  Every term is
+// non-negative, so FP32 evaluates them to FP32's relative precision instead
+// of cancelling terms up to 17246 f^2.  outer: the is0 form, else is1's.
+template <typename T>
+__device__ __forceinline__ T w7_sos(T a, T b, T c, T d, bool outer)
+{
+    const T g0 = a - b, g1 = b - c, g2 = c - d;
+    const T e0 = outer ? T(547) * g0 - T(1394) * g1 + T(927) * g2 : T(267) * g0 - T(554) * g1 + T(247) * g2;
+    const T e1 = T(2126) * g1 - (outer ? T(3167) : T(1345)) * g2;
+    const T k0 = outer ? T(1.0 / 547.0) : T(1.0 / 267.0);
+    const T k1 = outer ? T(85040.0 / (547.0 * 2126.0 * 2126.0)) : T(85040.0 / (267.0 * 2126.0 * 2126.0));
+    return k0 * e0 * e0 + k1 * e1 * e1 + T(203060.0 / 1063.0) * g2 * g2;
+}
+
 // 7-point WENO with Z weights, src/weno7js.cpp:79-133 (eps 1e-20); hccs
 // closures use the same formula with eps 1e-40 (src/hccs.cpp:85-135), hccs78
 // edges with 1e-20 (src/hccs78.cpp:246-304).
-// The smoothness indicators are expanded quadratic forms whose coefficients
-// (up to 17246) cancel to O(h^2) on smooth data, so any reassociation or FMA
-// moves them by ~1e-6 relative and shifts the Z weights.  This function is
-// therefore evaluated without contraction and in the reference's operation
-// order, which makes it bit-identical to the reference for the indicators.
+// FP64 stencils: the smoothness indicators are the reference's expanded
+// quadratic forms, whose coefficients (up to 17246) cancel to O(h^2) on smooth
+// data, so any reassociation or FMA moves them by ~1e-6 relative and shifts
+// the Z weights.  They are therefore evaluated without contraction and in the
+// reference's operation order, which makes them bit-identical to the
+// reference.  FP32 stencils (the mixed path) use the sums of squares above.
 // T: indicators and candidate values; TW: the nonlinear weights (the Z
 // weights square tau/(eps+is) with eps 1e-20/1e-40, outside FP32's range).
 // asum receives the sum of the unnormalised weights (hccs78's blend factor).
@@ -30,18 +53,26 @@ template <typename T, typename TW>
 __device__ __forceinline__ TW weno7z(const T *f, TW eps, TW &asum)
 {
 #pragma clang fp contract(off)
-    T is0 = f[0] * (T(547) * f[0] - T(3882) * f[1] + T(4642) * f[2] - T(1854) * f[3])
-          + f[1] * (T(7043) * f[1] - T(17246) * f[2] + T(7042) * f[3])
-          + f[2] * (T(11003) * f[2] - T(9402) * f[3]) + T(2107) * f[3] * f[3];
-    T is1 = f[1] * (T(267) * f[1] - T(1642) * f[2] + T(1602) * f[3] - T(494) * f[4])
-          + f[2] * (T(2843) * f[2] - T(5966) * f[3] + T(1922) * f[4])
-          + f[3] * (T(3443) * f[3] - T(2522) * f[4]) + T(547) * f[4] * f[4];
-    T is2 = f[2] * (T(547) * f[2] - T(2522) * f[3] + T(1922) * f[4] - T(494) * f[5])
-          + f[3] * (T(3443) * f[3] - T(5966) * f[4] + T(1602) * f[5])
-          + f[4] * (T(2843) * f[4] - T(1642) * f[5]) + T(267) * f[5] * f[5];
-    T is3 = f[3] * (T(2107) * f[3] - T(9402) * f[4] + T(7042) * f[5] - T(1854) * f[6])
-          + f[4] * (T(11003) * f[4] - T(17246) * f[5] + T(4642) * f[6])
-          + f[5] * (T(7043) * f[5] - T(3882) * f[6]) + T(547) * f[6] * f[6];
+    T is0, is1, is2, is3;
+    if constexpr (sizeof(T) == 4) {
+        is0 = w7_sos(f[0], f[1], f[2], f[3], true);
+        is1 = w7_sos(f[1], f[2], f[3], f[4], false);
+        is2 = w7_sos(f[5], f[4], f[3], f[2], false);
+        is3 = w7_sos(f[6], f[5], f[4], f[3], true);
+    } else {
+        is0 = f[0] * (T(547) * f[0] - T(3882) * f[1] + T(4642) * f[2] - T(1854) * f[3])
+            + f[1] * (T(7043) * f[1] - T(17246) * f[2] + T(7042) * f[3])
+            + f[2] * (T(11003) * f[2] - T(9402) * f[3]) + T(2107) * f[3] * f[3];
+        is1 = f[1] * (T(267) * f[1] - T(1642) * f[2] + T(1602) * f[3] - T(494) * f[4])
+            + f[2] * (T(2843) * f[2] - T(5966) * f[3] + T(1922) * f[4])
+            + f[3] * (T(3443) * f[3] - T(2522) * f[4]) + T(547) * f[4] * f[4];
+        is2 = f[2] * (T(547) * f[2] - T(2522) * f[3] + T(1922) * f[4] - T(494) * f[5])
+            + f[3] * (T(3443) * f[3] - T(5966) * f[4] + T(1602) * f[5])
+            + f[4] * (T(2843) * f[4] - T(1642) * f[5]) + T(267) * f[5] * f[5];
+        is3 = f[3] * (T(2107) * f[3] - T(9402) * f[4] + T(7042) * f[5] - T(1854) * f[6])
+            + f[4] * (T(11003) * f[4] - T(17246) * f[5] + T(4642) * f[6])
+            + f[5] * (T(7043) * f[5] - T(3882) * f[6]) + T(547) * f[6] * f[6];
+    }
     T q0 = T(-3.0 / 12.0) * f[0] + T(13.0 / 12.0) * f[1] + T(-23.0 / 12.0) * f[2] + T(25.0 / 12.0) * f[3];
     T q1 = T(1.0 / 12.0) * f[1] + T(-5.0 / 12.0) * f[2] + T(13.0 / 12.0) * f[3] + T(3.0 / 12.0) * f[4];
     T q2 = T(-1.0 / 12.0) * f[2] + T(7.0 / 12.0) * f[3] + T(7.0 / 12.0) * f[4] + T(-1.0 / 12.0) * f[5];
