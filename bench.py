@@ -29,11 +29,14 @@ steps of the same 256^3 workload, run as that many emulated MPI ranks (the
 box cut into the reference's rank blocks, one thread per rank's share of the
 line loops).
 
-mixed: the same workload with NUC_MIXED (FP32 split fluxes, reconstruction and
-viscous stencils; FP64 weights, tridiagonal solves, RK3 and reductions), timed
-the same way, with the relative error of every global statistic (kinetic
-energy, dissipation, enstrophy, u_rms, Re_lambda, ...) against the FP64 run
-from the same initial state after the same steps.
+mixed: the same workload with NUC_MIXED (FP32 split fluxes, stencils, CRWENO5's
+nonlinear weights and viscous stencils; FP64 tridiagonal solves, RK3 and
+reductions), timed the same way, with the relative error of every global
+statistic (kinetic energy, dissipation, enstrophy, u_rms, Re_lambda, ...)
+against the FP64 run from the same initial state after the same steps.
+
+weno5js: the same box and initial state with WENO5-JS in place of CRWENO5
+(FP64 and mixed), timed the same way; a secondary figure.
 
 serving: the serving stream -- 256 independent boxes (Taylor-Green or decaying
 turbulence, N log-uniform in [32, 192] per side, Mach U(0.1,0.5),
@@ -57,9 +60,10 @@ HBM_PEAK_GBS = 8000.0
 B_ALG_STAGE = 320.0 / 3.0   # bytes per cell per RK stage
 
 
-def workload_config(G, n=256, precision=None):
+def workload_config(G, n=256, precision=None, scheme=None):
     import numpy as np
-    return G.make_config(n=(n, n, n), npr=(1, 1, 1), scheme=G.CRWENO5, vis=G.CD8, riemann=G.LF,
+    return G.make_config(n=(n, n, n), npr=(1, 1, 1), scheme=G.CRWENO5 if scheme is None else scheme,
+                         vis=G.CD8, riemann=G.LF,
                          model=G.NAVIERSTOKES3D, viscosity=G.SUTHERLAND, mach=0.3 / np.sqrt(3.0), reynolds=200.0,
                          prandtl=0.72, gamma=1.4, cfl=0.5, precision=G.FP64 if precision is None else precision)
 
@@ -352,6 +356,7 @@ def main():
     ap.add_argument("--profile-slots", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-weno5", action="store_true", help="skip the WENO5-JS secondary figure")
     ap.add_argument("--serve-jobs", type=int, default=SERVE_JOBS,
                     help="jobs of the serving stream (default: the full 256-job stream)")
     ap.add_argument("--serve-steps", type=int, default=SERVE_STEPS)
@@ -422,6 +427,23 @@ def main():
             for k, (t, c) in mprof.items():
                 print(f"  mixed {k:10s} avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
 
+    # secondary figure: the same box and initial state with WENO5-JS (explicit
+    # five-point reconstruction, no tridiagonal solve) in place of CRWENO5
+    weno5 = None
+    if not args.no_weno5:
+        print("bench: WENO5-JS", file=sys.stderr, flush=True)
+        weno5 = {"config": f"decaying compressible isotropic turbulence {n}^3, WENO5-JS x3 + LF splitting + "
+                           "cd8 viscous + TVD-RK3, CFL 0.5"}
+        for label, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
+            if label == "mixed" and args.no_mixed:
+                continue
+            wsecs, _, wprof, _, _ = stage_stats(G, workload_config(G, n, prec, G.WENO5JS), q, args.steps,
+                                                args.warmup)
+            wname, (wtot, wl) = max(wprof.items(), key=lambda kv: kv[1][0])
+            weno5[label] = {"value": round(cells * 3 * args.steps / wsecs / 1e6, 3),
+                            "ms_per_step": round(wsecs / args.steps * 1e3, 4),
+                            "dominant_kernel": wname, "avg_launch_ms": round(wtot / max(wl, 1), 4)}
+
     serving = None
     if not args.no_serving:
         print("bench: serving stream", file=sys.stderr, flush=True)
@@ -458,6 +480,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "weno5js": weno5,
         "serving": serving,
     }
     print(json.dumps(out))

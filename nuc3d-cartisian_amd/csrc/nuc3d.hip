@@ -420,6 +420,29 @@ __device__ __forceinline__ void split_var(const T *sp, T qv, int v, int d, T &fp
     }
 }
 
+// Element of a field at a 32-bit byte offset, through a buffer resource
+// (base address and size in scalar registers): the offset is the only
+// per-lane operand, and an offset past the field's `bytes` reads 0 and drops
+// the store.  Word 3 (0x00020000) selects 32-bit data, the raw-buffer form.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t field_rsrc(const void GMEM *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(reinterpret_cast<const void *>(p)), 0, (int)bytes,
+                                             0x00020000);
+}
+__device__ __forceinline__ double ld_at(const double GMEM *p, unsigned ob, unsigned bytes)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(field_rsrc(p, bytes), ob, 0, 0));
+}
+__device__ __forceinline__ float ld_at(const float GMEM *p, unsigned ob, unsigned bytes)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(field_rsrc(p, bytes), ob, 0, 0));
+}
+__device__ __forceinline__ void st_at(double GMEM *p, unsigned ob, unsigned bytes, double v)
+{
+    using u2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), field_rsrc(p, bytes), ob, 0, 0);
+}
+
 // Workgroup index with consecutive indices placed on the same XCD: the
 // dispatcher deals workgroups round-robin over the 8 XCDs (each with its own
 // L2), so physical block p runs on XCD p%8; giving that XCD the contiguous
@@ -478,21 +501,30 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     else if (d == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
     else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
 
-    // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344)
-    T sp[CPL][NSP];
+    // byte offsets of the thread's cells within a field: a field holds at
+    // most 512^3 doubles (1 GiB), so 32 bits suffice, and every access is the
+    // field's base address (uniform, in scalar registers) plus this offset
+    unsigned ob[CPL];
+#pragma unroll
+    for (int k = 0; k < CPL; k++) ob[k] = (unsigned)(base + (long)(g + 64 * k) * stride) * 8u;
+    const unsigned fb = (unsigned)((long)A.nlines * n * 8);   // bytes of one FP64 field
+
+    // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344),
+    // kept in LDS (one column per thread, read back by the same thread) rather
+    // than in CPL*NSP registers held across every variable's reconstruction
+    __shared__ T sSP[CPL * NSP][64 * LPB];
     unsigned innermask = 0;   // cd8 inner-form cells (rank-local position)
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
         int c = g + 64 * k;
         bool ok = FULL || (lv && c < n);
-        long id = base + (long)c * stride;
         T q[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) q[v] = ok ? T(A.q[v][id]) : (v == 0 ? T(1) : T(0));
+        for (int v = 0; v < 5; v++) q[v] = ok ? T(ld_at(A.q[v], ob[k], fb)) : (v == 0 ? T(1) : T(0));
         T tmp[5];
         split_params<RIE, T>(q, d, T(A.gamma), tmp, A.riemann == RIE_AUSMP);
 #pragma unroll
-        for (int j = 0; j < NSP; j++) sp[k][j] = tmp[j];
+        for (int j = 0; j < NSP; j++) sSP[k * NSP + j][t] = tmp[j];
         int cl = c % A.nloc;
         if (cl >= A.tsv && cl < A.nloc - A.tsv) innermask |= 1u << k;
     }
@@ -542,11 +574,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             qo[k] = 0.0;
             xo[k] = T(0);
             if (FULL || (lv && c < n)) {
-                long id = base + (long)c * stride;
-                qo[k] = A.q[v][id];
+                qo[k] = ld_at(A.q[v], ob[k], fb);
                 if (A.vis != VIS_NONE && v > 0)
-                    xo[k] = A.fv_f32 ? T(static_cast<const float GMEM *>(A.fv[v - 1])[id])
-                                     : T(static_cast<const double GMEM *>(A.fv[v - 1])[id]);
+                    xo[k] = A.fv_f32 ? T(ld_at(static_cast<const float GMEM *>(A.fv[v - 1]), ob[k] >> 1, fb >> 1))
+                                     : T(ld_at(static_cast<const double GMEM *>(A.fv[v - 1]), ob[k], fb));
             }
         }
     };
@@ -558,7 +589,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             int c = g + 64 * k;
             if (FULL || (lv && c < n)) {
                 T fp, fm;
-                split_var<RIE, T>(sp[k], T(qv[k]), v, d, fp, fm);
+                T spk[NSP];
+#pragma unroll
+                for (int j = 0; j < NSP; j++) spk[j] = sSP[k * NSP + j][t];
+                split_var<RIE, T>(spk, T(qv[k]), v, d, fp, fm);
                 Pl[slotP(k)] = fp; Ml[slotM(k)] = fm;
                 if (A.vis != VIS_NONE && v > 0) Vl[c] = xv[k];
                 // the cell's periodic image: cells within HALO of an end are
@@ -575,21 +609,6 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         double qnx[CPL];
         T xnx[CPL];
         if (v < 4) fetch(v + 1, qnx, xnx);
-        // the final phase's own global reads (rhs of the previous sweeps, u^n of
-        // the RK update) are issued before the reconstruction, so that their
-        // latency is hidden behind it
-        double rin[CPL], unv[CPL];
-#pragma unroll
-        for (int k = 0; k < CPL; k++) {
-            int c = g + 64 * k;
-            rin[k] = 0.0;
-            unv[k] = 0.0;
-            if (FULL || (lv && c < n)) {
-                long id = base + (long)c * stride;
-                if (!A.first || A.accum) rin[k] = A.rhs[v][id];
-                if (A.epilogue && A.stage > 0) unv[k] = A.qn[v][id];
-            }
-        }
         sync_lines(d);
         double out[R];
         // weights and rows in T where FP32 keeps them in range; the Z-type
@@ -618,12 +637,28 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
             if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
         }
+        // the final phase's own global reads (rhs of the previous sweeps; u^n and
+        // the current state of the RK update), issued after the reconstruction:
+        // held across it they would cost 3*CPL registers, and the other waves
+        // of the SIMD cover their latency
+        double rin[CPL], unv[CPL], qcur[CPL];
+#pragma unroll
+        for (int k = 0; k < CPL; k++) {
+            int c = g + 64 * k;
+            rin[k] = 0.0;
+            unv[k] = 0.0;
+            qcur[k] = 0.0;
+            if (FULL || (lv && c < n)) {
+                if (!A.first || A.accum) rin[k] = ld_at(A.rhs[v], ob[k], fb);
+                if (A.epilogue && A.stage > 0) unv[k] = ld_at(A.qn[v], ob[k], fb);
+                if (A.epilogue) qcur[k] = ld_at(A.q[v], ob[k], fb);
+            }
+        }
         sync_lines(d);
 #pragma unroll
         for (int k = 0; k < CPL; k++) {
             int c = g + 64 * k;
             if (FULL || (lv && c < n)) {
-                long id = base + (long)c * stride;
                 double r = rin[k];
                 // F = f^+ + f^- at the interfaces, then (F[i+1]-F[i])/dx (src/euler3d.cpp:23-56,132-166)
                 r += ((Pd[slotP1(k)] + Md[slotM1(k)]) - (Pd[slotP(k)] + Md[slotM(k)])) * A.inv_h;
@@ -631,19 +666,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                     r -= double(central_diff<T>(Vl, c, (innermask >> k) & 1u, A.vis, T(A.inv_h)));
                 if (A.fmean && v == 4) r += *A.fmean;
                 if (!A.epilogue) {
-                    A.rhs[v][id] = r;
+                    st_at(A.rhs[v], ob[k], fb, r);
                 } else {
                     // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (0), :97-101 (1), :140-144 (2)
                     double rr = -r, nu;
                     if (A.stage == 0) {
-                        nu = qv[k] + rr * dt;
+                        nu = qcur[k] + rr * dt;
                     } else {
                         double un = unv[k];
-                        if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + qv[k]);
-                        else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qv[k]);
+                        if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + qcur[k]);
+                        else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qcur[k]);
                         if (A.stage == 2) res += sq(nu - un);
                     }
-                    A.qout[v][id] = nu;
+                    st_at(A.qout[v], ob[k], fb, nu);
                 }
             }
         }

@@ -135,6 +135,11 @@ __device__ __forceinline__ void swap16(float x, float &up, float &dn)
     up = __int_as_float((int)u); dn = __int_as_float((int)d);
 }
 
+// An empty asm that takes x and gives it back: the value must exist at this
+// point, and code that uses the result cannot move above it (volatile asm
+// statements keep their order).  Used to evaluate a lane's rows in order.
+template <typename T> __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
+
 // normalised parallel cyclic reduction over the 32 lanes of a half wave:
 // row k: x_k + al*x_{k-1} + ga*x_{k+1} = de  ->  x_k = de on return
 template <typename T>
@@ -267,7 +272,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
 #pragma unroll
     for (int t = 0; t < W; t++) win[t] = Lw[Skew<R>::at(t - RAD - 1)];
     T d2[W];
-    if constexpr (SCH == WENO5JS || SCH == WENO5Z || SCH == CRWENO5) w5_d2<W, T>(win, d2);
+    if constexpr (SCH == WENO5JS || SCH == WENO5Z) w5_d2<4, T>(win, d2);
     // rank-local position of each row, without integer division in the loop
     int ml[R];
     {
@@ -282,21 +287,43 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         }
     }
     if constexpr (SCH == WENO5JS) {
+        // rows in order through a rolling window (see CRWENO5 below)
+        T fw[5], e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) fw[j] = win[j];
+        e[1] = d2[1]; e[2] = d2[2];
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            fw[4] = win[r + 4];
+            e[3] = T(13.0 / 12.0) * sq(fw[2] - T(2) * fw[3] + fw[4]);
             T b0, b1, b2, q0, q1, q2;
-            w5_parts<T>(win + r, d2 + r, b0, b1, b2, q0, q1, q2);
+            w5_parts<T>(fw, e, b0, b1, b2, q0, q1, q2);
             // alpha_k = c_k / s_k, normalised with a single reciprocal
             TW n0, n1, n2;
             w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(0.1), TW(0.6), TW(0.3), n0, n1, n2);
-            out[r] = TS((n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2));
+            TS o = TS((n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2));
+            pin(o);
+#pragma unroll
+            for (int j = 1; j < 5; j++) pin(fw[j]);
+            pin(e[2]); pin(e[3]);
+            out[r] = o;
+#pragma unroll
+            for (int j = 0; j < 4; j++) fw[j] = fw[j + 1];
+            e[1] = e[2]; e[2] = e[3];
         }
     } else if constexpr (SCH == WENO5Z) {
+        // rows in order through a rolling window (see CRWENO5 below)
+        T fw[5], e[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) fw[j] = win[j];
+        e[1] = d2[1]; e[2] = d2[2];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const T *f = win + r;
+            fw[4] = win[r + 4];
+            e[3] = T(13.0 / 12.0) * sq(fw[2] - T(2) * fw[3] + fw[4]);
+            const T *f = fw;
             T b0, b1, b2, q0, q1, q2;
-            w5_parts<T>(f, d2 + r, b0, b1, b2, q0, q1, q2);
+            w5_parts<T>(f, e, b0, b1, b2, q0, q1, q2);
             // the positive inner sweep's is2 reads f_{i+1} twice (src/weno5z.cpp:80)
             bool typo = !neg && ml[r] >= z.ts && ml[r] <= z.nloc - z.ts;
             if (typo) b2 = T(13.0 / 12.0) * sq(f[2] - f[3]) + T(0.25) * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
@@ -306,11 +333,32 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TW a0 = TW(0.1) * (TW(1) + sq(tau / (B0 + eps)));
             TW a1 = TW(0.6) * (TW(1) + sq(tau / (B1 + eps)));
             TW a2 = TW(0.3) * (TW(1) + sq(tau / (B2 + eps)));
-            out[r] = TS((a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2));
+            TS o = TS((a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2));
+            pin(o);
+#pragma unroll
+            for (int j = 1; j < 5; j++) pin(fw[j]);
+            pin(e[2]); pin(e[3]);
+            out[r] = o;
+#pragma unroll
+            for (int j = 0; j < 4; j++) fw[j] = fw[j + 1];
+            e[1] = e[2]; e[2] = e[3];
         }
     } else if constexpr (SCH == WENO7JS) {
+        // rows in order through a rolling 7-point window (see CRWENO5 below)
+        T fw[7];
 #pragma unroll
-        for (int r = 0; r < R; r++) out[r] = TS(weno7z<T, TW>(win + r, TW(1.0e-20)));
+        for (int j = 0; j < 6; j++) fw[j] = win[j];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            fw[6] = win[r + 6];
+            TS o = TS(weno7z<T, TW>(fw, TW(1.0e-20)));
+            pin(o);
+#pragma unroll
+            for (int j = 1; j < 7; j++) pin(fw[j]);
+            out[r] = o;
+#pragma unroll
+            for (int j = 0; j < 6; j++) fw[j] = fw[j + 1];
+        }
     } else if constexpr (SCH == WENO3JS) {
 #pragma unroll
         for (int r = 0; r < R; r++) out[r] = TS(weno3js<T, TW>(win + r));
@@ -321,18 +369,42 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
     } else if constexpr (SCH == HCCS78) {
         // blend on the inner interfaces ts..nloc-ts (src/hccs78.cpp:56-60),
         // WENO7-Z on the edge interfaces (:207-304)
+        T fw[8];
+#pragma unroll
+        for (int j = 0; j < 7; j++) fw[j] = win[j];
 #pragma unroll
         for (int r = 0; r < R; r++) {
+            fw[7] = win[r + 7];
             bool edge = ml[r] < z.ts || ml[r] > z.nloc - z.ts;
-            out[r] = edge ? TS(weno7z<T, TW>(win + r, TW(1.0e-20))) : TS(hccs78_inner<T, TW>(win + r));
+            TS o = edge ? TS(weno7z<T, TW>(fw, TW(1.0e-20))) : TS(hccs78_inner<T, TW>(fw));
+            pin(o);
+#pragma unroll
+            for (int j = 1; j < 8; j++) pin(fw[j]);
+            out[r] = o;
+#pragma unroll
+            for (int j = 0; j < 7; j++) fw[j] = fw[j + 1];
         }
     } else if constexpr (SCH == CRWENO5) {
         Chunk<R, TS> ch;
+        // Rows in order: a rolling window f0..f4 and second-difference terms
+        // e1..e3 of the current row, where the values the next row reuses and
+        // the row's own coefficients pass through an empty asm.  Left free,
+        // the compiler evaluates every row's weights before the first
+        // elimination step and holds about ten values per row (233 registers,
+        // 2 waves per SIMD, against 164 and 3 waves in order).
+        T f0 = win[0], f1 = win[1], f2 = win[2], f3 = win[3];
+        T e1 = T(13.0 / 12.0) * sq(f0 - T(2) * f1 + f2), e2 = T(13.0 / 12.0) * sq(f1 - T(2) * f2 + f3);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const T *f = win + r;
-            T b0, b1, b2, q0, q1, q2;
-            w5_parts<T>(f, d2 + r, b0, b1, b2, q0, q1, q2);
+            T f4 = win[r + 4];
+            T e3 = T(13.0 / 12.0) * sq(f2 - T(2) * f3 + f4);
+            const T g1 = T(0.25);
+            T b0 = e1 + g1 * sq(f0 - T(4) * f1 + T(3) * f2);
+            T b1 = e2 + g1 * sq(f1 - f3);
+            T b2 = e3 + g1 * sq(T(3) * f2 - T(4) * f3 + f4);
+            T q0 = T(1.0 / 3.0) * f0 + T(-7.0 / 6.0) * f1 + T(11.0 / 6.0) * f2;
+            T q1 = T(-1.0 / 6.0) * f1 + T(5.0 / 6.0) * f2 + T(1.0 / 3.0) * f3;
+            T q2 = T(1.0 / 3.0) * f2 + T(5.0 / 6.0) * f3 + T(-1.0 / 6.0) * f4;
             TW P0, P1, P2;   // weights with unit ideal weights
             w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(1), TW(1), TW(1), P0, P1, P2);
             // Every row is scaled by a factor of its own, which leaves the
@@ -348,8 +420,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TW lo = TW(2) * m0 + m1;
             TW di = m0 + TW(2) * (m1 + m2);
             TW hi = m2;
-            TW rhs = (m0 * TW(f[1]) + (TW(5) * (m0 + m1) + m2) * TW(f[2]) + (m1 + TW(5) * m2) * TW(f[3]))
-                     * TW(0.5);
+            TW rhs = (m0 * TW(f1) + (TW(5) * (m0 + m1) + m2) * TW(f2) + (m1 + TW(5) * m2) * TW(f3)) * TW(0.5);
             int m = m0r + mstep * r;
             bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
@@ -357,7 +428,10 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TS b = valid ? TS(expl ? be : di) : TS(1);
             TS c = (valid && !expl) ? TS(hi) : TS(0);
             TS d = valid ? TS(expl ? de : rhs) : TS(0);
+            asm volatile("" : "+v"(f1), "+v"(f2), "+v"(f3), "+v"(f4), "+v"(e2), "+v"(e3),
+                              "+v"(a), "+v"(b), "+v"(c), "+v"(d));
             ch.row(r, a, b, c, d);
+            f0 = f1; f1 = f2; f2 = f3; f3 = f4; e1 = e2; e2 = e3;
         }
         ch.solve(h);
 #pragma unroll
@@ -366,9 +440,13 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         // compact rows on ts+1..nloc-ts-1, WENO7-Z rows at ts and nloc-ts and
         // on the edges (src/hccs77.cpp:86-163, :349-455)
         Chunk<R, TS> ch;
+        T fw[7];
+#pragma unroll
+        for (int j = 0; j < 6; j++) fw[j] = win[j];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const T *f = win + r;
+            fw[6] = win[r + 6];
+            const T *f = fw;
             int m = m0r + mstep * r;
             bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
@@ -377,16 +455,26 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                 if (expl) d = weno7z<T, TW>(f, TW(1.0e-20));
                 else hccs77_row<T, TW>(f, a, b, c, d);
             }
-            ch.row(r, TS(a), TS(b), TS(c), TS(d));
+            TS as = TS(a), bs = TS(b), cs = TS(c), ds = TS(d);
+            pin(as); pin(bs); pin(cs); pin(ds);
+#pragma unroll
+            for (int j = 1; j < 7; j++) pin(fw[j]);
+            ch.row(r, as, bs, cs, ds);
+#pragma unroll
+            for (int j = 0; j < 6; j++) fw[j] = fw[j + 1];
         }
         ch.solve(h);
 #pragma unroll
         for (int r = 0; r < R; r++) out[r] = ch.Df[r];
     } else {  // HCCS
         Chunk<R, TS> ch;
+        T fw[7];   // 7-point window for the closures, 5-point from fw+1
+#pragma unroll
+        for (int j = 0; j < 6; j++) fw[j] = win[j];
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            const T *f = win + r;   // 7-point window for the closures, 5-point from f+1
+            fw[6] = win[r + 6];
+            const T *f = fw;
             int m = m0r + mstep * r;
             bool valid = full || m < z.nrows;
             bool expl = ml[r] <= z.ts || ml[r] >= z.nloc - z.ts;
@@ -398,7 +486,13 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                     hccs_row<T, TW>(f[1], f[2], f[3], f[4], f[5], a, b, c, d);
                 }
             }
-            ch.row(r, TS(a), TS(b), TS(c), TS(d));
+            TS as = TS(a), bs = TS(b), cs = TS(c), ds = TS(d);
+            pin(as); pin(bs); pin(cs); pin(ds);
+#pragma unroll
+            for (int j = 1; j < 7; j++) pin(fw[j]);
+            ch.row(r, as, bs, cs, ds);
+#pragma unroll
+            for (int j = 0; j < 6; j++) fw[j] = fw[j + 1];
         }
         ch.solve(h);
 #pragma unroll

@@ -202,3 +202,43 @@ def test_jobs_with_device_initial_conditions():
             ref = ctx.download()
         assert res[1].state == G.JOB_DONE
         assert np.array_equal(res[0], ref)
+
+
+def test_serving_stream_slice_identical_alone():
+    """Jobs of the benchmark's serving stream (its own generator and seed: the
+    boxes of at most 48^3 among its 256 jobs, plus a 64^3 one) through a
+    server configured as the benchmark configures it, so that packs form; in
+    FP64 and mixed precision every result equals the job run alone."""
+    import os
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if root not in sys.path:
+        sys.path.insert(0, root)
+    import bench
+    jobs, _ = bench.serving_jobs(G, bench.SERVE_JOBS, bench.SERVE_NMIN, bench.SERVE_NMAX, bench.SERVE_RATE,
+                                 bench.SERVE_SEED)
+    sl = [j for j in jobs if j[4] <= 48][:8] + [j for j in jobs if j[4] == 64][:1]
+    assert len(sl) == 9
+    steps = 4
+    for label, prec in (("fp64", G.FP64), ("mixed", G.MIXED)):
+        print("serving stream slice", label, [j[4] for j in sl], flush=True)
+        stream = bench.with_precision(sl, prec)
+        with G.Server(hbm_budget=bench.SERVE_HBM_GIB << 30, max_active=bench.SERVE_MAX_ACTIVE, steps_per_slice=2,
+                      pack_cells=bench.SERVE_PACK_CELLS, autotune=True) as srv:
+            ids = [srv.submit_ic(cfg, ic, steps, k0=4.0, u_rms=u_rms, seed=seed)
+                   for cfg, ic, u_rms, seed, _ in stream]
+            out = [_wait(srv, jid) for jid in ids]
+            st = srv.stats()
+        print("server stats", st, flush=True)
+        assert st["completed"] == len(sl) and st["failed"] == 0
+        assert st["pack_slices"] > 0
+        for (cfg, ic, u_rms, seed, n), (state, r) in zip(stream, out):
+            with G.Context(cfg) as ctx:
+                if ic == G.IC_HIT:
+                    ctx.init_hit_device(k0=4.0, u_rms=u_rms, seed=seed)
+                else:
+                    ctx.init_tgv_device()
+                ctx.step(steps)
+                ref = ctx.download()
+            assert r.state == G.JOB_DONE and r.info.step == steps
+            assert np.array_equal(state, ref), (label, n)
