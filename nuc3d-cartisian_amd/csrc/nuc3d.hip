@@ -66,6 +66,9 @@ __host__ __device__ inline T GMEM *gp(T *p)
     return (T GMEM *)p;
 }
 
+// whether a sweep applies the RK3 update: never, always, or as A.epilogue says
+enum { EPI_NO = 0, EPI_YES = 1, EPI_ARG = 2 };
+
 struct SweepArgs {
     int dir, n, nloc, ts, tsv, vis, riemann;
     int nx, ny, nz;
@@ -474,10 +477,16 @@ __device__ __forceinline__ void sync_lines(int d)
 // for the mixed path); weights, tridiagonal solves, interface values, the
 // flux difference and the RHS/RK3 accumulation are FP64 in both.
 // FULL: n = 32R and every workgroup's lines exist, so no bounds checks.
+// EPI: EPI_NO / EPI_YES fix at compile time whether this sweep applies the
+// RK3 update (the last sweep of a stage); EPI_ARG reads it from A.epilogue.
+// The last sweep keeps the current state's values in registers from the load
+// phase to the update; the other forms read them again in the final phase,
+// so that the first two sweeps do not hold them across the reconstruction.
 // blk: the workgroup's logical index within its job (line group blk*LPB ..)
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 {
+    const bool epi = EPI == EPI_ARG ? A.epilogue != 0 : EPI == EPI_YES;
     constexpr int NP = Skew<R>::pitch(32 * R);
     constexpr int NV = plain_pitch(32 * R);
     constexpr int CPL = R / 2;             // cells per thread in the load mapping
@@ -560,7 +569,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     const int m0r = neg ? m0 + R - 1 : m0;
     const int mstep = neg ? -1 : 1;
     double res = 0.0;
-    const double dt = A.epilogue ? *A.dtp : 0.0;
+    const double dt = epi ? *A.dtp : 0.0;
 
     // global loads of variable v (state and viscous flux) are issued one
     // variable ahead, so that they are in flight during the previous
@@ -650,8 +659,9 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             qcur[k] = 0.0;
             if (FULL || (lv && c < n)) {
                 if (!A.first || A.accum) rin[k] = ld_at(A.rhs[v], ob[k], fb);
-                if (A.epilogue && A.stage > 0) unv[k] = ld_at(A.qn[v], ob[k], fb);
-                if (A.epilogue) qcur[k] = ld_at(A.q[v], ob[k], fb);
+                if (epi && A.stage > 0) unv[k] = ld_at(A.qn[v], ob[k], fb);
+                if constexpr (EPI == EPI_YES) qcur[k] = qv[k];
+                else if (epi) qcur[k] = ld_at(A.q[v], ob[k], fb);
             }
         }
         sync_lines(d);
@@ -665,7 +675,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                 if (A.vis != VIS_NONE && v > 0)
                     r -= double(central_diff<T>(Vl, c, (innermask >> k) & 1u, A.vis, T(A.inv_h)));
                 if (A.fmean && v == 4) r += *A.fmean;
-                if (!A.epilogue) {
+                if (!epi) {
                     st_at(A.rhs[v], ob[k], fb, r);
                 } else {
                     // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (0), :97-101 (1), :140-144 (2)
@@ -688,7 +698,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             for (int k = 0; k < CPL; k++) { qv[k] = qnx[k]; xv[k] = xnx[k]; }
         }
     }
-    if (A.epilogue && A.stage == 2) {
+    if (epi && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) res += __shfl_down(res, o);
@@ -702,10 +712,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     }
 }
 
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
-    sweep_body<SCH, RIE, R, LPB, T, FULL>(A, xcd_block(blockIdx.x, gridDim.x));
+    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 struct ResArgs {
@@ -961,15 +971,23 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
     const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
+    // FULL kernels come in a form with and one without the RK3 update; the
+    // others take it as an argument
     if constexpr (mixed_fp32<SCH>()) {
         if (A.mixed) {
-            if (full) hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, A);
+            if (full && A.epilogue)
+                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_YES>), grid, block, 0, s, A);
+            else if (full)
+                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_NO>), grid, block, 0, s, A);
+            else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG>), grid, block, 0, s, A);
             return;
         }
     }
-    if (full) hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true>), grid, block, 0, s, A);
-    else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false>), grid, block, 0, s, A);
+    if (full && A.epilogue)
+        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true, EPI_YES>), grid, block, 0, s, A);
+    else if (full)
+        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true, EPI_NO>), grid, block, 0, s, A);
+    else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false, EPI_ARG>), grid, block, 0, s, A);
 }
 
 template <int SCH, int RIE>
@@ -1050,7 +1068,7 @@ __global__ __launch_bounds__(64 * LPB) void k_sweep_pack(const SweepArgs *__rest
 {
     const long b = xcd_block(blockIdx.x, gridDim.x);
     const int j = pack_job(off, nj, b);
-    sweep_body<SCH, RIE, R, LPB, T, FULL>(tab[j], b - off[j]);
+    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI_ARG>(tab[j], b - off[j]);
 }
 
 struct PackLaunch {

@@ -98,8 +98,8 @@ def edit(text, edits, what, every=False):
 
 def build_tu(tmp, inst):
     src = open(os.path.join(HERE, "nuc3d.hip")).read()
-    s1 = src.index("struct SweepArgs {")
-    e1 = src.index("\n};\n", s1) + 4
+    s1 = src.index("enum { EPI_NO")
+    e1 = src.index("\n};\n", src.index("struct SweepArgs {")) + 4
     s2 = src.rfind("template", 0, src.index("__device__ __forceinline__ void split_params"))
     e2 = src.index("struct ResArgs")
     body = src[s2:e2]
@@ -150,8 +150,8 @@ def count(asm):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--inst", default="3, 0, 8, 4, double, true",
-                    help="k_sweep template arguments (default: CRWENO5, LF, R=8, 4 lines, FP64, FULL)")
+    ap.add_argument("--inst", default="3, 0, 8, 4, double, true, 0",
+                    help="k_sweep template arguments (default: CRWENO5, LF, R=8, 4 lines, FP64, FULL, no RK3 update)")
     ap.add_argument("--measured", type=float, default=None, help="measured VALU wave-instructions per wave")
     a = ap.parse_args()
     with tempfile.TemporaryDirectory() as tmp:
