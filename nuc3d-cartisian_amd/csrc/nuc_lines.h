@@ -229,28 +229,25 @@ template <int R, typename T> struct Chunk {
     }
 };
 
-// 13/12 (f[i-1] - 2 f[i] + f[i+1])^2 of every interior window point: the
-// second-difference part of the smoothness indicators, shared by the three
-// rows whose stencils contain it
-template <int W, typename T>
-__device__ __forceinline__ void w5_d2(const T *win, T d2[W])
-{
-#pragma unroll
-    for (int i = 1; i < W - 1; i++) d2[i] = T(13.0 / 12.0) * sq(win[i - 1] - T(2) * win[i] + win[i + 1]);
-}
-
 // WENO5 smoothness indicators and candidate values of a 5-point upwind stencil
-// f; d2 holds the shared second-difference terms at f[1], f[2], f[3]
+// f0..f4 (src/weno5js.cpp, src/crweno5.cpp:87-129), computed from the
+// stencil's differences d_j = f_j - f_{j-1} (j = 1..4) and second-difference
+// terms e_k = 13/12 (d_{k+1} - d_k)^2 (k = 1..3), which a lane's consecutive
+// rows share: f0-4f1+3f2 = 3d2-d1, f1-f3 = -(d2+d3), 3f2-4f3+f4 = d4-3d3,
+// q0 = f2+(5d2-2d1)/6, q1 = f2+(d2+2d3)/6, q2 = f3-(2d3+d4)/6.  The linear
+// parts are halved (h_k), so that b_k = e_k + h_k^2 with the 1/4 folded in.
 template <typename T>
-__device__ __forceinline__ void w5_parts(const T *f, const T *d2, T &b0, T &b1, T &b2, T &q0, T &q1, T &q2)
+__device__ __forceinline__ void w5_diff(T f2, T f3, T d1, T d2, T d3, T d4, T e1, T e2, T e3, T &b0, T &b1, T &b2,
+                                        T &q0, T &q1, T &q2, T &h2)
 {
-    const T g1 = T(0.25);
-    b0 = d2[1] + g1 * sq(f[0] - T(4) * f[1] + T(3) * f[2]);
-    b1 = d2[2] + g1 * sq(f[1] - f[3]);
-    b2 = d2[3] + g1 * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
-    q0 = T(1.0 / 3.0) * f[0] + T(-7.0 / 6.0) * f[1] + T(11.0 / 6.0) * f[2];
-    q1 = T(-1.0 / 6.0) * f[1] + T(5.0 / 6.0) * f[2] + T(1.0 / 3.0) * f[3];
-    q2 = T(1.0 / 3.0) * f[2] + T(5.0 / 6.0) * f[3] + T(-1.0 / 6.0) * f[4];
+    const T h0 = T(1.5) * d2 - T(0.5) * d1, h1 = T(0.5) * (d2 + d3);
+    h2 = T(0.5) * d4 - T(1.5) * d3;
+    b0 = e1 + h0 * h0;
+    b1 = e2 + h1 * h1;
+    b2 = e3 + h2 * h2;
+    q0 = f2 + (T(5.0 / 6.0) * d2 - T(1.0 / 3.0) * d1);
+    q1 = f2 + (T(1.0 / 6.0) * d2 + T(1.0 / 3.0) * d3);
+    q2 = f3 - (T(1.0 / 3.0) * d3 + T(1.0 / 6.0) * d4);
 }
 
 // Reconstruct one sign of a line.  Lw points at the lane's window in the
@@ -271,8 +268,6 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
     T win[W];
 #pragma unroll
     for (int t = 0; t < W; t++) win[t] = Lw[Skew<R>::at(t - RAD - 1)];
-    T d2[W];
-    if constexpr (SCH == WENO5JS || SCH == WENO5Z) w5_d2<4, T>(win, d2);
     // rank-local position of each row, without integer division in the loop
     int ml[R];
     {
@@ -287,46 +282,41 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         }
     }
     if constexpr (SCH == WENO5JS) {
-        // rows in order through a rolling window (see CRWENO5 below)
-        T fw[5], e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) fw[j] = win[j];
-        e[1] = d2[1]; e[2] = d2[2];
+        // rows in order through a rolling window of differences (see CRWENO5)
+        T f2 = win[2], f3 = win[3];
+        T d1 = win[1] - win[0], d2 = f2 - win[1], d3 = f3 - f2;
+        T e1 = T(13.0 / 12.0) * sq(d2 - d1), e2 = T(13.0 / 12.0) * sq(d3 - d2);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            fw[4] = win[r + 4];
-            e[3] = T(13.0 / 12.0) * sq(fw[2] - T(2) * fw[3] + fw[4]);
-            T b0, b1, b2, q0, q1, q2;
-            w5_parts<T>(fw, e, b0, b1, b2, q0, q1, q2);
+            T f4 = win[r + 4];
+            T d4 = f4 - f3;
+            T e3 = T(13.0 / 12.0) * sq(d4 - d3);
+            T b0, b1, b2, q0, q1, q2, h2;
+            w5_diff<T>(f2, f3, d1, d2, d3, d4, e1, e2, e3, b0, b1, b2, q0, q1, q2, h2);
             // alpha_k = c_k / s_k, normalised with a single reciprocal
             TW n0, n1, n2;
             w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(0.1), TW(0.6), TW(0.3), n0, n1, n2);
             TS o = TS((n0 * TW(q0) + n1 * TW(q1) + n2 * TW(q2)) * frcp(n0 + n1 + n2));
-            pin(o);
-#pragma unroll
-            for (int j = 1; j < 5; j++) pin(fw[j]);
-            pin(e[2]); pin(e[3]);
+            pin(o); pin(f3); pin(f4); pin(d3); pin(d4); pin(e2); pin(e3);
             out[r] = o;
-#pragma unroll
-            for (int j = 0; j < 4; j++) fw[j] = fw[j + 1];
-            e[1] = e[2]; e[2] = e[3];
+            f2 = f3; f3 = f4; d1 = d2; d2 = d3; d3 = d4; e1 = e2; e2 = e3;
         }
     } else if constexpr (SCH == WENO5Z) {
-        // rows in order through a rolling window (see CRWENO5 below)
-        T fw[5], e[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) fw[j] = win[j];
-        e[1] = d2[1]; e[2] = d2[2];
+        // rows in order through a rolling window of differences (see CRWENO5)
+        T f2 = win[2], f3 = win[3];
+        T d1 = win[1] - win[0], d2 = f2 - win[1], d3 = f3 - f2;
+        T e1 = T(13.0 / 12.0) * sq(d2 - d1), e2 = T(13.0 / 12.0) * sq(d3 - d2);
 #pragma unroll
         for (int r = 0; r < R; r++) {
-            fw[4] = win[r + 4];
-            e[3] = T(13.0 / 12.0) * sq(fw[2] - T(2) * fw[3] + fw[4]);
-            const T *f = fw;
-            T b0, b1, b2, q0, q1, q2;
-            w5_parts<T>(f, e, b0, b1, b2, q0, q1, q2);
-            // the positive inner sweep's is2 reads f_{i+1} twice (src/weno5z.cpp:80)
+            T f4 = win[r + 4];
+            T d4 = f4 - f3;
+            T e3 = T(13.0 / 12.0) * sq(d4 - d3);
+            T b0, b1, b2, q0, q1, q2, h2;
+            w5_diff<T>(f2, f3, d1, d2, d3, d4, e1, e2, e3, b0, b1, b2, q0, q1, q2, h2);
+            // the positive inner sweep's is2 reads f_{i+1} twice (src/weno5z.cpp:80):
+            // 13/12 (f2 - f3)^2 in place of its second-difference term
             bool typo = !neg && ml[r] >= z.ts && ml[r] <= z.nloc - z.ts;
-            if (typo) b2 = T(13.0 / 12.0) * sq(f[2] - f[3]) + T(0.25) * sq(T(3) * f[2] - T(4) * f[3] + f[4]);
+            if (typo) b2 = T(13.0 / 12.0) * sq(d3) + h2 * h2;
             const TW eps = TW(1.0e-40);
             const TW B0 = b0, B1 = b1, B2 = b2;
             TW tau = fabs(B2 - B0);
@@ -334,14 +324,9 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TW a1 = TW(0.6) * (TW(1) + sq(tau / (B1 + eps)));
             TW a2 = TW(0.3) * (TW(1) + sq(tau / (B2 + eps)));
             TS o = TS((a0 * TW(q0) + a1 * TW(q1) + a2 * TW(q2)) * frcp(a0 + a1 + a2));
-            pin(o);
-#pragma unroll
-            for (int j = 1; j < 5; j++) pin(fw[j]);
-            pin(e[2]); pin(e[3]);
+            pin(o); pin(f3); pin(f4); pin(d3); pin(d4); pin(e2); pin(e3);
             out[r] = o;
-#pragma unroll
-            for (int j = 0; j < 4; j++) fw[j] = fw[j + 1];
-            e[1] = e[2]; e[2] = e[3];
+            f2 = f3; f3 = f4; d1 = d2; d2 = d3; d3 = d4; e1 = e2; e2 = e3;
         }
     } else if constexpr (SCH == WENO7JS) {
         // rows in order through a rolling 7-point window (see CRWENO5 below)
@@ -386,25 +371,23 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
         }
     } else if constexpr (SCH == CRWENO5) {
         Chunk<R, TS> ch;
-        // Rows in order: a rolling window f0..f4 and second-difference terms
-        // e1..e3 of the current row, where the values the next row reuses and
-        // the row's own coefficients pass through an empty asm.  Left free,
+        // Rows in order: a rolling window (f1..f4, the differences d1..d4 and
+        // the second-difference terms e1..e3 of the current row), where the
+        // values the next row reuses and the row's own coefficients pass
+        // through an empty asm.  Left free,
         // the compiler evaluates every row's weights before the first
         // elimination step and holds about ten values per row (233 registers,
         // 2 waves per SIMD, against 164 and 3 waves in order).
-        T f0 = win[0], f1 = win[1], f2 = win[2], f3 = win[3];
-        T e1 = T(13.0 / 12.0) * sq(f0 - T(2) * f1 + f2), e2 = T(13.0 / 12.0) * sq(f1 - T(2) * f2 + f3);
+        T f1 = win[1], f2 = win[2], f3 = win[3];
+        T d1 = f1 - win[0], d2 = f2 - f1, d3 = f3 - f2;
+        T e1 = T(13.0 / 12.0) * sq(d2 - d1), e2 = T(13.0 / 12.0) * sq(d3 - d2);
 #pragma unroll
         for (int r = 0; r < R; r++) {
             T f4 = win[r + 4];
-            T e3 = T(13.0 / 12.0) * sq(f2 - T(2) * f3 + f4);
-            const T g1 = T(0.25);
-            T b0 = e1 + g1 * sq(f0 - T(4) * f1 + T(3) * f2);
-            T b1 = e2 + g1 * sq(f1 - f3);
-            T b2 = e3 + g1 * sq(T(3) * f2 - T(4) * f3 + f4);
-            T q0 = T(1.0 / 3.0) * f0 + T(-7.0 / 6.0) * f1 + T(11.0 / 6.0) * f2;
-            T q1 = T(-1.0 / 6.0) * f1 + T(5.0 / 6.0) * f2 + T(1.0 / 3.0) * f3;
-            T q2 = T(1.0 / 3.0) * f2 + T(5.0 / 6.0) * f3 + T(-1.0 / 6.0) * f4;
+            T d4 = f4 - f3;
+            T e3 = T(13.0 / 12.0) * sq(d4 - d3);
+            T b0, b1, b2, q0, q1, q2, h2;
+            w5_diff<T>(f2, f3, d1, d2, d3, d4, e1, e2, e3, b0, b1, b2, q0, q1, q2, h2);
             TW P0, P1, P2;   // weights with unit ideal weights
             w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(1), TW(1), TW(1), P0, P1, P2);
             // Every row is scaled by a factor of its own, which leaves the
@@ -428,10 +411,10 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TS b = valid ? TS(expl ? be : di) : TS(1);
             TS c = (valid && !expl) ? TS(hi) : TS(0);
             TS d = valid ? TS(expl ? de : rhs) : TS(0);
-            asm volatile("" : "+v"(f1), "+v"(f2), "+v"(f3), "+v"(f4), "+v"(e2), "+v"(e3),
+            asm volatile("" : "+v"(f2), "+v"(f3), "+v"(f4), "+v"(d3), "+v"(d4), "+v"(e2), "+v"(e3),
                               "+v"(a), "+v"(b), "+v"(c), "+v"(d));
             ch.row(r, a, b, c, d);
-            f0 = f1; f1 = f2; f2 = f3; f3 = f4; e1 = e2; e2 = e3;
+            f1 = f2; f2 = f3; f3 = f4; d1 = d2; d2 = d3; d3 = d4; e1 = e2; e2 = e3;
         }
         ch.solve(h);
 #pragma unroll

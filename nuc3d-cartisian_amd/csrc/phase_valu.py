@@ -58,9 +58,12 @@ SWEEP_EDITS = [
 LINES_EDITS = [
     ("    T win[W];\n#pragma unroll\n    for (int t = 0; t < W; t++) win[t]",
      "    PH(window);\n    T win[W];\n#pragma unroll\n    for (int t = 0; t < W; t++) win[t]"),
-    ("            T f4 = win[r + 4];\n            T e3 = T(13.0 / 12.0) * sq(f2 - T(2) * f3 + f4);\n            const T g1 = T(0.25);",
-     "            PH(indicators);\n            T f4 = win[r + 4];\n            T e3 = T(13.0 / 12.0) * sq(f2 - T(2) * f3 + f4);\n"
-     "            const T g1 = T(0.25);"),
+    ("            T f4 = win[r + 4];\n            T d4 = f4 - f3;\n            T e3 = T(13.0 / 12.0) * sq(d4 - d3);\n"
+     "            T b0, b1, b2, q0, q1, q2, h2;\n            w5_diff<T>(f2, f3, d1, d2, d3, d4, e1, e2, e3, b0, b1, b2, q0, q1, q2, h2);\n"
+     "            TW P0, P1, P2;",
+     "            PH(indicators);\n            T f4 = win[r + 4];\n            T d4 = f4 - f3;\n            T e3 = T(13.0 / 12.0) * sq(d4 - d3);\n"
+     "            T b0, b1, b2, q0, q1, q2, h2;\n            w5_diff<T>(f2, f3, d1, d2, d3, d4, e1, e2, e3, b0, b1, b2, q0, q1, q2, h2);\n"
+     "            TW P0, P1, P2;"),
     ("            TW P0, P1, P2;   // weights with unit ideal weights\n"
      "            w5_weights<TW>(TW(b0), TW(b1), TW(b2), TW(1), TW(1), TW(1), P0, P1, P2);\n"
      "            // Every row is scaled",
