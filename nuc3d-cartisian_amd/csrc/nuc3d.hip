@@ -226,17 +226,24 @@ constexpr int GTX = 32, GTY = 16;
 // boxes; short ones give small boxes enough workgroups
 inline int grad_planes(int nz) { return nz >= 256 ? 32 : (nz >= 128 ? 16 : (nz >= 64 ? 8 : 4)); }
 
-__device__ __forceinline__ void con2prim_uvwT(const double GMEM *const *q, long id, double gamma, double mach,
-                                              double &u, double &v, double &w, double &T)
+// primitives u, v, w, T of one cell from its five conserved values
+// (src/physicsModel.cpp:331-344, the same expression as k_prim)
+__device__ __forceinline__ void con2prim_uvwT(const double qc[5], double gamma, double mach, double &u, double &v,
+                                              double &w, double &T)
 {
-    // src/physicsModel.cpp:331-344, the same expression as k_prim
     const double gm1 = gamma - 1.0;
-    double rho = q[0][id];
+    double rho = qc[0];
     double ir = 1.0 / rho;
-    u = q[1][id] * ir; v = q[2][id] * ir; w = q[3][id] * ir;
-    double E = q[4][id];
+    u = qc[1] * ir; v = qc[2] * ir; w = qc[3] * ir;
+    double E = qc[4];
     double p = fabs((E - 0.5 * rho * (u * u + v * v + w * w)) * gm1);
     T = gamma * mach * mach * p * ir;
+}
+
+__device__ __forceinline__ void load_cell(const double GMEM *const *q, long id, double qc[5])
+{
+#pragma unroll
+    for (int f = 0; f < 5; f++) qc[f] = q[f][id];
 }
 
 template <typename T>
@@ -252,16 +259,41 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
     const int iw = wrapi(i % nx, nx), jw = wrapi(j % ny, ny);
     const bool own = i < nx && j < ny;
     const double gamma = A.gamma, mach = A.mach;
+    const int kend = min(k0 + A.gkz, nz);
     // register queue: planes k-4 .. k+4 of u, v, w, T at (iw, jw)
     T qz[4][9];
 #pragma unroll
     for (int o = 0; o < 8; o++) {
         int kk = wrapi((k0 - 4 + o) % nz, nz);
-        double u, v, w, Tt;
-        con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
+        double qc[5], u, v, w, Tt;
+        load_cell(A.q, iw + sy * jw + sz * kk, qc);
+        con2prim_uvwT(qc, gamma, mach, u, v, w, Tt);
         qz[0][o + 1] = T(u); qz[1][o + 1] = T(v); qz[2][o + 1] = T(w); qz[3][o + 1] = T(Tt);
     }
-    const int kend = min(k0 + A.gkz, nz);
+    // The tile's 4-cell ring around the current plane: (RING points, at most
+    // one per thread, the same (x, y) on every plane).  Its state and the
+    // column's plane k+4 are loaded one plane ahead (ring_q, col_q), so that
+    // a plane step does not wait for its own loads: a 512-thread workgroup
+    // is alone on its CU at this register count.
+    constexpr int RING_ROWS = 8 * PX;                 // 4 rows above and below the tile
+    constexpr int RING = RING_ROWS + GTY * 8;          // + 4 columns left and right
+    static_assert(RING <= GTX * GTY, "one ring point per thread");
+    const bool has_ring = t < RING;
+    int rcx = 0, rcy = 0;
+    if (t < RING_ROWS) {
+        rcx = t % PX;
+        rcy = t / PX;
+        rcy = rcy < 4 ? rcy : rcy + GTY;
+    } else {
+        const int u8 = t - RING_ROWS;
+        rcy = 4 + u8 / 8;
+        rcx = u8 % 8;
+        rcx = rcx < 4 ? rcx : rcx + GTX;
+    }
+    const long ring_xy = wrapi((i0 + rcx - 4 + 4 * nx) % nx, nx) + sy * wrapi((j0 + rcy - 4 + 4 * ny) % ny, ny);
+    double ring_q[5], col_q[5];
+    if (has_ring) load_cell(A.q, ring_xy + sz * k0, ring_q);
+    load_cell(A.q, iw + sy * jw + sz * wrapi((k0 + 4) % nz, nz), col_q);
     const T S = T(A.tref / A.tinf);
     const T coeff_den = T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
     const int cli = iw % A.nloc[0], clj = jw % A.nloc[1];
@@ -275,34 +307,21 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
 #pragma unroll
             for (int o = 0; o < 8; o++) qz[f][o] = qz[f][o + 1];
         {
-            int kk = wrapi((k + 4) % nz, nz);
             double u, v, w, Tt;
-            con2prim_uvwT(A.q, iw + sy * jw + sz * kk, gamma, mach, u, v, w, Tt);
+            con2prim_uvwT(col_q, gamma, mach, u, v, w, Tt);   // plane k+4, loaded one step ago
             qz[0][8] = T(u); qz[1][8] = T(v); qz[2][8] = T(w); qz[3][8] = T(Tt);
         }
+        if (k + 1 < kend) load_cell(A.q, iw + sy * jw + sz * wrapi((k + 5) % nz, nz), col_q);
         // tile of plane k with wrapped x/y halos: the thread's own (wrapped)
         // cell is the centre of its register queue; only the 4-cell ring
         // around the tile is read from the state
 #pragma unroll
         for (int f = 0; f < 4; f++) tile[f][ty + 4][tx + 4] = qz[f][4];
-        constexpr int RING_ROWS = 8 * PX;                  // 4 rows above and below the tile
-        constexpr int RING = RING_ROWS + GTY * 8;          // + 4 columns left and right
-        for (int c = t; c < RING; c += GTX * GTY) {
-            int cx, cy;
-            if (c < RING_ROWS) {
-                cx = c % PX;
-                cy = c / PX;
-                cy = cy < 4 ? cy : cy + GTY;
-            } else {
-                const int u8 = c - RING_ROWS;
-                cy = 4 + u8 / 8;
-                cx = u8 % 8;
-                cx = cx < 4 ? cx : cx + GTX;
-            }
-            int gi = wrapi((i0 + cx - 4 + 4 * nx) % nx, nx), gj = wrapi((j0 + cy - 4 + 4 * ny) % ny, ny);
+        if (has_ring) {
             double u, v, w, Tt;
-            con2prim_uvwT(A.q, gi + sy * gj + sz * k, gamma, mach, u, v, w, Tt);
-            tile[0][cy][cx] = T(u); tile[1][cy][cx] = T(v); tile[2][cy][cx] = T(w); tile[3][cy][cx] = T(Tt);
+            con2prim_uvwT(ring_q, gamma, mach, u, v, w, Tt);   // plane k, loaded one step ago
+            tile[0][rcy][rcx] = T(u); tile[1][rcy][rcx] = T(v); tile[2][rcy][rcx] = T(w); tile[3][rcy][rcx] = T(Tt);
+            if (k + 1 < kend) load_cell(A.q, ring_xy + sz * (k + 1), ring_q);
         }
         __syncthreads();
         if (own) {
