@@ -148,9 +148,15 @@ __device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
         }
         if (A.do_lam) {
             double c = sqrt(A.gamma * p * ir);
-            long i = id % A.nx, r2 = id / A.nx;
-            long j = r2 % A.ny, k = r2 / A.ny;
-            int rank = (int)(i / A.nlx) + A.npx * ((int)(j / A.nly) + A.npy * (int)(k / A.nlz));
+            // owning rank of the cell; a box holds at most 512^3 cells, so
+            // 32-bit index arithmetic suffices (and one rank needs none)
+            int rank = 0;
+            if (A.P > 1) {
+                const unsigned ui = (unsigned)id, nx = (unsigned)A.nx, ny = (unsigned)A.ny;
+                const unsigned i = ui % nx, r2 = ui / nx;
+                const unsigned j = r2 % ny, k = r2 / ny;
+                rank = (int)(i / (unsigned)A.nlx) + A.npx * ((int)(j / (unsigned)A.nly) + A.npy * (int)(k / (unsigned)A.nlz));
+            }
             if (rank != cur) {
                 if (cur >= 0) {
                     atomicMax(&sl[3 * cur + 0], d2bits(m0));
@@ -537,10 +543,14 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     for (int k = 0; k < CPL; k++) ob[k] = (unsigned)(base + (long)(g + 64 * k) * stride) * 8u;
     const unsigned fb = (unsigned)((long)A.nlines * n * 8);   // bytes of one FP64 field
 
-    // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344),
-    // kept in LDS (one column per thread, read back by the same thread) rather
-    // than in CPL*NSP registers held across every variable's reconstruction
-    __shared__ T sSP[CPL * NSP][64 * LPB];
+    // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344).
+    // The compact schemes keep them in LDS (one column per thread, read back
+    // by the same thread): their chunk solve needs the registers.  The
+    // explicit schemes keep them in CPL*NSP registers: with the LDS copy the
+    // workgroup's 53 KB (LF) would allow only three workgroups per CU.
+    constexpr bool SP_LDS = SCH == CRWENO5 || SCH == HCCS || SCH == HCCS77;
+    __shared__ T sSP[SP_LDS ? CPL * NSP : 1][SP_LDS ? 64 * LPB : 1];
+    T spr[SP_LDS ? 1 : CPL][SP_LDS ? 1 : NSP];
     unsigned innermask = 0;   // cd8 inner-form cells (rank-local position)
 #pragma unroll
     for (int k = 0; k < CPL; k++) {
@@ -552,7 +562,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         T tmp[5];
         split_params<RIE, T>(q, d, T(A.gamma), tmp, A.riemann == RIE_AUSMP);
 #pragma unroll
-        for (int j = 0; j < NSP; j++) sSP[k * NSP + j][t] = tmp[j];
+        for (int j = 0; j < NSP; j++) {
+            if constexpr (SP_LDS) sSP[k * NSP + j][t] = tmp[j];
+            else spr[k][j] = tmp[j];
+        }
         int cl = c % A.nloc;
         if (cl >= A.tsv && cl < A.nloc - A.tsv) innermask |= 1u << k;
     }
@@ -601,7 +614,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             int c = g + 64 * k;
             qo[k] = 0.0;
             xo[k] = T(0);
-            if (FULL || (lv && c < n)) {
+            if ((FULL || (lv && c < n)) && v < 5) {
                 qo[k] = ld_at(A.q[v], ob[k], fb);
                 if (A.vis != VIS_NONE && v > 0)
                     xo[k] = A.fv_f32 ? T(ld_at(static_cast<const float GMEM *>(A.fv[v - 1]), ob[k] >> 1, fb >> 1))
@@ -619,7 +632,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                 T fp, fm;
                 T spk[NSP];
 #pragma unroll
-                for (int j = 0; j < NSP; j++) spk[j] = sSP[k * NSP + j][t];
+                for (int j = 0; j < NSP; j++) {
+                    if constexpr (SP_LDS) spk[j] = sSP[k * NSP + j][t];
+                    else spk[j] = spr[k][j];
+                }
                 split_var<RIE, T>(spk, T(qv[k]), v, d, fp, fm);
                 Pl[slotP(k)] = fp; Ml[slotM(k)] = fm;
                 if (A.vis != VIS_NONE && v > 0) Vl[c] = xv[k];
@@ -636,7 +652,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         }
         double qnx[CPL];
         T xnx[CPL];
-        if (v < 4) fetch(v + 1, qnx, xnx);
+        fetch(v + 1, qnx, xnx);   // v + 1 = 5 gives zeros
         sync_lines(d);
         double out[R];
         // weights and rows in T where FP32 keeps them in range; the Z-type
@@ -712,10 +728,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             }
         }
         sync_lines(d);
-        if (v < 4) {
+        // unconditional, so that the current values are dead after the load
+        // phase instead of being carried to the end of the iteration
 #pragma unroll
-            for (int k = 0; k < CPL; k++) { qv[k] = qnx[k]; xv[k] = xnx[k]; }
-        }
+        for (int k = 0; k < CPL; k++) { qv[k] = qnx[k]; xv[k] = xnx[k]; }
     }
     if (epi && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)

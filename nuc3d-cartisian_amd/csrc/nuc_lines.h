@@ -411,6 +411,11 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
             TS b = valid ? TS(expl ? be : di) : TS(1);
             TS c = (valid && !expl) ? TS(hi) : TS(0);
             TS d = valid ? TS(expl ? de : rhs) : TS(0);
+            // the previous row's elimination step is done before this row's
+            // coefficients are released, so that rows do not queue up for it
+            if (r > 0) {
+                pin(ch.Af[r - 1]); pin(ch.Cf[r - 1]); pin(ch.Df[r - 1]);
+            }
             asm volatile("" : "+v"(f2), "+v"(f3), "+v"(f4), "+v"(d3), "+v"(d4), "+v"(e2), "+v"(e3),
                               "+v"(a), "+v"(b), "+v"(c), "+v"(d));
             ch.row(r, a, b, c, d);
@@ -439,6 +444,9 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                 else hccs77_row<T, TW>(f, a, b, c, d);
             }
             TS as = TS(a), bs = TS(b), cs = TS(c), ds = TS(d);
+            if (r > 0) {
+                pin(ch.Af[r - 1]); pin(ch.Cf[r - 1]); pin(ch.Df[r - 1]);
+            }
             pin(as); pin(bs); pin(cs); pin(ds);
 #pragma unroll
             for (int j = 1; j < 7; j++) pin(fw[j]);
@@ -470,6 +478,9 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
                 }
             }
             TS as = TS(a), bs = TS(b), cs = TS(c), ds = TS(d);
+            if (r > 0) {
+                pin(ch.Af[r - 1]); pin(ch.Cf[r - 1]); pin(ch.Df[r - 1]);
+            }
             pin(as); pin(bs); pin(cs); pin(ds);
 #pragma unroll
             for (int j = 1; j < 7; j++) pin(fw[j]);
