@@ -50,7 +50,7 @@ SWEEP_EDITS = [
     ("#pragma unroll 1\n    for (int v = 0; v < 5; v++) {",
      "#pragma unroll\n    for (int v = 0; v < 5; v++) {\n        PH(split);"),
     ("    // split parameters of the own cells", "    PH(prologue);\n    // split parameters of the own cells"),
-    ("        if (v < 4) fetch(v + 1, qnx, xnx);", "        PH(fetch);\n        if (v < 4) fetch(v + 1, qnx, xnx);"),
+    ("        fetch(v + 1, qnx, xnx);", "        PH(fetch);\n        fetch(v + 1, qnx, xnx);"),
     ("        sync_lines(d);\n        double out[R];", "        PH(barrier);\n        sync_lines(d);\n        double out[R];"),
     ("        sync_wave();\n        if (wv) {", "        PH(write_interfaces);\n        sync_wave();\n        if (wv) {"),
     ("        // the final phase's own global reads", "        PH(final);\n        // the final phase's own global reads"),
