@@ -35,6 +35,9 @@ reductions), timed the same way, with the relative error of every global
 statistic (kinetic energy, dissipation, enstrophy, u_rms, Re_lambda, ...)
 against the FP64 run from the same initial state after the same steps.
 
+fp32: the same with NUC_FP32 (the mixed path's arithmetic with the conserved
+state and the RHS stored in FP32), timed and compared the same way.
+
 weno5js: the same box and initial state with WENO5-JS in place of CRWENO5
 (FP64 and mixed), timed the same way; a secondary figure.
 
@@ -355,6 +358,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--profile-slots", action="store_true", help="print per-kernel times to stderr")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-weno5", action="store_true", help="skip the WENO5-JS secondary figure")
     ap.add_argument("--serve-jobs", type=int, default=SERVE_JOBS,
@@ -362,8 +366,8 @@ def main():
     ap.add_argument("--serve-steps", type=int, default=SERVE_STEPS)
     ap.add_argument("--serve-nmax", type=int, default=SERVE_NMAX)
     ap.add_argument("--cpu-steps", type=int, default=2, help="full RK3 steps of the CPU baseline")
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
-                    help="precision of the timed leg (mixed: for profiling the mixed path's kernels)")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "fp32"), default="fp64",
+                    help="precision of the timed leg (mixed, fp32: for profiling those paths' kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs one box per GPU and measures one MI355X; "
@@ -374,7 +378,7 @@ def main():
     import nuc3d_amd as G
 
     n = args.n
-    cfg = workload_config(G, n, G.MIXED if args.precision == "mixed" else G.FP64)
+    cfg = workload_config(G, n, {"fp64": G.FP64, "mixed": G.MIXED, "fp32": G.FP32}[args.precision])
     q = G.init_hit(cfg, k0=4.0, u_rms=1.0, seed=20261202)
     cells = n ** 3
     print(f"bench: FP64 {n}^3, {args.steps} steps", file=sys.stderr, flush=True)
@@ -410,22 +414,30 @@ def main():
         for k, (t, c) in prof.items():
             print(f"  {k:10s} total {t:9.3f} ms  launches {c:5d}  avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
 
+    def reduced_leg(label, prec):
+        # the same workload at a reduced precision, timed the same way, with
+        # its statistics' errors against the FP64 run after the same steps
+        print(f"bench: {label}", file=sys.stderr, flush=True)
+        rsecs, strd, rprof, _, _ = stage_stats(G, workload_config(G, n, prec), q, args.steps, args.warmup)
+        rel = np.abs(strd - st64) / np.maximum(np.abs(st64), 1e-300)
+        eps64, epsrd = st64[25] + st64[26], strd[25] + strd[26]
+        if args.profile_slots:
+            for k, (t, c) in rprof.items():
+                print(f"  {label} {k:10s} avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
+        return {"value": round(cells * 3 * args.steps / rsecs / 1e6, 3),
+                "ms_per_step": round(rsecs / args.steps * 1e3, 4),
+                "speedup_vs_fp64": round(secs / rsecs, 4),
+                "rel_err_vs_fp64": {"KE": float(rel[15]), "dissipation": float(abs(epsrd - eps64) / eps64),
+                                    **{STAT_NAMES[i]: float(rel[i]) for i in (12, 16, 17, 18, 19, 20, 21, 24, 25, 26)}},
+                "after_steps": args.warmup + args.steps}
+
     mixed = None
     if not args.no_mixed:
-        print("bench: mixed", file=sys.stderr, flush=True)
-        msecs, stmx, mprof, _, _ = stage_stats(G, workload_config(G, n, G.MIXED), q, args.steps, args.warmup)
-        rel = np.abs(stmx - st64) / np.maximum(np.abs(st64), 1e-300)
-        eps64, epsmx = st64[25] + st64[26], stmx[25] + stmx[26]
-        mixed = {"value": round(cells * 3 * args.steps / msecs / 1e6, 3),
-                 "ms_per_step": round(msecs / args.steps * 1e3, 4),
-                 "speedup_vs_fp64": round(secs / msecs, 4),
-                 "rel_err_vs_fp64": {"KE": float(rel[15]), "dissipation": float(abs(epsmx - eps64) / eps64),
-                                     **{STAT_NAMES[i]: float(rel[i]) for i in (12, 16, 17, 18, 19, 20, 21, 24, 25, 26)}},
-                 "after_steps": args.warmup + args.steps}
+        mixed = reduced_leg("mixed", G.MIXED)
         mixed["tke_budget"] = budget_error(G, n, q)
-        if args.profile_slots:
-            for k, (t, c) in mprof.items():
-                print(f"  mixed {k:10s} avg {t / max(c, 1):8.4f} ms", file=sys.stderr)
+    fp32 = None
+    if not args.no_fp32:
+        fp32 = reduced_leg("FP32", G.FP32)
 
     # secondary figure: the same box and initial state with WENO5-JS (explicit
     # five-point reconstruction, no tridiagonal solve) in place of CRWENO5
@@ -468,7 +480,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f64 state, f32 stencils",
+        "dtype": {"fp64": "f64", "mixed": "f64 state, f32 stencils", "fp32": "f32 state, f32 stencils"}[args.precision],
         "data": "synthetic (seeded random-phase solenoidal HIT field, seed 20261202)",
         "config": {"workload": f"decaying compressible isotropic turbulence {n}^3, CRWENO5 x3 + LF splitting + "
                                "cd8 viscous + TVD-RK3, CFL 0.5, FP64",
@@ -480,6 +492,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "mixed": mixed,
+        "fp32": fp32,
         "weno5js": weno5,
         "serving": serving,
     }

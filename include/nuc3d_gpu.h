@@ -44,8 +44,15 @@ enum nuc_viscosity { NUC_SUTHERLAND = 0, NUC_CONSTANT = 1 };
 /* forcing (src/physicsModel.cpp:50-56): Const (FFT band forcing), ABC and TG
  * (analytic fields at the cell centres, src/physicsModel.cpp:1679-1799) */
 enum nuc_forcing { NUC_FORCING_NONE = 0, NUC_FORCING_CONST = 1, NUC_FORCING_ABC = 2, NUC_FORCING_TG = 3 };
-/* arithmetic of the stencils; the conserved state is FP64 in both */
-enum nuc_precision { NUC_FP64 = 0, NUC_MIXED = 1 };
+/* precision.  NUC_FP64: everything FP64.  NUC_MIXED: FP32 split fluxes,
+ * stencils and viscous fluxes, FP64 conserved state, tridiagonal solves, RHS
+ * accumulation and RK3 update.  NUC_FP32: the mixed arithmetic with the
+ * conserved state and the RHS also stored in FP32 (half their memory
+ * traffic); uploads, downloads, statistics, spectra and averages see the
+ * state widened to FP64.  NUC_FP32 does not take hccs or hccs77 (their
+ * blend needs FP64 stencils), forcing or rotating-frame sources, packs or
+ * nuc_rhs_noalloc (NUC_EINVAL). */
+enum nuc_precision { NUC_FP64 = 0, NUC_MIXED = 1, NUC_FP32 = 2 };
 
 typedef struct nuc_config {
     int nx, ny, nz;           /* global grid (inp/block.in line 2) */

@@ -74,17 +74,19 @@ struct SweepArgs {
     int nx, ny, nz;
     long nlines;
     double inv_h, gamma;
-    const double GMEM *q[5];
+    // state fields: element type S of the kernel (double; float with NUC_FP32)
+    const void GMEM *q[5];
     const void GMEM *fv[4];   // viscous flux components (float when fv_f32, else double)
     int fv_f32;
-    double GMEM *rhs[5];
+    void GMEM *rhs[5];        // S, as the state
     int stage, epilogue;
     int first;                // the stage's first sweep: writes rhs without reading it
     int accum;                // the first sweep adds to rhs (forcing initialised it)
     int mixed;                // FP32 stencils (host-side dispatch only)
+    int fp32;                 // FP32 state and RHS (NUC_FP32; host-side dispatch only)
     const double GMEM *fmean; // last sweep: <f.u> added back to the energy RHS
-    const double GMEM *qn[5];
-    double GMEM *qout[5];
+    const void GMEM *qn[5];
+    void GMEM *qout[5];
     const double GMEM *dtp;
     double GMEM *res_part;
 };
@@ -95,7 +97,7 @@ struct SweepArgs {
 constexpr int kSweepOrder[3] = {2, 1, 0};
 
 struct PrimArgs {
-    const double GMEM *q[5];
+    const void GMEM *q[5];    // state, element type S of the kernel
     double GMEM *prim[4];
     long N;
     int nx, ny, nlx, nly, nlz, npx, npy, P;
@@ -105,7 +107,7 @@ struct PrimArgs {
 };
 
 struct GradArgs {
-    const double GMEM *q[5];  // conserved state (k_gradq)
+    const void GMEM *q[5];    // conserved state, element type S of the kernel
     const double GMEM *prim[4];
     void GMEM *out[9];        // 9 viscous flux components, T of the kernel
     int nx, ny, nz, gkz;
@@ -123,8 +125,12 @@ __device__ __forceinline__ unsigned long long d2bits(double d) { return (unsigne
 // ---------------------------------------------------------------------------
 // bid/nbk: this workgroup's index and the workgroup count of the job's
 // grid-stride loop
+template <typename S>
 __device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
 {
+    const S GMEM *const q0 = static_cast<const S GMEM *>(A.q[0]), *const q1 = static_cast<const S GMEM *>(A.q[1]);
+    const S GMEM *const q2 = static_cast<const S GMEM *>(A.q[2]), *const q3 = static_cast<const S GMEM *>(A.q[3]);
+    const S GMEM *const q4 = static_cast<const S GMEM *>(A.q[4]);
     __shared__ unsigned long long sl[3 * MAXRANK];
     const int t = threadIdx.x;
     if (A.do_lam) {
@@ -135,10 +141,10 @@ __device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     const double gm1 = A.gamma - 1.0;
     for (long id = bid * 256 + t; id < A.N; id += nbk * 256) {
-        double rho = A.q[0][id];
+        double rho = q0[id];
         double ir = 1.0 / rho;
-        double u = A.q[1][id] * ir, v = A.q[2][id] * ir, w = A.q[3][id] * ir;
-        double E = A.q[4][id];
+        double u = double(q1[id]) * ir, v = double(q2[id]) * ir, w = double(q3[id]) * ir;
+        double E = q4[id];
         double p = fabs((E - 0.5 * rho * (u * u + v * v + w * w)) * gm1);
         if (A.write_prim) {
             A.prim[0][id] = u;
@@ -184,7 +190,8 @@ __device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
     }
 }
 
-__global__ __launch_bounds__(256) void k_prim(PrimArgs A) { prim_body(A, blockIdx.x, gridDim.x); }
+template <typename S>
+__global__ __launch_bounds__(256) void k_prim(PrimArgs A) { prim_body<S>(A, blockIdx.x, gridDim.x); }
 
 // dt per rank and min over ranks; resets the eigenvalue slots for the next step
 struct DtArgs {
@@ -246,13 +253,14 @@ __device__ __forceinline__ void con2prim_uvwT(const double qc[5], double gamma, 
     T = gamma * mach * mach * p * ir;
 }
 
-__device__ __forceinline__ void load_cell(const double GMEM *const *q, long id, double qc[5])
+template <typename S>
+__device__ __forceinline__ void load_cell(const void GMEM *const *q, long id, double qc[5])
 {
 #pragma unroll
-    for (int f = 0; f < 5; f++) qc[f] = q[f][id];
+    for (int f = 0; f < 5; f++) qc[f] = static_cast<const S GMEM *>(q[f])[id];
 }
 
-template <typename T>
+template <typename T, typename S>
 __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, int bz)
 {
     constexpr int PX = GTX + 8, PY = GTY + 8;
@@ -272,7 +280,7 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
     for (int o = 0; o < 8; o++) {
         int kk = wrapi((k0 - 4 + o) % nz, nz);
         double qc[5], u, v, w, Tt;
-        load_cell(A.q, iw + sy * jw + sz * kk, qc);
+        load_cell<S>(A.q, iw + sy * jw + sz * kk, qc);
         con2prim_uvwT(qc, gamma, mach, u, v, w, Tt);
         qz[0][o + 1] = T(u); qz[1][o + 1] = T(v); qz[2][o + 1] = T(w); qz[3][o + 1] = T(Tt);
     }
@@ -298,9 +306,9 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
     }
     const long ring_xy = wrapi((i0 + rcx - 4 + 4 * nx) % nx, nx) + sy * wrapi((j0 + rcy - 4 + 4 * ny) % ny, ny);
     double ring_q[5], col_q[5];
-    if (has_ring) load_cell(A.q, ring_xy + sz * k0, ring_q);
-    load_cell(A.q, iw + sy * jw + sz * wrapi((k0 + 4) % nz, nz), col_q);
-    const T S = T(A.tref / A.tinf);
+    if (has_ring) load_cell<S>(A.q, ring_xy + sz * k0, ring_q);
+    load_cell<S>(A.q, iw + sy * jw + sz * wrapi((k0 + 4) % nz, nz), col_q);
+    const T Ssu = T(A.tref / A.tinf);   // Sutherland constant over T_inf
     const T coeff_den = T((A.gamma - 1.0) * A.mach * A.mach * A.pr);
     const int cli = iw % A.nloc[0], clj = jw % A.nloc[1];
     const bool inx = cli >= A.tsv[0] && cli < A.nloc[0] - A.tsv[0];
@@ -317,7 +325,7 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
             con2prim_uvwT(col_q, gamma, mach, u, v, w, Tt);   // plane k+4, loaded one step ago
             qz[0][8] = T(u); qz[1][8] = T(v); qz[2][8] = T(w); qz[3][8] = T(Tt);
         }
-        if (k + 1 < kend) load_cell(A.q, iw + sy * jw + sz * wrapi((k + 5) % nz, nz), col_q);
+        if (k + 1 < kend) load_cell<S>(A.q, iw + sy * jw + sz * wrapi((k + 5) % nz, nz), col_q);
         // tile of plane k with wrapped x/y halos: the thread's own (wrapped)
         // cell is the centre of its register queue; only the 4-cell ring
         // around the tile is read from the state
@@ -327,7 +335,7 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
             double u, v, w, Tt;
             con2prim_uvwT(ring_q, gamma, mach, u, v, w, Tt);   // plane k, loaded one step ago
             tile[0][rcy][rcx] = T(u); tile[1][rcy][rcx] = T(v); tile[2][rcy][rcx] = T(w); tile[3][rcy][rcx] = T(Tt);
-            if (k + 1 < kend) load_cell(A.q, ring_xy + sz * (k + 1), ring_q);
+            if (k + 1 < kend) load_cell<S>(A.q, ring_xy + sz * (k + 1), ring_q);
         }
         __syncthreads();
         if (own) {
@@ -352,7 +360,7 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
             T tx_ = g[3][0], ty_ = g[3][1], tz_ = g[3][2];
             T Tt = qz[3][4];
             T miu0;
-            if (A.law == 0) miu0 = (T(1) + S) / (Tt + S) * (Tt * sqrt(Tt)) / T(A.re);   // src/physicsModel.cpp:220
+            if (A.law == 0) miu0 = (T(1) + Ssu) / (Tt + Ssu) * (Tt * sqrt(Tt)) / T(A.re);   // src/physicsModel.cpp:220
             else miu0 = T(1) / T(A.re);
             T coeff0 = miu0 / coeff_den;
             T div = ux + vy + wz;
@@ -379,8 +387,8 @@ __device__ __forceinline__ void gradq_body(const GradArgs &A, int bx, int by, in
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A) { gradq_body<T>(A, blockIdx.x, blockIdx.y, blockIdx.z); }
+template <typename T, typename S>
+__global__ __launch_bounds__(GTX * GTY) void k_gradq(GradArgs A) { gradq_body<T, S>(A, blockIdx.x, blockIdx.y, blockIdx.z); }
 
 // ---------------------------------------------------------------------------
 // the direction sweep
@@ -470,6 +478,10 @@ __device__ __forceinline__ void st_at(double GMEM *p, unsigned ob, unsigned byte
     using u2 = __attribute__((ext_vector_type(2))) unsigned;
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), field_rsrc(p, bytes), ob, 0, 0);
 }
+__device__ __forceinline__ void st_at(float GMEM *p, unsigned ob, unsigned bytes, float v)
+{
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), field_rsrc(p, bytes), ob, 0, 0);
+}
 
 // Workgroup index with consecutive indices placed on the same XCD: the
 // dispatcher deals workgroups round-robin over the 8 XCDs (each with its own
@@ -508,7 +520,7 @@ __device__ __forceinline__ void sync_lines(int d)
 // phase to the update; the other forms read them again in the final phase,
 // so that the first two sweeps do not hold them across the reconstruction.
 // blk: the workgroup's logical index within its job (line group blk*LPB ..)
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI>
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 {
     const bool epi = EPI == EPI_ARG ? A.epilogue != 0 : EPI == EPI_YES;
@@ -542,6 +554,12 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 #pragma unroll
     for (int k = 0; k < CPL; k++) ob[k] = (unsigned)(base + (long)(g + 64 * k) * stride) * 8u;
     const unsigned fb = (unsigned)((long)A.nlines * n * 8);   // bytes of one FP64 field
+    // state and RHS fields hold S (ob, fb are for 8-byte elements)
+    constexpr unsigned SSH = sizeof(S) == 8 ? 0 : 1;
+    auto lds = [&](const void GMEM *p, int k) -> double {
+        return double(ld_at(static_cast<const S GMEM *>(p), ob[k] >> SSH, fb >> SSH));
+    };
+    auto sts = [&](void GMEM *p, int k, double val) { st_at(static_cast<S GMEM *>(p), ob[k] >> SSH, fb >> SSH, S(val)); };
 
     // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344).
     // The compact schemes keep them in LDS (one column per thread, read back
@@ -558,7 +576,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         bool ok = FULL || (lv && c < n);
         T q[5];
 #pragma unroll
-        for (int v = 0; v < 5; v++) q[v] = ok ? T(ld_at(A.q[v], ob[k], fb)) : (v == 0 ? T(1) : T(0));
+        for (int v = 0; v < 5; v++) q[v] = ok ? T(lds(A.q[v], k)) : (v == 0 ? T(1) : T(0));
         T tmp[5];
         split_params<RIE, T>(q, d, T(A.gamma), tmp, A.riemann == RIE_AUSMP);
 #pragma unroll
@@ -615,7 +633,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             qo[k] = 0.0;
             xo[k] = T(0);
             if ((FULL || (lv && c < n)) && v < 5) {
-                qo[k] = ld_at(A.q[v], ob[k], fb);
+                qo[k] = lds(A.q[v], k);
                 if (A.vis != VIS_NONE && v > 0)
                     xo[k] = A.fv_f32 ? T(ld_at(static_cast<const float GMEM *>(A.fv[v - 1]), ob[k] >> 1, fb >> 1))
                                      : T(ld_at(static_cast<const double GMEM *>(A.fv[v - 1]), ob[k], fb));
@@ -693,10 +711,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             unv[k] = 0.0;
             qcur[k] = 0.0;
             if (FULL || (lv && c < n)) {
-                if (!A.first || A.accum) rin[k] = ld_at(A.rhs[v], ob[k], fb);
-                if (epi && A.stage > 0) unv[k] = ld_at(A.qn[v], ob[k], fb);
+                if (!A.first || A.accum) rin[k] = lds(A.rhs[v], k);
+                if (epi && A.stage > 0) unv[k] = lds(A.qn[v], k);
                 if constexpr (EPI == EPI_YES) qcur[k] = qv[k];
-                else if (epi) qcur[k] = ld_at(A.q[v], ob[k], fb);
+                else if (epi) qcur[k] = lds(A.q[v], k);
             }
         }
         sync_lines(d);
@@ -711,7 +729,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                     r -= double(central_diff<T>(Vl, c, (innermask >> k) & 1u, A.vis, T(A.inv_h)));
                 if (A.fmean && v == 4) r += *A.fmean;
                 if (!epi) {
-                    st_at(A.rhs[v], ob[k], fb, r);
+                    sts(A.rhs[v], k, r);
                 } else {
                     // TVD-RK3 stage update, src/TVD-RK3.cpp:59-64 (0), :97-101 (1), :140-144 (2)
                     double rr = -r, nu;
@@ -721,9 +739,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                         double un = unv[k];
                         if (A.stage == 1) nu = 0.75 * un + 0.25 * (rr * dt + qcur[k]);
                         else nu = (1.0 / 3.0) * un + (2.0 / 3.0) * (rr * dt + qcur[k]);
+                        nu = double(S(nu));   // the stored value (an FP32 state rounds here)
                         if (A.stage == 2) res += sq(nu - un);
                     }
-                    st_at(A.qout[v], ob[k], fb, nu);
+                    sts(A.qout[v], k, nu);
                 }
             }
         }
@@ -747,10 +766,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     }
 }
 
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI>
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
-    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI>(A, xcd_block(blockIdx.x, gridDim.x));
+    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI, S>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 struct ResArgs {
@@ -1009,6 +1028,16 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
     // FULL kernels come in a form with and one without the RK3 update; the
     // others take it as an argument
     if constexpr (mixed_fp32<SCH>()) {
+        if (A.fp32) {
+            // NUC_FP32: FP32 state and RHS around the mixed path's arithmetic
+            if (full && A.epilogue)
+                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_YES, float>), grid, block, 0, s, A);
+            else if (full)
+                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_NO, float>), grid, block, 0, s, A);
+            else
+                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG, float>), grid, block, 0, s, A);
+            return;
+        }
         if (A.mixed) {
             if (full && A.epilogue)
                 hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_YES>), grid, block, 0, s, A);
@@ -1081,7 +1110,7 @@ __device__ __forceinline__ int pack_job(const int *off, int nj, long b)
 __global__ __launch_bounds__(256) void k_prim_pack(const PrimArgs *__restrict__ tab, const int *__restrict__ off, int nj)
 {
     const int j = pack_job(off, nj, blockIdx.x);
-    prim_body(tab[j], (long)blockIdx.x - off[j], (long)off[j + 1] - off[j]);
+    prim_body<double>(tab[j], (long)blockIdx.x - off[j], (long)off[j + 1] - off[j]);
 }
 
 __global__ void k_dt_pack(const DtArgs *__restrict__ tab) { dt_body(tab[blockIdx.x]); }
@@ -1095,7 +1124,7 @@ __global__ __launch_bounds__(GTX * GTY) void k_gradq_pack(const GradArgs *__rest
     const GradArgs &A = tab[j];
     const int rel = (int)blockIdx.x - off[j];
     const int gx = (A.nx + GTX - 1) / GTX, gy = (A.ny + GTY - 1) / GTY;
-    gradq_body<T>(A, rel % gx, (rel / gx) % gy, rel / (gx * gy));
+    gradq_body<T, double>(A, rel % gx, (rel / gx) % gy, rel / (gx * gy));
 }
 
 template <int SCH, int RIE, int R, int LPB, typename T, bool FULL>
@@ -1204,7 +1233,9 @@ struct nuc_ctx {
     double *post_buf = nullptr;
     bool forcing = false;
     bool sources = false;   // ABC/TG forcing or rotating-frame sources (k_source)
-    bool mixed = false;     // NUC_MIXED: FP32 stencils, FP64 weights/solves/RK3
+    bool mixed = false;     // NUC_MIXED (and NUC_FP32): FP32 stencils, FP64 weights/solves/RK3
+    bool fp32 = false;      // NUC_FP32: state and RHS stored in FP32 (the first half of each buffer)
+    double *q64 = nullptr;  // NUC_FP32: the state widened to FP64 for I/O, statistics, post-processing
     // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
     int gkz = 0;
     // next RK stage of nuc_rk3_stage; the current state is q[next_stage]
@@ -1297,7 +1328,21 @@ static int validate(const nuc_config *c, std::string &err)
     if (c->riemann < 0 || c->riemann > NUC_AUSMP || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
     if (c->forcing < NUC_FORCING_NONE || c->forcing > NUC_FORCING_TG) { err = "unknown forcing id"; return NUC_EINVAL; }
     if (c->centrifugal && !(c->well_rad > 0)) { err = "centrifugal well radius must be positive"; return NUC_EINVAL; }
-    if (c->precision != NUC_FP64 && c->precision != NUC_MIXED) { err = "unknown precision id"; return NUC_EINVAL; }
+    if (c->precision != NUC_FP64 && c->precision != NUC_MIXED && c->precision != NUC_FP32) {
+        err = "unknown precision id";
+        return NUC_EINVAL;
+    }
+    if (c->precision == NUC_FP32) {
+        for (int d = 0; d < 3; d++)
+            if (c->scheme[d] == NUC_HCCS || c->scheme[d] == NUC_HCCS77) {
+                err = "NUC_FP32 does not take hccs or hccs77 (their theta blend needs FP64 stencils)";
+                return NUC_EINVAL;
+            }
+        if (c->forcing != NUC_FORCING_NONE || c->coriolis || c->centrifugal) {
+            err = "NUC_FP32 does not take forcing or rotating-frame sources";
+            return NUC_EINVAL;
+        }
+    }
     if (!(c->lx > 0 && c->ly > 0 && c->lz > 0)) { err = "box lengths must be positive"; return NUC_EINVAL; }
     return NUC_OK;
 }
@@ -1305,11 +1350,14 @@ static int validate(const nuc_config *c, std::string &err)
 extern "C" long long nuc_footprint(const nuc_config *c)
 {
     long long N = (long long)c->nx * c->ny * c->nz;
-    // 3 states + rhs; 9 viscous flux fields (Navier-Stokes); u,v,w,T (forcing)
-    long long doubles = 15 * N + 5 * N + (c->model == NUC_NAVIERSTOKES3D ? 9 * N : 0);
-    if (c->forcing == NUC_FORCING_CONST) doubles += 4 * N;
+    // 3 states + rhs (FP32 with NUC_FP32, and its FP64 view); 9 viscous flux
+    // fields (Navier-Stokes; FP32 when the stencils are); u,v,w,T (forcing)
+    const long long es = c->precision == NUC_FP32 ? 4 : 8, fes = c->precision == NUC_FP64 ? 8 : 4;
+    long long bytes = 20 * N * es + (c->model == NUC_NAVIERSTOKES3D ? 9 * N * fes : 0);
+    if (c->precision == NUC_FP32) bytes += 5 * N * 8;
+    if (c->forcing == NUC_FORCING_CONST) bytes += 4 * N * 8;
     long long fb = (c->forcing == NUC_FORCING_CONST) ? nucforce::plan_bytes(*c) : 0;
-    return doubles * 8 + (N / 32 + 4096) * 8 + 4096 + fb;
+    return bytes + (N / 32 + 4096) * 8 + 4096 + fb;
 }
 
 // message of the last failed call without a context on this thread
@@ -1369,16 +1417,21 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     auto dalloc = [&](auto **ptr, size_t bytes) {
         return hipMallocAsync(reinterpret_cast<void **>(ptr), bytes, x->stream) == hipSuccess;
     };
-    for (int b = 0; b < 3; b++)
-        if (!dalloc(&x->q[b], 5 * N * sizeof(double))) return fail("device allocation (state)");
-    if (!dalloc(&x->rhs, 5 * N * sizeof(double))) return fail("device allocation (rhs)");
     x->forcing = (cfg->forcing == NUC_FORCING_CONST);
     x->sources = cfg->forcing == NUC_FORCING_ABC || cfg->forcing == NUC_FORCING_TG || cfg->coriolis || cfg->centrifugal;
-    x->mixed = (cfg->precision == NUC_MIXED);
+    x->mixed = (cfg->precision != NUC_FP64);
+    x->fp32 = (cfg->precision == NUC_FP32);
+    // element bytes of the state/rhs fields and of the viscous-flux fields
+    const size_t es = x->fp32 ? sizeof(float) : sizeof(double), fes = x->mixed ? sizeof(float) : sizeof(double);
+    for (int b = 0; b < 3; b++)
+        if (!dalloc(&x->q[b], 5 * N * es)) return fail("device allocation (state)");
+    if (!dalloc(&x->rhs, 5 * N * es)) return fail("device allocation (rhs)");
+    if (x->fp32)
+        if (!dalloc(&x->q64, 5 * N * sizeof(double))) return fail("device allocation (FP64 view)");
     if (x->forcing)
         if (!dalloc(&x->prim, 4 * N * sizeof(double))) return fail("device allocation (prim)");
     if (cfg->model == NUC_NAVIERSTOKES3D)
-        if (!dalloc(&x->fv, 9 * N * sizeof(double))) return fail("device allocation (fv)");
+        if (!dalloc(&x->fv, 9 * N * fes)) return fail("device allocation (fv)");
     if (x->forcing) {
         std::string ferr;
         int frc = nucforce::plan_create(x->fplan, *cfg, x->stream, ferr);
@@ -1411,7 +1464,7 @@ extern "C" int nuc_destroy(nuc_ctx *x)
     if (x->gexec) (void)hipGraphExecDestroy(x->gexec);
     if (x->graph) (void)hipGraphDestroy(x->graph);
     void *bufs[] = {x->q[0], x->q[1], x->q[2], x->rhs, x->prim, x->fv, x->lam, x->scal, x->res_part,
-                    x->stat_part, x->stat_out, x->post_buf};
+                    x->stat_part, x->stat_out, x->post_buf, x->q64};
     for (void *b : bufs)
         if (b) (void)hipFreeAsync(b, x->stream);
     if (x->forcing) nucforce::plan_destroy(x->fplan, x->stream);
@@ -1423,11 +1476,60 @@ extern "C" int nuc_destroy(nuc_ctx *x)
 
 extern "C" const char *nuc_last_error(const nuc_ctx *x) { return x ? x->err.c_str() : g_create_err.c_str(); }
 
+// NUC_FP32 conversions between an FP64 field set and the FP32 state
+__global__ __launch_bounds__(256) void k_narrow(const double *a, float *b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = float(a[i]);
+}
+__global__ __launch_bounds__(256) void k_widen(const float *a, double *b, long n)
+{
+    for (long i = (long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long)gridDim.x * 256) b[i] = double(a[i]);
+}
+static dim3 conv_grid(long n) { return dim3((unsigned)std::min<long>((n + 255) / 256, 65536)); }
+
+// field v of RK slot b and of the RHS in the kernels' state type (FP32 with
+// NUC_FP32: the buffers then hold 5 FP32 fields)
+static const void *slot_field(const nuc_ctx *x, int b, int v)
+{
+    return x->fp32 ? (const void *)(reinterpret_cast<const float *>(x->q[b]) + (size_t)v * x->N)
+                   : (const void *)(x->q[b] + (size_t)v * x->N);
+}
+static void *slot_field_w(nuc_ctx *x, int b, int v) { return const_cast<void *>(slot_field(x, b, v)); }
+static void *rhs_field(nuc_ctx *x, int v)
+{
+    return x->fp32 ? (void *)(reinterpret_cast<float *>(x->rhs) + (size_t)v * x->N) : (void *)(x->rhs + (size_t)v * x->N);
+}
+// viscous-flux component o (FP32 fields, k_gradq<float>, when the stencils are FP32)
+static void *fv_field(const nuc_ctx *x, int o)
+{
+    return x->mixed ? (void *)(reinterpret_cast<float *>(x->fv) + (size_t)o * x->N) : (void *)(x->fv + (size_t)o * x->N);
+}
+
+// The current state in FP64 for the statistics, post-processing and
+// downloads: the state itself, or with NUC_FP32 its widened copy (enqueued)
+static const double *state64(nuc_ctx *x)
+{
+    if (!x->fp32) return x->q[x->next_stage];
+    hipLaunchKernelGGL(k_widen, conv_grid(5 * x->N), dim3(256), 0, x->stream,
+                       reinterpret_cast<const float *>(x->q[x->next_stage]), x->q64, 5 * x->N);
+    return x->q64;
+}
+// Where the initial-condition generators write a new FP64 state (RK slot 0,
+// or with NUC_FP32 the FP64 view), and its narrowing into slot 0 afterwards
+static double *state64_w(nuc_ctx *x) { return x->fp32 ? x->q64 : x->q[0]; }
+static void state64_commit(nuc_ctx *x)
+{
+    if (x->fp32)
+        hipLaunchKernelGGL(k_narrow, conv_grid(5 * x->N), dim3(256), 0, x->stream, x->q64,
+                           reinterpret_cast<float *>(x->q[0]), 5 * x->N);
+}
+
 extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
 {
     if (!x || !q) return NUC_EINVAL;
     x->next_stage = 0;   // an upload starts a new step
-    NUC_CHECK(x, hipMemcpyAsync(x->q[0], q, 5 * x->N * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    NUC_CHECK(x, hipMemcpyAsync(state64_w(x), q, 5 * x->N * sizeof(double), hipMemcpyHostToDevice, x->stream));
+    state64_commit(x);
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }
@@ -1435,7 +1537,7 @@ extern "C" int nuc_upload_state(nuc_ctx *x, const double *q)
 extern "C" int nuc_download_state(nuc_ctx *x, double *q)
 {
     if (!x || !q) return NUC_EINVAL;
-    NUC_CHECK(x, hipMemcpyAsync(q, x->q[x->next_stage], 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    NUC_CHECK(x, hipMemcpyAsync(q, state64(x), 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
 }
@@ -1463,12 +1565,11 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
 {
     const nuc_config &c = x->c;
     const long N = x->N;
-    double *qi = x->q[in];
     const bool ns = (c.model == NUC_NAVIERSTOKES3D);
     const bool lam_needed = (stage == 0);
     {
         PrimArgs &P = sa.prim;
-        for (int v = 0; v < 5; v++) P.q[v] = gp(qi + v * N);
+        for (int v = 0; v < 5; v++) P.q[v] = gp(slot_field(x, in, v));
         const bool wp = x->forcing;   // the gradient kernel reads Q itself
         if (wp) for (int f = 0; f < 4; f++) P.prim[f] = gp(x->prim + f * N);
         P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
@@ -1483,8 +1584,8 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
     }
     if (ns) {
         GradArgs &G = sa.grad;
-        for (int v = 0; v < 5; v++) G.q[v] = gp(qi + v * N);
-        for (int o = 0; o < 9; o++) G.out[o] = gp(x->fv + o * N);
+        for (int v = 0; v < 5; v++) G.q[v] = gp(slot_field(x, in, v));
+        for (int o = 0; o < 9; o++) G.out[o] = gp(fv_field(x, o));
         G.nx = x->n[0]; G.ny = x->n[1]; G.nz = x->n[2];
         for (int d = 0; d < 3; d++) {
             G.nloc[d] = x->nloc[d]; G.tsv[d] = x->tsv[d];
@@ -1508,18 +1609,19 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         S.nlines = N / x->n[d];
         S.inv_h = 1.0 / x->h[d];
         S.gamma = c.gamma;
-        for (int v = 0; v < 5; v++) { S.q[v] = gp(qi + v * N); S.rhs[v] = gp(x->rhs + v * N); }
-        if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(x->fv + (long)fvmap[d][o] * N);
+        for (int v = 0; v < 5; v++) { S.q[v] = gp(slot_field(x, in, v)); S.rhs[v] = gp(rhs_field(x, v)); }
+        if (ns) for (int o = 0; o < 4; o++) S.fv[o] = gp(fv_field(x, fvmap[d][o]));
         S.stage = stage;
         const bool first = (d == kSweepOrder[0]), last = (d == kSweepOrder[2]);
         S.first = first ? 1 : 0;
         S.accum = (first && (x->forcing || x->sources)) ? 1 : 0;
         S.mixed = x->mixed ? 1 : 0;
+        S.fp32 = x->fp32 ? 1 : 0;
         S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
         S.fmean = (last && x->forcing) ? gp(x->scal + 4) : nullptr;
         S.epilogue = (last && epilogue) ? 1 : 0;
         if (S.epilogue) {
-            for (int v = 0; v < 5; v++) { S.qn[v] = gp(x->q[0] + v * N); S.qout[v] = gp(x->q[outb] + v * N); }
+            for (int v = 0; v < 5; v++) { S.qn[v] = gp(slot_field(x, 0, v)); S.qout[v] = gp(slot_field_w(x, outb, v)); }
             S.dtp = gp(x->scal);
             S.res_part = gp(x->res_part);
         }
@@ -1603,6 +1705,14 @@ static SourceArgs source_args(nuc_ctx *x, int in, bool accumulate)
     return A;
 }
 
+// the gradient kernel for the context's precision (flux type, state type)
+static void launch_gradq(nuc_ctx *x, dim3 grid, const GradArgs &G)
+{
+    if (x->fp32) hipLaunchKernelGGL((k_gradq<float, float>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+    else if (x->mixed) hipLaunchKernelGGL((k_gradq<float, double>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+    else hipLaunchKernelGGL((k_gradq<double, double>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+}
+
 static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
 {
     StageArgs sa;
@@ -1610,7 +1720,8 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     const long N = x->N;
     if (sa.do_prim) {
         ProfScope ps(x, 0);
-        hipLaunchKernelGGL(k_prim, dim3((unsigned)sa.prim_nb), dim3(256), 0, x->stream, sa.prim);
+        if (x->fp32) hipLaunchKernelGGL(k_prim<float>, dim3((unsigned)sa.prim_nb), dim3(256), 0, x->stream, sa.prim);
+        else hipLaunchKernelGGL(k_prim<double>, dim3((unsigned)sa.prim_nb), dim3(256), 0, x->stream, sa.prim);
     }
     if (sa.do_dt) {
         ProfScope ps(x, 1);
@@ -1618,8 +1729,7 @@ static int enqueue_stage(nuc_ctx *x, int in, int stage, int outb, bool epilogue)
     }
     if (sa.do_grad) {
         ProfScope ps(x, 2);
-        if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, sa.ggrid, dim3(GTX, GTY, 1), 0, x->stream, sa.grad);
-        else hipLaunchKernelGGL(k_gradq<double>, sa.ggrid, dim3(GTX, GTY, 1), 0, x->stream, sa.grad);
+        launch_gradq(x, sa.ggrid, sa.grad);
     }
     if (x->forcing) {
         // Const forcing (src/physicsModel.cpp:1368-1677) initialises rhs; the
@@ -1689,7 +1799,7 @@ extern "C" int nuc_autotune(nuc_ctx *x)
 {
     if (!x) return NUC_EINVAL;
     if (x->c.model != NUC_NAVIERSTOKES3D) return NUC_OK;   // no gradient kernel
-    const auto key = std::make_tuple(x->n[0], x->n[1], x->n[2], x->mixed ? 1 : 0);
+    const auto key = std::make_tuple(x->n[0], x->n[1], x->n[2], (int)x->c.precision);
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_gkz.find(key);
@@ -1716,8 +1826,7 @@ extern "C" int nuc_autotune(nuc_ctx *x)
         float tot = 0.f;
         for (int rep = 0; rep < 4; rep++) {
             (void)hipEventRecord(e0, x->stream);
-            if (x->mixed) hipLaunchKernelGGL(k_gradq<float>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
-            else hipLaunchKernelGGL(k_gradq<double>, grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+            launch_gradq(x, grid, G);
             (void)hipEventRecord(e1, x->stream);
             NUC_CHECK(x, hipEventSynchronize(e1));
             float ms = 0.f;
@@ -1858,8 +1967,15 @@ extern "C" int nuc_rhs(nuc_ctx *x, double *rhs_host, double *dt_local)
     enqueue_stage(x, x->next_stage, 0, 0, false);
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
-    if (rhs_host)
-        NUC_CHECK(x, hipMemcpyAsync(rhs_host, x->rhs, 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    if (rhs_host) {
+        const double *r = x->rhs;
+        if (x->fp32) {   // widened through the FP64 view (the state itself is not touched)
+            hipLaunchKernelGGL(k_widen, conv_grid(5 * x->N), dim3(256), 0, x->stream,
+                               reinterpret_cast<const float *>(x->rhs), x->q64, 5 * x->N);
+            r = x->q64;
+        }
+        NUC_CHECK(x, hipMemcpyAsync(rhs_host, r, 5 * x->N * sizeof(double), hipMemcpyDeviceToHost, x->stream));
+    }
     double sc[8];
     NUC_CHECK(x, hipMemcpyAsync(sc, x->scal, sizeof(sc), hipMemcpyDeviceToHost, x->stream));
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
@@ -1908,6 +2024,8 @@ extern "C" int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, doubl
                                     std::to_string(NOALLOC_MAX_CELLS) + " cells (NUC_NOALLOC_MAX_CELLS)");
     if (cfg->forcing == NUC_FORCING_CONST)
         return fail(NUC_EINVAL, "Const forcing needs its transform plan's buffers: use a context");
+    if (cfg->precision == NUC_FP32)
+        return fail(NUC_EINVAL, "the state and rhs buffers are FP64: NUC_FP32 needs a context");
     int dev = -1;
     if (hipGetDevice(&dev) != hipSuccess) return fail(NUC_ENODEV, "no current device");
     double *base = nullptr;
@@ -1939,7 +2057,7 @@ extern "C" int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, doubl
     x.scal = base + NOALLOC_FV + NOALLOC_LAM;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune_gkz.find(std::make_tuple(x.n[0], x.n[1], x.n[2], x.mixed ? 1 : 0));
+        auto it = g_tune_gkz.find(std::make_tuple(x.n[0], x.n[1], x.n[2], (int)cfg->precision));
         if (it != g_tune_gkz.end()) x.gkz = it->second;
     }
     // the array keeps the last call's contents: the eigenvalue maxima start
@@ -1972,15 +2090,16 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
         NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&prim), 4 * N * sizeof(double), x->stream));
         own_prim = true;
     }
+    const double *q64 = state64(x);
     PrimArgs P{};
-    for (int v = 0; v < 5; v++) P.q[v] = gp(x->q[x->next_stage] + v * N);
+    for (int v = 0; v < 5; v++) P.q[v] = gp(q64 + v * N);
     for (int f = 0; f < 4; f++) P.prim[f] = gp(prim + f * N);
     P.N = N; P.nx = x->n[0]; P.ny = x->n[1];
     P.nlx = x->nloc[0]; P.nly = x->nloc[1]; P.nlz = x->nloc[2]; P.npx = x->np[0]; P.npy = x->np[1]; P.P = x->P;
     P.riemann = c.riemann; P.do_lam = 0; P.write_prim = 1; P.gamma = c.gamma; P.mach = c.mach; P.lam = gp(x->lam);
     long nbp = (N + 4095) / 4096;
     if (nbp > 65536) nbp = 65536;
-    hipLaunchKernelGGL(k_prim, dim3((unsigned)nbp), dim3(256), 0, x->stream, P);
+    hipLaunchKernelGGL(k_prim<double>, dim3((unsigned)nbp), dim3(256), 0, x->stream, P);
     dim3 grid((x->n[0] + 63) / 64, (x->n[1] + 3) / 4, x->n[2]);
     long nb = (long)grid.x * grid.y * grid.z;
     if (nb > x->stat_nb) {
@@ -1993,7 +2112,7 @@ extern "C" int nuc_stats_ex(nuc_ctx *x, double *out, int nout)
         NUC_CHECK(x, hipMallocAsync(reinterpret_cast<void **>(&x->stat_out), NSTAT * sizeof(double), x->stream));
     StatArgs S{};
     for (int f = 0; f < 4; f++) S.prim[f] = prim + f * N;
-    S.rho = x->q[x->next_stage];
+    S.rho = q64;
     S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
     for (int d = 0; d < 3; d++) {
         S.nloc[d] = x->nloc[d]; S.tsv[d] = x->tsv[d];
@@ -2480,7 +2599,7 @@ extern "C" int nuc_average_update(nuc_ctx *x)
     double *b = x->post_buf;
     const nuc_config &c = x->c;
     PostArgs A{};
-    A.q = gp(x->q[x->next_stage]);
+    A.q = gp(state64(x));
     A.tp = gp(b + L.tp); A.ap = gp(b + L.ap); A.tg = gp(b + L.tg); A.ag = gp(b + L.ag);
     if (x->post_type >= 2) { A.td = gp(b + L.td); A.ad = gp(b + L.ad); }
     if (x->post_type >= 3) { A.tt = gp(b + L.tt); A.at = gp(b + L.at); A.bu = gp(b + L.bu); }
@@ -2784,7 +2903,7 @@ static int init_analytic(nuc_ctx *x, int kind)
     const nuc_config &c = x->c;
     AnalyticIcArgs A{};
     A.kind = kind;
-    A.q = gp(x->q[0]);
+    A.q = gp(state64_w(x));
     A.nx = c.nx; A.ny = c.ny; A.nz = c.nz;
     for (int d = 0; d < 3; d++) A.nl[d] = x->nloc[d];
     A.N = x->N;
@@ -2794,6 +2913,7 @@ static int init_analytic(nuc_ctx *x, int kind)
     A.ox = c.origin[0]; A.oy = c.origin[1]; A.oz = c.origin[2];
     x->next_stage = 0;
     hipLaunchKernelGGL(k_ic_analytic, dim3((unsigned)((x->N + 255) / 256)), dim3(256), 0, x->stream, A);
+    state64_commit(x);
     NUC_CHECK(x, hipGetLastError());
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
     return NUC_OK;
@@ -2841,7 +2961,8 @@ extern "C" int nuc_init_hit_device(nuc_ctx *x, double k0, double u_rms, unsigned
         A.tw[d] = gp((const double2 *)(tw + o));
         o += S.tw[d].size();
     }
-    A.g = gp(g); A.h = gp(h); A.vel = gp(vel); A.q = gp(x->q[0]);
+    double *q0 = state64_w(x);
+    A.g = gp(g); A.h = gp(h); A.vel = gp(vel); A.q = gp(q0);
     for (int d = 0; d < 3; d++) { A.n[d] = S.n[d]; A.W[d] = S.W[d]; }
     A.N = N;
     A.gamma = x->c.gamma; A.mach = x->c.mach; A.scale_num = u_rms;
@@ -2851,10 +2972,11 @@ extern "C" int nuc_init_hit_device(nuc_ctx *x, double k0, double u_rms, unsigned
     hipLaunchKernelGGL(k_ic_y, blocks(nh), dim3(256), 0, x->stream, A);
     hipLaunchKernelGGL(k_ic_z, blocks(3 * (size_t)N), dim3(256), 0, x->stream, A);
     hipLaunchKernelGGL(k_ic_sq, blocks((size_t)N), dim3(256), 0, x->stream, A);
-    hipLaunchKernelGGL(k_post_mean_part, dim3(POST_PART, 1), dim3(256), 0, x->stream, (const double *)x->q[0], N, part);
+    hipLaunchKernelGGL(k_post_mean_part, dim3(POST_PART, 1), dim3(256), 0, x->stream, (const double *)q0, N, part);
     hipLaunchKernelGGL(k_post_mean_fin, dim3(1), dim3(256), 0, x->stream, (const double *)part, 1.0 / (double)N, mean);
     A.mean_sq = gp((const double *)mean);
     hipLaunchKernelGGL(k_ic_q, blocks((size_t)N), dim3(256), 0, x->stream, A);
+    state64_commit(x);
     NUC_CHECK(x, hipGetLastError());
     (void)hipFreeAsync(buf, x->stream);
     NUC_CHECK(x, hipStreamSynchronize(x->stream));
@@ -3048,8 +3170,9 @@ extern "C" int nuc_q_fields(nuc_ctx *x, double *out)
     }
     double *uvw = buf, *f = buf + 3 * N;
     const dim3 grid((unsigned)((N + 255) / 256)), blk(256);
+    const double *q64 = state64(x);
     SpecPrep P{};
-    P.q = gp((const double *)x->q[x->next_stage]);
+    P.q = gp(q64);
     P.uvw = gp(uvw);
     P.what = 3;
     P.N = N;
@@ -3062,7 +3185,7 @@ extern "C" int nuc_q_fields(nuc_ctx *x, double *out)
                            x->n[1], d, x->n[d], x->nloc[d], x->tsv[d], vis_id(x->c.vis_scheme[d]), 1.0 / x->h[d]};
             hipLaunchKernelGGL(k_post_diff, grid, blk, 0, x->stream, D);
         }
-    QArgs Q{gp((const double *)x->q[x->next_stage]), gp((const double *)uvw), gp(f), N};
+    QArgs Q{gp(q64), gp((const double *)uvw), gp(f), N};
     hipLaunchKernelGGL(k_qfields, grid, blk, 0, x->stream, Q);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, f, 16 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost, x->stream);
@@ -3096,7 +3219,7 @@ static int spectra_run(nuc_ctx *x, int what, double *out, int kmax_out)
     double *sum = part + (size_t)SPEC_BLOCKS * nh;
     double *uvw = sum + nh, *grad = uvw + 3 * N;
     SpecPrep P{};
-    P.q = gp((const double *)x->q[x->next_stage]);
+    P.q = gp(state64(x));
     for (int c = 0; c < nf; c++) P.c[c] = gp(cf + (size_t)c * N);
     P.what = what;
     P.N = N;
@@ -3170,7 +3293,8 @@ extern "C" int nuc_spectrum_scalar(nuc_ctx *x, int field, double *out, int kmax)
 // ---------------------------------------------------------------------------
 static bool packable(const nuc_ctx *x)
 {
-    if (x->forcing || x->sources) return false;
+    // the packed kernels take FP64 state (the FP32-state path runs alone)
+    if (x->forcing || x->sources || x->fp32) return false;
     for (int d = 0; d < 3; d++)
         if (x->cpt[d] > PACK_RMAX) return false;
     return true;
@@ -3251,7 +3375,7 @@ extern "C" int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nu
         for (int k = 0; k < i; k++)
             if (members[k] == members[i]) { g_pack_err = "context listed twice in pack"; return NUC_EINVAL; }
         if (!packable(members[i])) {
-            g_pack_err = "context not packable (forcing on, or a line longer than 128 cells)";
+            g_pack_err = "context not packable (forcing or sources on, NUC_FP32, or a line longer than 128 cells)";
             return NUC_EINVAL;
         }
         if (members[i]->next_stage != 0) {

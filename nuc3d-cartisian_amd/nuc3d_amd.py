@@ -21,7 +21,7 @@ EULER3D, NAVIERSTOKES3D = 0, 1
 SUTHERLAND, CONSTANT = 0, 1
 FORCING_NONE, FORCING_CONST, FORCING_ABC, FORCING_TG = 0, 1, 2, 3
 NSTATS_EX = 27
-FP64, MIXED = 0, 1
+FP64, MIXED, FP32 = 0, 1, 2
 SCHEMES = {"weno5js": WENO5JS, "weno5z": WENO5Z, "weno7js": WENO7JS, "crweno5": CRWENO5, "hccs": HCCS}
 STATUS = {0: "NUC_OK", 1: "NUC_EINVAL", 2: "NUC_ENOMEM", 3: "NUC_EHIP", 4: "NUC_ENONFINITE", 5: "NUC_ENODEV",
           6: "NUC_EBUSY"}

@@ -204,3 +204,29 @@ def test_create_failure_message_without_a_context():
     assert "unknown inviscid scheme" in L.nuc_last_error(None).decode()
     with pytest.raises(G.NucError, match="divide evenly"):
         G.Context(bad)
+
+
+def test_fp32_rejects_what_it_does_not_take():
+    """NUC_FP32 refuses hccs/hccs77 (their theta blend needs FP64 stencils),
+    forcing and rotating-frame sources, and the caller-buffer RHS entry point
+    (whose buffers are FP64), with NUC_EINVAL before any device call; the
+    footprint counts FP32 state/RHS/viscous fields and the FP64 view."""
+    L = G.lib()
+    h = ctypes.c_void_p()
+    bad = [dict(scheme=G.HCCS), dict(scheme=(G.WENO5JS, G.HCCS77, G.WENO5JS)),
+           dict(forcing=G.FORCING_CONST, forcing_k=2.0), dict(forcing=G.FORCING_ABC),
+           dict(forcing=G.FORCING_TG), dict(coriolis=True, rossby=1.0)]
+    for kw in bad:
+        cfg = G.make_config(n=(16, 16, 16), precision=G.FP32, **kw)
+        assert L.nuc_create(ctypes.byref(cfg), ctypes.byref(h)) == 1, kw
+        assert "NUC_FP32" in L.nuc_last_error(None).decode(), kw
+        assert not h.value
+    ok = G.make_config(n=(16, 16, 16), precision=G.FP32)
+    buf = np.zeros(5 * 16 ** 3)
+    rc = L.nuc_rhs_noalloc(ctypes.byref(ok), buf.ctypes.data_as(ctypes.c_void_p),
+                           buf.ctypes.data_as(ctypes.c_void_p), None, None)
+    assert rc == 1 and "NUC_FP32" in L.nuc_last_error(None).decode()
+    n = 64 ** 3
+    f64, fmx, f32 = (G.footprint(G.make_config(n=(64, 64, 64), precision=p)) for p in (G.FP64, G.MIXED, G.FP32))
+    # per cell: FP64 20 x 8 + 9 x 8, mixed 20 x 8 + 9 x 4, FP32 20 x 4 + 9 x 4 + 5 x 8 (+ small tables)
+    assert abs((f64 - fmx) / n - 36) < 0.01 and abs((fmx - f32) / n - 40) < 0.01
