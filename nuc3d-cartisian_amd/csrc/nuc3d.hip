@@ -73,6 +73,9 @@ struct SweepArgs {
     int dir, n, nloc, ts, tsv, vis, riemann;
     int nx, ny, nz;
     long nlines;
+    // lines longer than 512 cells: nseg segments of n cells each (n is then the
+    // segment length and nfull the line's); otherwise nseg = 1, nfull = n
+    int nseg, nfull;
     double inv_h, gamma;
     // state fields: element type S of the kernel (double; float with NUC_FP32)
     const void GMEM *q[5];
@@ -154,7 +157,7 @@ __device__ __forceinline__ void prim_body(const PrimArgs &A, long bid, long nbk)
         }
         if (A.do_lam) {
             double c = sqrt(A.gamma * p * ir);
-            // owning rank of the cell; a box holds at most 512^3 cells, so
+            // owning rank of the cell; a box holds at most 2^29 cells, so
             // 32-bit index arithmetic suffices (and one rank needs none)
             int rank = 0;
             if (A.P > 1) {
@@ -520,7 +523,11 @@ __device__ __forceinline__ void sync_lines(int d)
 // phase to the update; the other forms read them again in the final phase,
 // so that the first two sweeps do not hold them across the reconstruction.
 // blk: the workgroup's logical index within its job (line group blk*LPB ..)
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double>
+// SEG: a wave owns one segment of a line longer than 512 cells instead of a
+// whole line; the 2*HALO cells around the segment are read from the
+// neighbouring segments (periodic over the full line) and the segment's n+1
+// interfaces are all computed, so that no periodic image is needed.
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double, bool SEG = false>
 __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
 {
     const bool epi = EPI == EPI_ARG ? A.epilogue != 0 : EPI == EPI_YES;
@@ -536,30 +543,41 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     __shared__ T sV[LPB * NV];
     __shared__ double sRed[LPB];
     const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
-    const int n = A.n, d = A.dir;
+    const int n = A.n, d = A.dir;   // SEG: n is the segment length
     int ll, g;
     if (d == 0) { ll = wave; g = lane; }
     else { ll = t % LPB; g = t / LPB; }
-    const long L = blk * LPB + ll;
-    const bool lv = FULL || L < A.nlines;
+    const long L = blk * LPB + ll;   // SEG: the segment L % nseg of line L / nseg
+    const long nll = SEG ? A.nlines * A.nseg : A.nlines;
+    const bool lv = FULL || L < nll;
+    long line = L;
+    int seg = 0;
+    if constexpr (SEG) { line = L / A.nseg; seg = (int)(L - line * A.nseg); }
     long base = 0, stride = 1;
-    if (d == 0) { base = L * A.nx; stride = 1; }
-    else if (d == 1) { long i = L % A.nx, k = L / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
-    else { long i = L % A.nx, j = L / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
+    if (d == 0) { base = line * A.nx; stride = 1; }
+    else if (d == 1) { long i = line % A.nx, k = line / A.nx; base = i + (long)A.nx * A.ny * k; stride = A.nx; }
+    else { long i = line % A.nx, j = line / A.nx; base = i + (long)A.nx * j; stride = (long)A.nx * A.ny; }
+    const long lbase = base;          // the full line's first cell
+    const int off = SEG ? seg * n : 0;   // the segment's first cell on the line
+    base += (long)off * stride;
 
     // byte offsets of the thread's cells within a field: a field holds at
-    // most 512^3 doubles (1 GiB), so 32 bits suffice, and every access is the
+    // most 2^29 doubles (4 GiB), so 32 bits suffice, and every access is the
     // field's base address (uniform, in scalar registers) plus this offset
     unsigned ob[CPL];
 #pragma unroll
     for (int k = 0; k < CPL; k++) ob[k] = (unsigned)(base + (long)(g + 64 * k) * stride) * 8u;
-    const unsigned fb = (unsigned)((long)A.nlines * n * 8);   // bytes of one FP64 field
+    // bytes of one FP64 field (the buffer's bound: 2^32 - 1 at the largest box)
+    const unsigned fb = (unsigned)std::min<long>((long)A.nlines * (SEG ? A.nfull : n) * 8, 0xFFFFFFFFl);
     // state and RHS fields hold S (ob, fb are for 8-byte elements)
     constexpr unsigned SSH = sizeof(S) == 8 ? 0 : 1;
     auto lds = [&](const void GMEM *p, int k) -> double {
         return double(ld_at(static_cast<const S GMEM *>(p), ob[k] >> SSH, fb >> SSH));
     };
     auto sts = [&](void GMEM *p, int k, double val) { st_at(static_cast<S GMEM *>(p), ob[k] >> SSH, fb >> SSH, S(val)); };
+    auto ldo = [&](const void GMEM *p, unsigned o) -> double {
+        return double(ld_at(static_cast<const S GMEM *>(p), o >> SSH, fb >> SSH));
+    };
 
     // split parameters of the own cells (con2prim, src/physicsModel.cpp:331-344).
     // The compact schemes keep them in LDS (one column per thread, read back
@@ -584,8 +602,26 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             if constexpr (SP_LDS) sSP[k * NSP + j][t] = tmp[j];
             else spr[k][j] = tmp[j];
         }
-        int cl = c % A.nloc;
+        int cl = (c + off) % A.nloc;
         if (cl >= A.tsv && cl < A.nloc - A.tsv) innermask |= 1u << k;
+    }
+    // SEG: the thread g < 2*HALO of a line owns its segment's halo cell hc
+    // (-HALO..-1 left, n..n+HALO-1 right), a cell of the neighbouring segment
+    // (periodic over the full line); its split parameters stay in registers
+    const bool hown = SEG && lv && g < 2 * HALO;
+    int hc = 0;
+    unsigned obh = 0;
+    T sph[5];
+    if constexpr (SEG) {
+        hc = g < HALO ? g - HALO : n + g - HALO;
+        long gpos = (long)off + hc;
+        if (gpos < 0) gpos += A.nfull;
+        else if (gpos >= A.nfull) gpos -= A.nfull;
+        obh = (unsigned)(lbase + gpos * stride) * 8u;
+        T q[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) q[v] = hown ? T(ldo(A.q[v], obh)) : (v == 0 ? T(1) : T(0));
+        split_params<RIE, T>(q, d, T(A.gamma), sph, A.riemann == RIE_AUSMP);
     }
 
     // f^+ / f^- line buffers (skewed, f^- mirrored: Skew<R>::at / mir) and the
@@ -596,8 +632,11 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     T *Vl = sV + ll * NV + HALO;
     double *Pw = sP + wave * NP, *Mw = sM + wave * NP;                 // interface values
     double *Pd = sP + ll * NP, *Md = sM + ll * NP;
-    const bool wv = FULL || blk * LPB + wave < A.nlines;
-    const Zone z{n, A.nloc, A.ts};
+    const bool wv = FULL || blk * LPB + wave < nll;
+    // the zones of the line this wave reconstructs (line `wave` of the
+    // workgroup; in the y/z sweeps not the line of the thread's own cells)
+    const int offw = SEG ? (int)((blk * LPB + wave) % A.nseg) * n : 0;
+    const Zone z{SEG ? n + 1 : n, A.nloc, A.ts, offw};
     // line-buffer slots of the thread's cells c = g + 64k and of c + 1 (f^+
     // skewed, f^- mirrored).  When R divides 64 they are affine in k (the
     // pads between cells 64 apart are 64/R), so every access is one base
@@ -641,6 +680,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         }
     };
     fetch(0, qv, xv);
+    double qh = 0.0;   // SEG: the halo cell's values of the current variable
+    T xh = T(0);
+    auto fetch_h = [&](int v, double &qo, T &xo) {
+        qo = 0.0;
+        xo = T(0);
+        if (hown && v < 5) {
+            qo = ldo(A.q[v], obh);
+            if (A.vis != VIS_NONE && v > 0)
+                xo = A.fv_f32 ? T(ld_at(static_cast<const float GMEM *>(A.fv[v - 1]), obh >> 1, fb >> 1))
+                              : T(ld_at(static_cast<const double GMEM *>(A.fv[v - 1]), obh, fb));
+        }
+    };
+    if constexpr (SEG) fetch_h(0, qh, xh);
 #pragma unroll 1
     for (int v = 0; v < 5; v++) {
 #pragma unroll
@@ -661,16 +713,27 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
                 // copied past the other end, all others to a dummy slot; with
                 // n = 32R only the first and last cell of a thread can be near
                 // an end, so the other cells skip the copy
-                if (!FULL || k == 0 || k == CPL - 1) {
+                if (!SEG && (!FULL || k == 0 || k == CPL - 1)) {
                     const int cw = (k == 0 && c < HALO) ? c + n : (c >= n - HALO ? c - n : 32 * R + HALO);
                     Pl[Skew<R>::at(cw)] = fp; Ml[Skew<R>::mir(cw)] = fm;
                     if (A.vis != VIS_NONE && v > 0) Vl[cw] = xv[k];
                 }
             }
         }
+        if constexpr (SEG) {
+            if (hown) {
+                T fp, fm;
+                split_var<RIE, T>(sph, T(qh), v, d, fp, fm);
+                Pl[Skew<R>::at(hc)] = fp; Ml[Skew<R>::mir(hc)] = fm;
+                if (A.vis != VIS_NONE && v > 0) Vl[hc] = xh;
+            }
+        }
         double qnx[CPL];
         T xnx[CPL];
         fetch(v + 1, qnx, xnx);   // v + 1 = 5 gives zeros
+        double qhn = 0.0;
+        T xhn = T(0);
+        if constexpr (SEG) fetch_h(v + 1, qhn, xhn);
         sync_lines(d);
         double out[R];
         // weights and rows in T where FP32 keeps them in range; the Z-type
@@ -689,15 +752,19 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
             // for 0 < r < R
             double *Ob = (neg ? Mw : Pw) + h * (R + 1);
             double *Ow = Ob - (neg ? 1 : 0);
-            if (FULL || m0r < n) Ob[neg ? Skew<R>::at(-1) : Skew<R>::at(0)] = out[0];
+            // a segment's interfaces are 0..n, a periodic line's 0..n-1
+            const int nif = SEG ? n + 1 : n;
+            if (FULL || m0r < nif) Ob[neg ? Skew<R>::at(-1) : Skew<R>::at(0)] = out[0];
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 int m = m0r + mstep * r;
-                if (FULL || m < n) Ow[Skew<R>::at(r)] = out[r];
+                if (FULL || m < nif) Ow[Skew<R>::at(r)] = out[r];
             }
             // periodic copy of interface 0 to interface n
-            if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
-            if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
+            if constexpr (!SEG) {
+                if (!neg && h == 0) Pw[Skew<R>::at(n)] = out[0];
+                if (neg && h == 31) Mw[Skew<R>::mir(n)] = out[R - 1];
+            }
         }
         // the final phase's own global reads (rhs of the previous sweeps; u^n and
         // the current state of the RK update), issued after the reconstruction:
@@ -751,6 +818,7 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
         // phase instead of being carried to the end of the iteration
 #pragma unroll
         for (int k = 0; k < CPL; k++) { qv[k] = qnx[k]; xv[k] = xnx[k]; }
+        if constexpr (SEG) { qh = qhn; xh = xhn; }
     }
     if (epi && A.stage == 2) {
         // residual partial of this workgroup, fixed reduction order (src/PDEData3d.cpp:93-128)
@@ -766,10 +834,10 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     }
 }
 
-template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double>
+template <int SCH, int RIE, int R, int LPB, typename T, bool FULL, int EPI, typename S = double, bool SEG = false>
 __global__ __launch_bounds__(64 * LPB) void k_sweep(SweepArgs A)
 {
-    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI, S>(A, xcd_block(blockIdx.x, gridDim.x));
+    sweep_body<SCH, RIE, R, LPB, T, FULL, EPI, S, SEG>(A, xcd_block(blockIdx.x, gridDim.x));
 }
 
 struct ResArgs {
@@ -1019,9 +1087,35 @@ constexpr bool mixed_fp32()
     return SCH != HCCS && SCH != HCCS77;
 }
 
+constexpr bool compact_scheme(int sch) { return sch == CRWENO5 || sch == HCCS || sch == HCCS77; }
+
+// segments of lines longer than 512 cells: R = 8 (up to 255 cells), and for
+// the compact schemes R = 16 (one rank block of up to 511 cells)
+template <int SCH, int RIE, int R>
+void launch_sweep_seg(const SweepArgs &A, hipStream_t s)
+{
+    const long nb = (A.nlines * A.nseg + SWEEP_LPB - 1) / SWEEP_LPB;
+    const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
+    if constexpr (mixed_fp32<SCH>()) {
+        if (A.fp32) {
+            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG, float, true>), grid, block, 0, s, A);
+            return;
+        }
+        if (A.mixed) {
+            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG, double, true>), grid, block, 0, s, A);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false, EPI_ARG, double, true>), grid, block, 0, s, A);
+}
+
 template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
+    if (A.nseg > 1) {
+        if constexpr (R == 8 || (R == 16 && compact_scheme(SCH))) launch_sweep_seg<SCH, RIE, R>(A, s);
+        return;
+    }
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
     const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
@@ -1215,6 +1309,7 @@ struct nuc_ctx {
     hipStream_t stream = nullptr;
     long N = 0;
     int n[3], np[3], nloc[3], ts[3], tsv[3], cpt[3];
+    int seg[3], nseg[3];   // sweep segment length and count per line (nseg = 1 up to 512 cells)
     double h[3];
     int P = 1;
     double *q[3] = {nullptr, nullptr, nullptr};
@@ -1310,6 +1405,28 @@ static int vis_id(int vis) { return vis == NUC_CD8 ? VIS_CD8 : (vis == NUC_CD2 ?
 // behind weno3js / upwind1st read outside the strip.  Widened to the reach.
 static int vis_tilesize(int scheme, int vis) { return std::max(scheme_halo(scheme), vis_reach(vis)); }
 
+// Lines longer than 512 cells are swept in segments of ls cells, one wave
+// each, with the neighbouring segments' cells as halo.  Explicit schemes: the
+// largest divisor of n in [64, 255].  Compact schemes, whose tridiagonal
+// systems end at rank-block edges (src/crweno5.cpp:83-206): whole rank blocks,
+// k blocks of together at most 255 cells, or one block of at most 511.
+// 0: no such segment.  Up to 512 cells a line is one segment.
+static int segment_length(int n, int nloc, int np, int scheme)
+{
+    if (n <= 512) return n;
+    if (compact_scheme(scheme)) {
+        if (nloc > 511) return 0;
+        if (nloc > 255) return nloc;
+        int best = nloc;
+        for (int k = 1; k <= np; k++)
+            if (np % k == 0 && (long)k * nloc <= 255) best = k * nloc;
+        return best;
+    }
+    for (int ls = 255; ls >= 64; ls--)
+        if (n % ls == 0) return ls;
+    return 0;
+}
+
 static int validate(const nuc_config *c, std::string &err)
 {
     int n[3] = {c->nx, c->ny, c->nz}, np[3] = {c->npx, c->npy, c->npz};
@@ -1322,8 +1439,15 @@ static int validate(const nuc_config *c, std::string &err)
             err = "rank-local line too short for the scheme's halo";
             return NUC_EINVAL;
         }
-        if (n[d] > 512) { err = "line length above 512 is not supported"; return NUC_EINVAL; }
+        if (segment_length(n[d], n[d] / np[d], np[d], c->scheme[d]) == 0) {
+            err = compact_scheme(c->scheme[d])
+                      ? "a compact scheme on a line above 512 cells needs rank blocks of at most 511 cells"
+                      : "a line above 512 cells needs a divisor in [64, 255] to be swept in segments";
+            return NUC_EINVAL;
+        }
     }
+    // the sweeps address a field with 32-bit byte offsets
+    if ((long)c->nx * c->ny * c->nz > (1l << 29)) { err = "box above 2^29 cells (4 GiB per FP64 field)"; return NUC_EINVAL; }
     if ((long)c->npx * c->npy * c->npz > MAXRANK) { err = "at most 64 ranks"; return NUC_EINVAL; }
     if (c->riemann < 0 || c->riemann > NUC_AUSMP || c->model < 0 || c->model > 1) { err = "bad riemann/model id"; return NUC_EINVAL; }
     if (c->forcing < NUC_FORCING_NONE || c->forcing > NUC_FORCING_TG) { err = "unknown forcing id"; return NUC_EINVAL; }
@@ -1378,7 +1502,9 @@ static void ctx_geometry(nuc_ctx *x, const nuc_config *cfg)
         x->h[d] = Lb[d] / x->n[d];
         x->ts[d] = scheme_halo(cfg->scheme[d]);
         x->tsv[d] = vis_tilesize(cfg->scheme[d], cfg->vis_scheme[d]);
-        x->cpt[d] = rows_for(x->n[d]);
+        x->seg[d] = segment_length(x->n[d], x->nloc[d], x->np[d], cfg->scheme[d]);
+        x->nseg[d] = x->n[d] / x->seg[d];
+        x->cpt[d] = x->nseg[d] > 1 ? (x->seg[d] <= 255 ? 8 : 16) : rows_for(x->n[d]);
     }
 }
 
@@ -1446,7 +1572,7 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     if (!dalloc(&x->scal, 8 * sizeof(double))) return fail("device allocation (scal)");
     // residual partials: one per workgroup of the last sweep
     const int dl = kSweepOrder[2];
-    long lines_last = x->N / x->n[dl];
+    long lines_last = x->N / x->n[dl] * x->nseg[dl];
     x->res_nb = (lines_last + sweep_lpb(x->cpt[dl]) - 1) / sweep_lpb(x->cpt[dl]);
     if (!dalloc(&x->res_part, x->res_nb * sizeof(double))) return fail("device allocation (residual)");
     (void)hipMemsetAsync(x->lam, 0, 3 * MAXRANK * sizeof(unsigned long long), x->stream);
@@ -1602,7 +1728,8 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
     static const int fvmap[3][4] = {{0, 3, 4, 6}, {3, 1, 5, 7}, {4, 5, 2, 8}};
     for (int d = 0; d < 3; d++) {
         SweepArgs &S = sa.sw[d];
-        S.dir = d; S.n = x->n[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
+        S.dir = d; S.n = x->seg[d]; S.nloc = x->nloc[d]; S.ts = x->ts[d]; S.tsv = x->tsv[d];
+        S.nseg = x->nseg[d]; S.nfull = x->n[d];
         S.vis = ns ? vis_id(c.vis_scheme[d]) : VIS_NONE;
         S.riemann = c.riemann;
         S.nx = x->n[0]; S.ny = x->n[1]; S.nz = x->n[2];
@@ -1625,7 +1752,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
             S.dtp = gp(x->scal);
             S.res_part = gp(x->res_part);
         }
-        sa.sw_nb[d] = (S.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
+        sa.sw_nb[d] = (S.nlines * S.nseg + SWEEP_LPB - 1) / SWEEP_LPB;
     }
     sa.do_res = epilogue && stage == 2;
     sa.res = ResArgs{gp(x->res_part), (int)x->res_nb, 1.0 / ((double)N * 5.0), gp(x->scal)};
@@ -3296,7 +3423,7 @@ static bool packable(const nuc_ctx *x)
     // the packed kernels take FP64 state (the FP32-state path runs alone)
     if (x->forcing || x->sources || x->fp32) return false;
     for (int d = 0; d < 3; d++)
-        if (x->cpt[d] > PACK_RMAX) return false;
+        if (x->cpt[d] > PACK_RMAX || x->nseg[d] > 1) return false;
     return true;
 }
 

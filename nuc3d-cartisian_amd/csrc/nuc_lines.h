@@ -23,9 +23,10 @@ namespace nucdev {
 constexpr int HALO = 4;
 
 struct Zone {
-    int nrows;  // interfaces on this line (n periodic, n+1 with ghost strips)
-    int nloc;   // rank-local line length
-    int ts;     // inviscid tilesize of the scheme
+    int nrows;    // interfaces on this line (n periodic, n+1 with ghost strips)
+    int nloc;     // rank-local line length
+    int ts;       // inviscid tilesize of the scheme
+    int off = 0;  // position of interface 0 on the full line (a segment of a long line)
 };
 
 // reciprocal: hardware estimate refined by two Newton steps (|err| <= 1 ulp)
@@ -272,7 +273,7 @@ __device__ __forceinline__ void line_half(const T *Lw, int m0r, int mstep, const
     int ml[R];
     {
         int m = m0r;
-        int l = m % z.nloc;
+        int l = (m + z.off) % z.nloc;
 #pragma unroll
         for (int r = 0; r < R; r++) {
             ml[r] = l;

@@ -462,3 +462,87 @@ def _source_case(case, origin):
     err = _rel_linf(q_gpu, q_ref, N)
     print("10-step rel Linf", err, flush=True)
     assert np.all(err <= 1e-12), err
+
+
+# Lines longer than 512 cells: swept in segments (one wave per segment, the
+# neighbouring segments' cells as halo, the segment's n+1 interfaces computed
+# on both sides of each cut).  Explicit schemes cut at a divisor of n in
+# [64, 255], with rank blocks (closure and inner zones) falling inside or
+# across the cuts; the compact schemes cut at rank-block edges (segments of
+# 128, 256 or 2 x 100 cells; R = 8 and R = 16).  Same policy as LONG_CASES:
+# 10 steps against the oracle, max(1e-12, 3 x the measured 1-ulp floor).
+SEG_CASES = [
+    # n, rank grid, scheme, vis, riemann, model
+    ((1024, 10, 10), (1, 1, 1), G.WENO5JS, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((10, 768, 10), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 12, 1536), (1, 1, 1), G.WENO7JS, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((600, 10, 10), (1, 1, 1), G.WENO3JS, G.CD2, G.AUSMP, G.NAVIERSTOKES3D),
+    ((10, 1040, 10), (1, 1, 1), G.UPWIND1ST, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((1024, 10, 10), (2, 1, 1), G.HCCS78, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((1000, 10, 10), (5, 1, 1), G.WENO5Z, G.CD6, G.LF, G.NAVIERSTOKES3D),
+    ((10, 10, 2048), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSM, G.EULER3D),
+    ((1024, 10, 10), (4, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((10, 1536, 10), (1, 12, 1), G.CRWENO5, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((10, 10, 1200), (1, 1, 12), G.CRWENO5, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((10, 10, 1024), (1, 1, 8), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((768, 10, 10), (3, 1, 1), G.HCCS77, G.CD8, G.LF, G.NAVIERSTOKES3D),
+]
+
+
+@pytest.mark.parametrize("case", SEG_CASES, ids=lambda c: f"{c[0]}-{c[1]}-s{c[2]}-v{c[3]}-r{c[4]}-m{c[5]}")
+def test_segmented_long_lines_ten_steps_match_oracle(case):
+    n, npr, scheme, vis, rie, model = case
+    print(f"segmented long-line case {case}", flush=True)
+    cfl = 0.5
+    if rie == G.LF and n[2] > min(n[0], n[1]):
+        cfl = 0.4 * min(n[0], n[1]) / n[2]   # LF's dt omits the z wave speed (src/physicsModel.cpp:653-654)
+    L = tuple(2 * np.pi * m / max(n) for m in n)   # cubic cells
+    kw = dict(n=n, npr=npr, L=L, scheme=scheme, vis=vis, riemann=rie, model=model, mach=0.1, reynolds=1600.0,
+              cfl=cfl)
+    gc, oc = G.make_config(**kw), O.make_config(**kw)
+    oc.nthreads = 8
+    q = _perturbed_tgv(oc, 5)
+    N = oc.nx * oc.ny * oc.nz
+    s = O.Solver(oc)
+    s.set_state(q)
+    dt_ref, res_ref, t_ref = s.step(10)
+    q_ref = s.get_state()
+    with G.Context(gc) as ctx:
+        ctx.upload(q)
+        dt_gpu, res_gpu, t_gpu = ctx.step(10)
+        q_gpu = ctx.download()
+    err = _rel_linf(q_gpu, q_ref, N)
+    print("10-step rel Linf per variable", err, flush=True)
+    tol = np.maximum(1e-12, 3.0 * _ulp_floor(oc, q, 10))
+    print("max(1e-12, 1-ulp floor x3)", tol, flush=True)
+    assert np.all(err <= tol), (err, tol)
+    assert abs(dt_gpu - dt_ref) <= max(1e-13, tol.max()) * dt_ref
+    assert abs(res_gpu - res_ref) <= max(1e-9, 100 * tol.max()) * abs(res_ref)
+
+
+@pytest.mark.parametrize("prec", [G.MIXED, G.FP32], ids=["mixed", "fp32"])
+def test_segmented_long_lines_reduced_precision(prec):
+    """The segmented sweeps in the mixed and FP32 paths: within 2e-5 of FP64
+    after 5 steps (relative L-infinity per variable, momentum scaled by its
+    largest component)."""
+    for n, npr, scheme, rie in (((1024, 12, 12), (1, 1, 1), G.WENO5Z, G.AUSM),
+                                ((12, 1024, 12), (1, 4, 1), G.CRWENO5, G.LF)):
+        print(f"segmented reduced precision {prec} {n} {scheme}", flush=True)
+        L = tuple(2 * np.pi * m / max(n) for m in n)
+        kw = dict(n=n, npr=npr, L=L, scheme=scheme, vis=G.CD8, riemann=rie, model=G.NAVIERSTOKES3D,
+                  viscosity=G.SUTHERLAND, mach=0.2, reynolds=400.0, cfl=0.5)
+        c64, cr = G.make_config(precision=G.FP64, **kw), G.make_config(precision=prec, **kw)
+        q = G.init_hit(c64, k0=3.0, u_rms=0.5, seed=23)
+        outs = []
+        for c in (c64, cr):
+            with G.Context(c) as ctx:
+                ctx.upload(q)
+                ctx.step(5)
+                outs.append(ctx.download())
+        Nc = n[0] * n[1] * n[2]
+        a, b = outs[0].reshape(5, Nc), outs[1].reshape(5, Nc)
+        mom = np.abs(a[1:4]).max()
+        scale = np.array([np.abs(a[0]).max(), mom, mom, mom, np.abs(a[4]).max()])
+        err = np.abs(a - b).max(axis=1) / scale
+        print("vs FP64 rel Linf", err, flush=True)
+        assert np.all(err <= 2e-5), err

@@ -230,3 +230,21 @@ def test_fp32_rejects_what_it_does_not_take():
     f64, fmx, f32 = (G.footprint(G.make_config(n=(64, 64, 64), precision=p)) for p in (G.FP64, G.MIXED, G.FP32))
     # per cell: FP64 20 x 8 + 9 x 8, mixed 20 x 8 + 9 x 4, FP32 20 x 4 + 9 x 4 + 5 x 8 (+ small tables)
     assert abs((f64 - fmx) / n - 36) < 0.01 and abs((fmx - f32) / n - 40) < 0.01
+
+
+def test_long_line_rules():
+    """Lines above 512 cells are swept in segments: explicit schemes need a
+    divisor of n in [64, 255], compact schemes rank blocks of at most 511
+    cells; a box holds at most 2^29 cells (refused with NUC_EINVAL before any
+    device call)."""
+    L = G.lib()
+    h = ctypes.c_void_p()
+    bad = [(dict(n=(1031, 12, 12)), "divisor in [64, 255]"),
+           (dict(n=(1024, 12, 12), scheme=G.CRWENO5), "rank blocks of at most 511"),
+           (dict(n=(1024, 12, 12), npr=(2, 1, 1), scheme=G.HCCS), "rank blocks of at most 511"),
+           (dict(n=(1024, 1024, 1024)), "2^29 cells")]
+    for kw, msg in bad:
+        cfg = G.make_config(**kw)
+        assert L.nuc_create(ctypes.byref(cfg), ctypes.byref(h)) == 1, kw
+        assert msg in L.nuc_last_error(None).decode(), (kw, L.nuc_last_error(None))
+        assert not h.value
