@@ -51,15 +51,17 @@ __device__ __forceinline__ void block_sum(double v[K], double *sred)
 }
 
 // X[c][a][L] = sum_i u_c(i, L) e^{-2 pi i kx_a i / nx}; one block per x line
+// NS: cells per thread (lines of up to 256 NS cells)
+template <int NS>
 __global__ __launch_bounds__(256) void k_fx(Dev D, const double *u, const double *v, const double *w)
 {
     __shared__ double sred[4 * 6];
     const long L = blockIdx.x;
     const int t = threadIdx.x;
-    double uu[2][3];
-    int ii[2];
+    double uu[NS][3];
+    int ii[NS];
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < NS; s++) {
         int i = t + 256 * s;
         ii[s] = i;
         bool ok = i < D.nx;
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(256) void k_fx(Dev D, const double *u, const double
         const int kx = D.kv[a];
         double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
+        for (int s = 0; s < NS; s++) {
             if (ii[s] < D.nx) {
                 int ph = wrapmod(kx * ii[s], D.nx);
                 double cr = D.tw[2 * ph], ci = D.tw[2 * ph + 1];
@@ -89,15 +91,16 @@ __global__ __launch_bounds__(256) void k_fx(Dev D, const double *u, const double
 }
 
 // Y[c][a][b][k] = sum_j X[c][a][j + ny k] e^{-2 pi i ky_b j / ny}; one block per (a, k)
+template <int NS>
 __global__ __launch_bounds__(256) void k_fy(Dev D)
 {
     __shared__ double sred[4 * 6];
     const int a = blockIdx.x % D.na, k = blockIdx.x / D.na;
     const int t = threadIdx.x;
-    double xr[2][3], xi[2][3];
-    int jj[2];
+    double xr[NS][3], xi[NS][3];
+    int jj[NS];
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < NS; s++) {
         int j = t + 256 * s;
         jj[s] = j;
         bool ok = j < D.ny;
@@ -114,7 +117,7 @@ __global__ __launch_bounds__(256) void k_fy(Dev D)
         const int ky = D.kv[MAXK + b];
         double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 2; s++) {
+        for (int s = 0; s < NS; s++) {
             if (jj[s] < D.ny) {
                 int ph = wrapmod(ky * jj[s], D.ny);
                 double cr = twy[2 * ph], ci = twy[2 * ph + 1];
@@ -241,6 +244,7 @@ struct ApplyOut {
     double *r[5];
 };
 
+template <int NS>
 __global__ __launch_bounds__(256) void k_fapply(Dev D, const double *u, const double *v, const double *w, ApplyOut O)
 {
     __shared__ double sg[3 * MAXK * 2];
@@ -254,7 +258,7 @@ __global__ __launch_bounds__(256) void k_fapply(Dev D, const double *u, const do
     __syncthreads();
     double fu_sum = 0.0;
 #pragma unroll
-    for (int s = 0; s < 2; s++) {
+    for (int s = 0; s < NS; s++) {
         int i = t + 256 * s;
         if (i >= D.nx) continue;
         double f[3] = {0.0, 0.0, 0.0};
@@ -375,7 +379,7 @@ int plan_create(Plan &p, const nuc_config &c, hipStream_t s, std::string &err)
     p.nx = c.nx; p.ny = c.ny; p.nz = c.nz;
     p.N = (long)c.nx * c.ny * c.nz;
     p.amp = c.forcing_amp; p.amp_hel = c.forcing_amp_hel;
-    if (c.nx > 512 || c.ny > 512 || c.nz > 512) { err = "forcing: line length above 512"; return NUC_EINVAL; }
+    if (c.nx > 2048 || c.ny > 2048) { err = "forcing: x or y line length above 2048"; return NUC_EINVAL; }
     HostModes h = shell_modes(c);
     p.nm = (int)h.kx.size();
     if (p.nm < 1) { err = "forcing: the shell |k - k_f| < 0.5 holds no resolved mode"; return NUC_EINVAL; }
@@ -439,13 +443,18 @@ void enqueue(const Plan &p, const double *u, const double *v, const double *w, d
 {
     Dev D = make_dev(p);
     const long NL = D.NL;
-    hipLaunchKernelGGL(k_fx, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w);
-    hipLaunchKernelGGL(k_fy, dim3((unsigned)(p.na * p.nz)), dim3(256), 0, s, D);
+    // x and y lines of up to 512 cells take 2 cells per thread, longer ones 8
+    const bool lx = p.nx > 512, ly = p.ny > 512;
+    if (lx) hipLaunchKernelGGL(k_fx<8>, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w);
+    else hipLaunchKernelGGL(k_fx<2>, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w);
+    if (ly) hipLaunchKernelGGL(k_fy<8>, dim3((unsigned)(p.na * p.nz)), dim3(256), 0, s, D);
+    else hipLaunchKernelGGL(k_fy<2>, dim3((unsigned)(p.na * p.nz)), dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_fband, dim3(1), dim3(256), 0, s, D);
     hipLaunchKernelGGL(k_fg, dim3((unsigned)((NL + 255) / 256)), dim3(256), 0, s, D);
     ApplyOut O;
     for (int q = 0; q < 5; q++) O.r[q] = rhs[q];
-    hipLaunchKernelGGL(k_fapply, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w, O);
+    if (lx) hipLaunchKernelGGL(k_fapply<8>, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w, O);
+    else hipLaunchKernelGGL(k_fapply<2>, dim3((unsigned)NL), dim3(256), 0, s, D, u, v, w, O);
     hipLaunchKernelGGL(k_fmean, dim3(1), dim3(256), 0, s, D, mean_out);
 }
 

@@ -213,6 +213,9 @@ FORCING_CASES = [
     ((16, 16, 16), (1, 1, 1), G.WENO5JS, G.CD6, G.LF, G.NAVIERSTOKES3D, 2.0, 0.1),
     ((24, 20, 16), (2, 1, 1), G.CRWENO5, G.CD8, G.AUSM, G.NAVIERSTOKES3D, 1.0, 0.0),
     ((16, 16, 16), (1, 1, 1), G.WENO5Z, G.CD6, G.LF, G.EULER3D, 2.0, 0.3),
+    # x and y lines above 512 cells: segmented sweeps, 8 cells per thread in the transforms
+    ((1024, 12, 12), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D, 2.0, 0.1),
+    ((12, 768, 12), (1, 2, 1), G.CRWENO5, G.CD8, G.AUSM, G.NAVIERSTOKES3D, 1.0, 0.0),
 ]
 
 
@@ -257,7 +260,13 @@ def test_forced_rhs_and_ten_steps_match_oracle(case):
         q_gpu = ctx.download()
     err = _rel_linf(q_gpu, q_ref, N)
     print("forced 10-step rel Linf", err, flush=True)
-    assert np.all(err <= 1e-12), err
+    tol = np.full(5, 1e-12)
+    if max(case[0]) > 512:
+        # the long-line boxes leave their 12-cell sides under-resolved: the
+        # same conditioning allowance as LONG_CASES
+        tol = np.maximum(tol, 3.0 * _ulp_floor(oc, q, 10))
+        print("max(1e-12, 1-ulp floor x3)", tol, flush=True)
+    assert np.all(err <= tol), (err, tol)
 
 
 # Long lines: every chunk length R (lane h holds R consecutive interfaces,
