@@ -533,7 +533,8 @@ __device__ __forceinline__ void sweep_body(const SweepArgs &A, const long blk)
     const bool epi = EPI == EPI_ARG ? A.epilogue != 0 : EPI == EPI_YES;
     constexpr int NP = Skew<R>::pitch(32 * R);
     constexpr int NV = plain_pitch(32 * R);
-    constexpr int CPL = R / 2;             // cells per thread in the load mapping
+    constexpr int CPL = (R + 1) / 2;       // cells per thread in the load mapping (odd R: the last slot of
+                                           // the upper half-wave is past the line)
     constexpr int NSP = (RIE == RIE_LF) ? 3 : 5;
     constexpr int RAD = Radius<SCH>::value;
     // sP/sM hold the split fluxes (as T) and then the interface values (as
@@ -1065,6 +1066,16 @@ int rows_for(int nrows)
     return -1;
 }
 
+// rows per lane of a sweep over a periodic line of n cells: also the odd
+// chunk lengths 3 and 5, so that lines of 65-96 and 129-160 cells do not
+// leave a quarter or a sixth of the lanes idle
+int sweep_rows(int n)
+{
+    int r = (n + 31) / 32;
+    if (r == 3 || r == 5) return r;
+    return rows_for(n);
+}
+
 // lines per workgroup: 4 (one wave per line).  With the sweep's register
 // budget two such workgroups share a CU, so one workgroup's barrier waits
 // overlap the other's work; the y/z load mapping still reads 4 consecutive
@@ -1109,6 +1120,25 @@ void launch_sweep_seg(const SweepArgs &A, hipStream_t s)
     hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false, EPI_ARG, double, true>), grid, block, 0, s, A);
 }
 
+// the kernel forms for one state type S and stencil type T: FULL kernels
+// (n = 32R, even R) with and without the RK3 update, else the one that takes
+// it as an argument
+template <int SCH, int RIE, int R, typename T, typename S>
+void launch_sweep_rt(const SweepArgs &A, hipStream_t s, bool full, dim3 grid, dim3 block)
+{
+    if constexpr (R % 2 == 0) {
+        if (full && A.epilogue) {
+            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, T, true, EPI_YES, S>), grid, block, 0, s, A);
+            return;
+        }
+        if (full) {
+            hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, T, true, EPI_NO, S>), grid, block, 0, s, A);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, T, false, EPI_ARG, S>), grid, block, 0, s, A);
+}
+
 template <int SCH, int RIE, int R>
 void launch_sweep_r(const SweepArgs &A, hipStream_t s)
 {
@@ -1117,35 +1147,20 @@ void launch_sweep_r(const SweepArgs &A, hipStream_t s)
         return;
     }
     long nb = (A.nlines + SWEEP_LPB - 1) / SWEEP_LPB;
-    const bool full = (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
+    // odd R: the load mapping's 64 (R+1)/2 cells overrun a 32R-cell line, so no FULL form
+    const bool full = (R % 2 == 0) && (A.n == 32 * R) && (A.nlines % SWEEP_LPB == 0);
     const dim3 grid((unsigned)nb), block(64 * SWEEP_LPB);
-    // FULL kernels come in a form with and one without the RK3 update; the
-    // others take it as an argument
     if constexpr (mixed_fp32<SCH>()) {
-        if (A.fp32) {
-            // NUC_FP32: FP32 state and RHS around the mixed path's arithmetic
-            if (full && A.epilogue)
-                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_YES, float>), grid, block, 0, s, A);
-            else if (full)
-                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_NO, float>), grid, block, 0, s, A);
-            else
-                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG, float>), grid, block, 0, s, A);
+        if (A.fp32) {   // NUC_FP32: FP32 state and RHS around the mixed path's arithmetic
+            launch_sweep_rt<SCH, RIE, R, float, float>(A, s, full, grid, block);
             return;
         }
         if (A.mixed) {
-            if (full && A.epilogue)
-                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_YES>), grid, block, 0, s, A);
-            else if (full)
-                hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, true, EPI_NO>), grid, block, 0, s, A);
-            else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, float, false, EPI_ARG>), grid, block, 0, s, A);
+            launch_sweep_rt<SCH, RIE, R, float, double>(A, s, full, grid, block);
             return;
         }
     }
-    if (full && A.epilogue)
-        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true, EPI_YES>), grid, block, 0, s, A);
-    else if (full)
-        hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, true, EPI_NO>), grid, block, 0, s, A);
-    else hipLaunchKernelGGL((k_sweep<SCH, RIE, R, SWEEP_LPB, double, false, EPI_ARG>), grid, block, 0, s, A);
+    launch_sweep_rt<SCH, RIE, R, double, double>(A, s, full, grid, block);
 }
 
 template <int SCH, int RIE>
@@ -1153,7 +1168,9 @@ void launch_sweep_sr(int R, const SweepArgs &A, hipStream_t s)
 {
     switch (R) {
     case 2: launch_sweep_r<SCH, RIE, 2>(A, s); break;
+    case 3: launch_sweep_r<SCH, RIE, 3>(A, s); break;
     case 4: launch_sweep_r<SCH, RIE, 4>(A, s); break;
+    case 5: launch_sweep_r<SCH, RIE, 5>(A, s); break;
     case 6: launch_sweep_r<SCH, RIE, 6>(A, s); break;
     case 8: launch_sweep_r<SCH, RIE, 8>(A, s); break;
     default: launch_sweep_r<SCH, RIE, 16>(A, s); break;
@@ -1237,25 +1254,36 @@ struct PackLaunch {
     bool mixed, full;
 };
 
-template <int SCH, int RIE, int R>
-void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
+template <int SCH, int RIE, int R, typename T>
+void launch_sweep_pack_rt(const PackLaunch &P, hipStream_t s)
 {
     const dim3 grid((unsigned)P.nblocks), block(64 * SWEEP_LPB);
-    if constexpr (mixed_fp32<SCH>()) {
-        if (P.mixed) {
-            if (P.full) hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, true>), grid, block, 0, s, P.tab, P.off, P.nj);
-            else hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, float, false>), grid, block, 0, s, P.tab, P.off, P.nj);
+    if constexpr (R % 2 == 0) {
+        if (P.full) {
+            hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, T, true>), grid, block, 0, s, P.tab, P.off, P.nj);
             return;
         }
     }
-    if (P.full) hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, double, true>), grid, block, 0, s, P.tab, P.off, P.nj);
-    else hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, double, false>), grid, block, 0, s, P.tab, P.off, P.nj);
+    hipLaunchKernelGGL((k_sweep_pack<SCH, RIE, R, SWEEP_LPB, T, false>), grid, block, 0, s, P.tab, P.off, P.nj);
+}
+
+template <int SCH, int RIE, int R>
+void launch_sweep_pack_r(const PackLaunch &P, hipStream_t s)
+{
+    if constexpr (mixed_fp32<SCH>()) {
+        if (P.mixed) {
+            launch_sweep_pack_rt<SCH, RIE, R, float>(P, s);
+            return;
+        }
+    }
+    launch_sweep_pack_rt<SCH, RIE, R, double>(P, s);
 }
 
 template <int SCH, int RIE>
 void launch_sweep_pack_sr(int R, const PackLaunch &P, hipStream_t s)
 {
     if (R == 2) launch_sweep_pack_r<SCH, RIE, 2>(P, s);
+    else if (R == 3) launch_sweep_pack_r<SCH, RIE, 3>(P, s);
     else launch_sweep_pack_r<SCH, RIE, 4>(P, s);
 }
 
@@ -1504,7 +1532,7 @@ static void ctx_geometry(nuc_ctx *x, const nuc_config *cfg)
         x->tsv[d] = vis_tilesize(cfg->scheme[d], cfg->vis_scheme[d]);
         x->seg[d] = segment_length(x->n[d], x->nloc[d], x->np[d], cfg->scheme[d]);
         x->nseg[d] = x->n[d] / x->seg[d];
-        x->cpt[d] = x->nseg[d] > 1 ? (x->seg[d] <= 255 ? 8 : 16) : rows_for(x->n[d]);
+        x->cpt[d] = x->nseg[d] > 1 ? (x->seg[d] <= 255 ? 8 : 16) : sweep_rows(x->n[d]);
     }
 }
 
@@ -3429,7 +3457,7 @@ static bool packable(const nuc_ctx *x)
 
 static bool sweep_full(const nuc_ctx *x, int d)
 {
-    return x->n[d] == 32 * x->cpt[d] && (x->N / x->n[d]) % SWEEP_LPB == 0;
+    return x->cpt[d] % 2 == 0 && x->n[d] == 32 * x->cpt[d] && (x->N / x->n[d]) % SWEEP_LPB == 0;
 }
 
 // same kernels for every launch of a step: scheme, splitting, model,

@@ -53,10 +53,10 @@ MEMBER_SETS = {
         (_cfg((24, 24, 24), scheme=G.WENO5Z, riemann=G.AUSM, vis=G.CD8, precision=G.MIXED, mach=0.4), "hit", 3),
         (_cfg((36, 30, 28), scheme=G.WENO5Z, riemann=G.AUSM, vis=G.CD8, precision=G.MIXED, mach=0.15), "tgv", 4),
     ],
-    # R = 4 (65..128 cells), WENO5-JS, plus a rank grid (closure placement per member)
+    # R = 4 (97..128 cells), WENO5-JS, plus a rank grid (closure placement per member)
     "weno5js-r4": [
-        (_cfg((96, 80, 72), scheme=G.WENO5JS), "hit", 5),
-        (_cfg((100, 72, 66), scheme=G.WENO5JS, npr=(2, 1, 1), mach=0.35), "hit", 6),
+        (_cfg((112, 100, 120), scheme=G.WENO5JS), "hit", 5),
+        (_cfg((100, 104, 98), scheme=G.WENO5JS, npr=(2, 1, 1), mach=0.35), "hit", 6),
     ],
     # Euler, hccs (7-point), lines of exactly 64 cells (no tail checks)
     "hccs-euler-full": [
@@ -72,9 +72,14 @@ MEMBER_SETS = {
         (_cfg((32, 32, 32), scheme=G.WENO3JS, precision=G.MIXED), "hit", 11),
         (_cfg((40, 34, 36), scheme=G.WENO3JS, precision=G.MIXED, mach=0.35), "hit", 12),
     ],
-    "hccs78-r4": [
+    # R = 3 (65..96 cells, the odd chunk length)
+    "hccs78-r3": [
         (_cfg((72, 66, 80), scheme=G.HCCS78, vis=G.CD8), "hit", 13),
         (_cfg((96, 70, 68), scheme=G.HCCS78, vis=G.CD8, npr=(1, 2, 1)), "hit", 14),
+    ],
+    "crweno5-r3-mixed": [
+        (_cfg((96, 96, 96), precision=G.MIXED), "hit", 15),
+        (_cfg((80, 90, 70), precision=G.MIXED, mach=0.3), "tgv", 16),
     ],
 }
 

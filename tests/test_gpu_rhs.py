@@ -270,7 +270,7 @@ def test_forced_rhs_and_ten_steps_match_oracle(case):
 
 
 # Long lines: every chunk length R (lane h holds R consecutive interfaces,
-# R in {2, 4, 6, 8, 16} from n up to 64, 128, 192, 256, 512) in every sweep
+# R in {2, 3, 4, 5, 6, 8, 16} for n up to 64, 96, 128, 160, 192, 256, 512) in every sweep
 # direction, both with n = 32 R (the kernels without tail checks) and with a
 # ragged tail, with closure rows inside the line (rank grids).  Thin boxes keep
 # the oracle's work small; the short sides are the minimum the halos allow.
@@ -293,6 +293,15 @@ LONG_CASES = [
     ((12, 160, 12), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
     ((12, 12, 100), (1, 1, 2), G.UPWIND1ST, G.CD8, G.LF, G.NAVIERSTOKES3D),
     ((12, 12, 256), (1, 1, 1), G.HCCS77, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    # odd chunk lengths: R = 3 (65-96 cells) and R = 5 (129-160 cells)
+    ((96, 12, 12), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 80, 12), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 12, 150), (1, 1, 1), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((136, 12, 12), (2, 1, 1), G.CRWENO5, G.CD8, G.AUSMP, G.NAVIERSTOKES3D),
+    ((12, 96, 12), (1, 1, 1), G.WENO7JS, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 12, 144), (1, 1, 1), G.HCCS77, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((72, 12, 12), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((160, 12, 12), (1, 1, 1), G.WENO3JS, G.CD2, G.LF, G.EULER3D),
 ]
 
 
