@@ -1,5 +1,5 @@
-"""Per-phase instruction counts of one direction sweep (k_sweep), from the
-gfx950 assembly of an instrumented copy of this directory's sources.
+"""Per-phase instruction counts of one direction sweep (k_sweep, nuc_sweep.h),
+from the gfx950 assembly of an instrumented copy of this directory's sources.
 
 The copy (in a temporary directory; the sources here are only read) differs
 from the product kernel in three ways, none of which changes what a phase
@@ -100,23 +100,17 @@ def edit(text, edits, what, every=False):
 
 
 def build_tu(tmp, inst):
-    src = open(os.path.join(HERE, "nuc3d.hip")).read()
-    s1 = src.index("enum { EPI_NO")
-    e1 = src.index("\n};\n", src.index("struct SweepArgs {")) + 4
-    s2 = src.rfind("template", 0, src.index("__device__ __forceinline__ void split_params"))
-    e2 = src.index("struct ResArgs")
-    body = src[s2:e2]
-    body = edit(body, FLAGS, "nuc3d.hip", every=True)
-    body = edit(body, SWEEP_EDITS, "nuc3d.hip")
-    tu = (PH + '#include "nuc_device.h"\n#include "nuc_lines.h"\n#include <hip/hip_runtime.h>\n#include <type_traits>\n'
-          'using namespace nucdev;\n#define GMEM __attribute__((address_space(1)))\nnamespace X {\n'
-          + src[s1:e1] + "constexpr int kSweepOrder[3] = {2, 1, 0};\n" + body + "}\nusing namespace X;\n"
-          f"template __global__ void X::k_sweep<{inst}>(X::SweepArgs);\n")
     for h in os.listdir(HERE):
         if h.endswith(".h"):
             shutil.copy(os.path.join(HERE, h), tmp)
+    sweep = open(os.path.join(HERE, "nuc_sweep.h")).read()
+    sweep = edit(sweep, FLAGS, "nuc_sweep.h", every=True)
+    sweep = edit(sweep, SWEEP_EDITS, "nuc_sweep.h")
+    open(os.path.join(tmp, "nuc_sweep.h"), "w").write(PH + sweep)
     lines = open(os.path.join(tmp, "nuc_lines.h")).read()
     open(os.path.join(tmp, "nuc_lines.h"), "w").write(PH + edit(lines, LINES_EDITS, "nuc_lines.h"))
+    tu = ('#include "nuc_sweep.h"\n'
+          f"template __global__ void nucsweep::k_sweep<{inst}>(nucsweep::SweepArgs);\n")
     path = os.path.join(tmp, "phase.hip")
     open(path, "w").write(tu)
     return path
@@ -124,7 +118,7 @@ def build_tu(tmp, inst):
 
 def count(asm):
     L = asm.split("\n")
-    start = next(i for i, l in enumerate(L) if re.match(r"^_ZN1X7k_sweep\S*:", l))
+    start = next(i for i, l in enumerate(L) if re.match(r"^_ZN8nucsweep7k_sweep\S*:", l))
     end = next(i for i in range(start, len(L)) if "s_endpgm" in L[i])
     cnt = collections.defaultdict(collections.Counter)
     cur = "entry"

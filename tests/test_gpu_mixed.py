@@ -121,7 +121,8 @@ def test_mixed_close_to_fp64_every_scheme(case):
 
 @pytest.mark.parametrize("n,scheme,riemann", [((256, 12, 12), G.CRWENO5, G.LF), ((12, 256, 12), G.WENO5JS, G.LF),
                                               ((12, 12, 512), G.WENO5Z, G.AUSM), ((192, 12, 12), G.CRWENO5, G.AUSM),
-                                              ((160, 12, 12), G.CRWENO5, G.LF), ((12, 96, 12), G.WENO5Z, G.AUSM)],
+                                              ((160, 12, 12), G.CRWENO5, G.LF), ((12, 96, 12), G.WENO5Z, G.AUSM),
+                                              ((384, 12, 12), G.CRWENO5, G.LF), ((12, 224, 12), G.WENO5JS, G.LF)],
                          ids=lambda v: str(v))
 def test_mixed_long_lines_state_error(n, scheme, riemann):
     """The FP32 line buffers and stencils at every long chunk length: the mixed

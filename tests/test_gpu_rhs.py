@@ -270,7 +270,7 @@ def test_forced_rhs_and_ten_steps_match_oracle(case):
 
 
 # Long lines: every chunk length R (lane h holds R consecutive interfaces,
-# R in {2, 3, 4, 5, 6, 8, 16} for n up to 64, 96, 128, 160, 192, 256, 512) in every sweep
+# R in {2, 3, 4, 5, 6, 8, 12, 16} for n up to 64, 96, 128, 160, 192, 256, 384, 512) in every sweep
 # direction, both with n = 32 R (the kernels without tail checks) and with a
 # ragged tail, with closure rows inside the line (rank grids).  Thin boxes keep
 # the oracle's work small; the short sides are the minimum the halos allow.
@@ -302,6 +302,13 @@ LONG_CASES = [
     ((12, 12, 144), (1, 1, 1), G.HCCS77, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
     ((72, 12, 12), (1, 1, 1), G.HCCS78, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
     ((160, 12, 12), (1, 1, 1), G.WENO3JS, G.CD2, G.LF, G.EULER3D),
+    # 193-224 cells (R = 8 with a tail) and R = 12 (289-384 cells; 384 without tail checks)
+    ((224, 12, 12), (1, 1, 1), G.CRWENO5, G.CD8, G.LF, G.NAVIERSTOKES3D),
+    ((12, 200, 12), (1, 1, 1), G.WENO5Z, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((384, 12, 12), (1, 1, 1), G.WENO5JS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 12, 320), (1, 1, 1), G.WENO7JS, G.CD8, G.AUSM, G.NAVIERSTOKES3D),
+    ((300, 12, 12), (3, 1, 1), G.HCCS, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
+    ((12, 352, 12), (1, 1, 1), G.HCCS77, G.CD6, G.AUSM, G.NAVIERSTOKES3D),
 ]
 
 
@@ -343,10 +350,11 @@ def test_long_lines_ten_steps_match_oracle(case):
 
 # The bar without any conditioning allowance, on the kernel variants the
 # 256^3 workload runs (chunks of R = 8 interfaces per lane on 256-cell lines,
-# the FULL variant without tail checks) and on 512-cell lines (R = 16): the
+# the FULL variant without tail checks), on 384-cell lines (R = 12, FULL) and
+# on 512-cell lines (R = 16): the
 # workload's own field -- seeded random-phase decaying turbulence,
 # E(k) ~ k^4 exp(-2 (k/4)^2), Mach_t 0.3, Re 200 -- on boxes whose short
-# sides (32 cells; 16 for the 512-cell lines) still resolve the spectrum, so
+# sides (32 cells; 16 for the 384- and 512-cell lines) still resolve the spectrum, so
 # the nonlinear weights sit near their smooth values and the compact rows stay
 # diagonally dominant.  CRWENO5 (src/crweno5.cpp:83-220) with LF and cd8 in
 # each sweep direction, at CFL 0.1; WENO5-Z (src/weno5z.cpp) with AUSM on
@@ -363,6 +371,7 @@ STRICT_CASES = [
     ((32, 32, 256), G.CRWENO5, G.CD8, G.LF),
     ((512, 16, 16), G.WENO5Z, G.CD6, G.AUSM),
     ((16, 512, 16), G.WENO5Z, G.CD6, G.AUSM),
+    ((384, 16, 16), G.CRWENO5, G.CD8, G.LF),     # R = 12 without tail checks
 ]
 
 
