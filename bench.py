@@ -41,6 +41,10 @@ state and the RHS stored in FP32), timed and compared the same way.
 weno5js: the same box and initial state with WENO5-JS in place of CRWENO5
 (FP64 and mixed), timed the same way; a secondary figure.
 
+configs: rates of the other named workloads (Taylor-Green 64^3 FP64;
+forced turbulence 512^3 hccs/AUSM mixed and FP64; the Shu-Osher box
+512x128x128 FP64, mixed and FP32), a few timed steps each.
+
 serving: the serving stream -- 256 independent boxes (Taylor-Green or decaying
 turbulence, N log-uniform in [32, 192] per side, Mach U(0.1,0.5),
 Re U(400,3200), scheme uniform over WENO5-JS/WENO5-Z/CRWENO5, LF, cd6,
@@ -177,6 +181,56 @@ def stage_stats(G, cfg, q, steps, warmup):
         q_end = ctx.download()
         PCIE["download_ms"] = (time.perf_counter() - t_dn) * 1e3
     return t1 - t0, st, prof, q_end, dt
+
+
+def config_rates(G, steps=3, warmup=1):
+    """Rates of the other named workloads (Mcell-updates/s per RK3 stage, state
+    resident in HBM, initial states generated on the device or uploaded before
+    timing): Taylor-Green 64^3 (WENO5-JS/LF/cd6, FP64, 200 steps, one captured
+    graph per step), forced turbulence 512^3 (hccs/AUSM/cd6, Const forcing on
+    1 < |k| < 2, mixed and FP64) and the Shu-Osher box 512x128x128
+    (WENO5-Z/AUSM/cd6, CFL 0.4, FP64, mixed and FP32)."""
+    import numpy as np
+    out = {}
+
+    def timed(cfg, init, nsteps, nwarm, graph=False):
+        with G.Context(cfg) as ctx:
+            init(ctx)
+            ctx.autotune()
+            if graph:
+                ctx.set_graph(True)
+            ctx.step_async(nwarm)
+            ctx.sync()
+            t0 = time.perf_counter()
+            ctx.step_async(nsteps)
+            ctx.sync()
+            t1 = time.perf_counter()
+            ok = bool(np.isfinite(ctx.stats()).all())
+        cells = cfg.nx * cfg.ny * cfg.nz
+        return {"value": round(cells * 3 * nsteps / (t1 - t0) / 1e6, 1),
+                "ms_per_step": round((t1 - t0) / nsteps * 1e3, 4), "steps": nsteps, "finite": ok}
+
+    print("bench: Taylor-Green 64^3", file=sys.stderr, flush=True)
+    c1 = G.make_config(n=(64, 64, 64), scheme=G.WENO5JS, vis=G.CD6, riemann=G.LF, model=G.NAVIERSTOKES3D,
+                       viscosity=G.SUTHERLAND, mach=0.1, reynolds=1600.0, prandtl=0.72, cfl=0.5)
+    out["tgv_64_weno5js_fp64"] = timed(c1, lambda c: c.init_tgv_device(), 200, 5, graph=True)
+    for label, prec in (("mixed", G.MIXED), ("fp64", G.FP64)):
+        print(f"bench: forced 512^3 hccs {label}", file=sys.stderr, flush=True)
+        c3 = G.make_config(n=(512, 512, 512), scheme=G.HCCS, vis=G.CD6, riemann=G.AUSM, model=G.NAVIERSTOKES3D,
+                           viscosity=G.SUTHERLAND, mach=0.5 / np.sqrt(3.0), reynolds=200.0, prandtl=0.72, cfl=0.5,
+                           forcing=G.FORCING_CONST, forcing_k=1.5, forcing_amp=0.05, forcing_amp_hel=0.0,
+                           precision=prec)
+        out[f"forced_512_hccs_{label}"] = timed(c3, lambda c: c.init_hit_device(k0=4.0, u_rms=1.0, seed=20261502),
+                                                steps, warmup)
+    n4 = (512, 128, 128)
+    kw4 = dict(n=n4, L=(10.0, 2.5, 2.5), scheme=G.WENO5Z, vis=G.CD6, riemann=G.AUSM, model=G.NAVIERSTOKES3D,
+               viscosity=G.SUTHERLAND, mach=1.0 / np.sqrt(1.4), reynolds=2000.0, prandtl=0.72, cfl=0.4)
+    q4 = G.init_shu_osher(G.make_config(**kw4), pert=0.05, seed=20261602)
+    for label, prec in (("fp64", G.FP64), ("mixed", G.MIXED), ("fp32", G.FP32)):
+        print(f"bench: Shu-Osher 512x128x128 {label}", file=sys.stderr, flush=True)
+        out[f"shu_osher_512x128x128_{label}"] = timed(G.make_config(precision=prec, **kw4), lambda c: c.upload(q4),
+                                                      5 * steps, warmup)
+    return out
 
 
 STAT_NAMES = ["ux", "uy", "uz", "vx", "vy", "vz", "wx", "wy", "wz", "om_x", "om_y", "om_z", "enstrophy", "Q",
@@ -361,6 +415,7 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-weno5", action="store_true", help="skip the WENO5-JS secondary figure")
+    ap.add_argument("--no-configs", action="store_true", help="skip the other named workloads' rates")
     ap.add_argument("--serve-jobs", type=int, default=SERVE_JOBS,
                     help="jobs of the serving stream (default: the full 256-job stream)")
     ap.add_argument("--serve-steps", type=int, default=SERVE_STEPS)
@@ -456,6 +511,8 @@ def main():
                             "ms_per_step": round(wsecs / args.steps * 1e3, 4),
                             "dominant_kernel": wname, "avg_launch_ms": round(wtot / max(wl, 1), 4)}
 
+    configs = None if args.no_configs else config_rates(G)
+
     serving = None
     if not args.no_serving:
         print("bench: serving stream", file=sys.stderr, flush=True)
@@ -494,6 +551,7 @@ def main():
         "mixed": mixed,
         "fp32": fp32,
         "weno5js": weno5,
+        "configs": configs,
         "serving": serving,
     }
     print(json.dumps(out))
