@@ -715,6 +715,11 @@ struct nuc_ctx {
     bool forcing = false;
     bool sources = false;   // ABC/TG forcing or rotating-frame sources (k_source)
     bool mixed = false;     // NUC_MIXED (and NUC_FP32): FP32 stencils, FP64 weights/solves/RK3
+    // viscous fluxes stored FP32 (k_gradq<float>): the mixed paths when some
+    // direction runs FP32 stencils.  hccs and hccs77 keep FP64 stencils, and
+    // their FP64 sweeps read FP32 fluxes more slowly than they save in the
+    // gradient kernel (forced 512^3 hccs: 277 against 257 ms per step)
+    bool fv32 = false;
     bool fp32 = false;      // NUC_FP32: state and RHS stored in FP32 (the first half of each buffer)
     double *q64 = nullptr;  // NUC_FP32: the state widened to FP64 for I/O, statistics, post-processing
     // planes per workgroup of the gradient kernel (nuc_autotune; 0: grad_planes)
@@ -857,12 +862,22 @@ static int validate(const nuc_config *c, std::string &err)
     return NUC_OK;
 }
 
+// whether the viscous fluxes are stored FP32: a mixed precision with FP32
+// stencils in some direction (hccs and hccs77 keep FP64 stencils)
+static bool fv_fp32(const nuc_config *c)
+{
+    if (c->precision == NUC_FP64) return false;
+    for (int d = 0; d < 3; d++)
+        if (c->scheme[d] != NUC_HCCS && c->scheme[d] != NUC_HCCS77) return true;
+    return false;
+}
+
 extern "C" long long nuc_footprint(const nuc_config *c)
 {
     long long N = (long long)c->nx * c->ny * c->nz;
     // 3 states + rhs (FP32 with NUC_FP32, and its FP64 view); 9 viscous flux
-    // fields (Navier-Stokes; FP32 when the stencils are); u,v,w,T (forcing)
-    const long long es = c->precision == NUC_FP32 ? 4 : 8, fes = c->precision == NUC_FP64 ? 8 : 4;
+    // fields (Navier-Stokes; FP32 when some direction's stencils are); u,v,w,T (forcing)
+    const long long es = c->precision == NUC_FP32 ? 4 : 8, fes = fv_fp32(c) ? 4 : 8;
     long long bytes = 20 * N * es + (c->model == NUC_NAVIERSTOKES3D ? 9 * N * fes : 0);
     if (c->precision == NUC_FP32) bytes += 5 * N * 8;
     if (c->forcing == NUC_FORCING_CONST) bytes += 4 * N * 8;
@@ -933,8 +948,9 @@ extern "C" int nuc_create(const nuc_config *cfg, nuc_ctx **out)
     x->sources = cfg->forcing == NUC_FORCING_ABC || cfg->forcing == NUC_FORCING_TG || cfg->coriolis || cfg->centrifugal;
     x->mixed = (cfg->precision != NUC_FP64);
     x->fp32 = (cfg->precision == NUC_FP32);
+    x->fv32 = fv_fp32(cfg);
     // element bytes of the state/rhs fields and of the viscous-flux fields
-    const size_t es = x->fp32 ? sizeof(float) : sizeof(double), fes = x->mixed ? sizeof(float) : sizeof(double);
+    const size_t es = x->fp32 ? sizeof(float) : sizeof(double), fes = x->fv32 ? sizeof(float) : sizeof(double);
     for (int b = 0; b < 3; b++)
         if (!dalloc(&x->q[b], 5 * N * es)) return fail("device allocation (state)");
     if (!dalloc(&x->rhs, 5 * N * es)) return fail("device allocation (rhs)");
@@ -1014,7 +1030,7 @@ static void *rhs_field(nuc_ctx *x, int v)
 // viscous-flux component o (FP32 fields, k_gradq<float>, when the stencils are FP32)
 static void *fv_field(const nuc_ctx *x, int o)
 {
-    return x->mixed ? (void *)(reinterpret_cast<float *>(x->fv) + (size_t)o * x->N) : (void *)(x->fv + (size_t)o * x->N);
+    return x->fv32 ? (void *)(reinterpret_cast<float *>(x->fv) + (size_t)o * x->N) : (void *)(x->fv + (size_t)o * x->N);
 }
 
 // The current state in FP64 for the statistics, post-processing and
@@ -1130,7 +1146,7 @@ static void stage_args(nuc_ctx *x, int in, int stage, int outb, bool epilogue, S
         S.accum = (first && (x->forcing || x->sources)) ? 1 : 0;
         S.mixed = x->mixed ? 1 : 0;
         S.fp32 = x->fp32 ? 1 : 0;
-        S.fv_f32 = x->mixed ? 1 : 0;   // k_gradq<float> in the mixed path
+        S.fv_f32 = x->fv32 ? 1 : 0;   // k_gradq<float>
         S.fmean = (last && x->forcing) ? gp(x->scal + 4) : nullptr;
         S.epilogue = (last && epilogue) ? 1 : 0;
         if (S.epilogue) {
@@ -1222,7 +1238,7 @@ static SourceArgs source_args(nuc_ctx *x, int in, bool accumulate)
 static void launch_gradq(nuc_ctx *x, dim3 grid, const GradArgs &G)
 {
     if (x->fp32) hipLaunchKernelGGL((k_gradq<float, float>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
-    else if (x->mixed) hipLaunchKernelGGL((k_gradq<float, double>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
+    else if (x->fv32) hipLaunchKernelGGL((k_gradq<float, double>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
     else hipLaunchKernelGGL((k_gradq<double, double>), grid, dim3(GTX, GTY, 1), 0, x->stream, G);
 }
 
@@ -1312,7 +1328,7 @@ extern "C" int nuc_autotune(nuc_ctx *x)
 {
     if (!x) return NUC_EINVAL;
     if (x->c.model != NUC_NAVIERSTOKES3D) return NUC_OK;   // no gradient kernel
-    const auto key = std::make_tuple(x->n[0], x->n[1], x->n[2], (int)x->c.precision);
+    const auto key = std::make_tuple(x->n[0], x->n[1], x->n[2], (int)x->c.precision * 2 + (x->fv32 ? 1 : 0));
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
         auto it = g_tune_gkz.find(key);
@@ -1563,6 +1579,7 @@ extern "C" int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, doubl
     x.stream = static_cast<hipStream_t>(stream);
     x.sources = cfg->forcing == NUC_FORCING_ABC || cfg->forcing == NUC_FORCING_TG || cfg->coriolis || cfg->centrifugal;
     x.mixed = (cfg->precision == NUC_MIXED);
+    x.fv32 = fv_fp32(cfg);
     x.q[0] = const_cast<double *>(q_dev);
     x.rhs = rhs_dev;
     x.fv = base;
@@ -1570,7 +1587,7 @@ extern "C" int nuc_rhs_noalloc(const nuc_config *cfg, const double *q_dev, doubl
     x.scal = base + NOALLOC_FV + NOALLOC_LAM;
     {
         std::lock_guard<std::mutex> lk(g_tune_mu);
-        auto it = g_tune_gkz.find(std::make_tuple(x.n[0], x.n[1], x.n[2], (int)cfg->precision));
+        auto it = g_tune_gkz.find(std::make_tuple(x.n[0], x.n[1], x.n[2], (int)cfg->precision * 2 + (x.fv32 ? 1 : 0)));
         if (it != g_tune_gkz.end()) x.gkz = it->second;
     }
     // the array keeps the last call's contents: the eigenvalue maxima start
@@ -2844,7 +2861,7 @@ struct nuc_pack {
     const ResArgs *res = nullptr;
     const int *prim_off = nullptr, *grad_off = nullptr, *sw_off[3] = {nullptr, nullptr, nullptr};
     long prim_total = 0, grad_total = 0, sw_total[3] = {0, 0, 0};
-    bool ns = false, mixed = false;
+    bool ns = false, mixed = false, fv32 = false;
     int scheme[3] = {0, 0, 0}, riemann = 0, cpt[3] = {0, 0, 0};
     bool full[3] = {false, false, false};
     bool use_graph = false;
@@ -2906,6 +2923,7 @@ extern "C" int nuc_pack_create(nuc_ctx *const *members, int n, int use_graph, nu
     const nuc_ctx *x0 = members[0];
     p->ns = (x0->c.model == NUC_NAVIERSTOKES3D);
     p->mixed = x0->mixed;
+    p->fv32 = x0->fv32;
     p->riemann = x0->c.riemann;
     for (int d = 0; d < 3; d++) { p->scheme[d] = x0->c.scheme[d]; p->cpt[d] = x0->cpt[d]; p->full[d] = sweep_full(x0, d); }
     p->use_graph = use_graph != 0;
@@ -2998,7 +3016,7 @@ static void pack_enqueue_step(nuc_pack *p)
             hipLaunchKernelGGL(k_dt_pack, dim3(nj), dim3(64), 0, p->stream, p->dt);
         }
         if (p->ns) {
-            if (p->mixed)
+            if (p->fv32)
                 hipLaunchKernelGGL(k_gradq_pack<float>, dim3((unsigned)p->grad_total), dim3(GTX, GTY, 1), 0, p->stream,
                                    p->grad[st], p->grad_off, nj);
             else

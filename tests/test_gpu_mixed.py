@@ -148,3 +148,29 @@ def test_mixed_long_lines_state_error(n, scheme, riemann):
     err = np.abs(a - b).max(axis=1) / scale
     print("mixed vs FP64 rel Linf", err, flush=True)
     assert np.all(err <= 1e-5), err
+
+
+def test_mixed_hccs_only_context_is_fp64():
+    """hccs and hccs77 keep FP64 stencils in the mixed path; a context whose
+    directions all use them keeps its viscous fluxes in FP64 too, so its steps
+    equal the FP64 context's bit for bit.  With one FP32-stencil direction the
+    viscous fluxes are FP32 again (and the result differs at rounding level)."""
+    print("mixed hccs-only context", flush=True)
+    n = 24
+    for scheme in (G.HCCS, (G.HCCS77, G.HCCS, G.HCCS77)):
+        kw = dict(scheme=scheme, vis=G.CD6, riemann=G.AUSM)
+        c64, cmx = _hit_cfg(n, G.FP64, **kw), _hit_cfg(n, G.MIXED, **kw)
+        q = G.init_hit(c64, k0=3.0, u_rms=1.0, seed=17)
+        a, _ = _run(c64, q, 3)
+        b, _ = _run(cmx, q, 3)
+        assert np.array_equal(a, b), np.abs(a - b).max()
+    assert G.footprint(_hit_cfg(n, G.MIXED, scheme=G.HCCS)) == G.footprint(_hit_cfg(n, G.FP64, scheme=G.HCCS))
+    kw = dict(scheme=(G.HCCS, G.CRWENO5, G.HCCS), vis=G.CD6, riemann=G.AUSM)
+    c64, cmx = _hit_cfg(n, G.FP64, **kw), _hit_cfg(n, G.MIXED, **kw)
+    q = G.init_hit(c64, k0=3.0, u_rms=1.0, seed=17)
+    a, _ = _run(c64, q, 3)
+    b, _ = _run(cmx, q, 3)
+    N = n ** 3
+    err = np.abs(a - b).reshape(5, N).max(axis=1) / np.abs(a).reshape(5, N).max(axis=1)
+    print("mixed hccs/crweno5/hccs vs FP64", err, flush=True)
+    assert not np.array_equal(a, b) and np.all(err <= 1e-5), err
