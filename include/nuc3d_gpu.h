@@ -46,7 +46,8 @@ enum nuc_viscosity { NUC_SUTHERLAND = 0, NUC_CONSTANT = 1 };
 enum nuc_forcing { NUC_FORCING_NONE = 0, NUC_FORCING_CONST = 1, NUC_FORCING_ABC = 2, NUC_FORCING_TG = 3 };
 /* precision.  NUC_FP64: everything FP64.  NUC_MIXED: FP32 split fluxes,
  * stencils and viscous fluxes, FP64 conserved state, tridiagonal solves, RHS
- * accumulation and RK3 update.  NUC_FP32: the mixed arithmetic with the
+ * accumulation and RK3 update (hccs and hccs77 keep FP64 stencils; a context
+ * that uses only them runs FP64 throughout).  NUC_FP32: the mixed arithmetic with the
  * conserved state and the RHS also stored in FP32 (half their memory
  * traffic); uploads, downloads, statistics, spectra and averages see the
  * state widened to FP64.  NUC_FP32 does not take hccs or hccs77 (their
