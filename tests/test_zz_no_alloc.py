@@ -110,3 +110,21 @@ def test_noalloc_rejects_oversize_and_const_forcing():
     frc = G.make_config(n=(16, 16, 16), forcing=G.FORCING_CONST, forcing_amp=0.1, forcing_k=2.0)
     rc = L.nuc_rhs_noalloc(ctypes.byref(frc), x.ctypes.data, x.ctypes.data, None, None)
     assert rc == 1 and "Const forcing" in L.nuc_last_error(None).decode()
+
+
+def test_noalloc_segmented_long_line_equals_context_rhs():
+    """A line above 512 cells (swept in segments) through the no-allocation
+    entry point: bit-identical to the context's RHS."""
+    h = _hip()
+    kw = dict(n=(1024, 12, 12), L=(2 * np.pi, 2 * np.pi * 12 / 1024, 2 * np.pi * 12 / 1024), scheme=G.WENO5Z,
+              vis=G.CD8, riemann=G.AUSM, model=G.NAVIERSTOKES3D)
+    print(f"no-alloc segmented case: {kw}", flush=True)
+    cfg = G.make_config(**kw)
+    q = _state(kw, 31)
+    with G.Context(cfg) as ctx:
+        ctx.upload(q)
+        r_ref, dt_ref = ctx.rhs()
+    r, dt = _noalloc(h, cfg, q, None)
+    assert np.isfinite(r).all()
+    assert np.array_equal(r, r_ref), np.abs(r - r_ref).max()
+    assert dt == dt_ref
