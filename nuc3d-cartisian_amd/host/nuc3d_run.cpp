@@ -11,7 +11,8 @@
 // src/block.cpp:238-266), with the averaged Tecplot files when AveragedSaveType
 // is 1 and of solveQ's per-point fields when SimutaneousSaveType is set.
 //
-// usage: nuc3d_run [inp_dir] [--seed S]
+// usage: nuc3d_run [inp_dir] [--seed S] [--precision fp64|mixed|fp32]
+// (the inp files select no precision: the reference computes in FP64, the default)
 #include "nuc3d_host.hpp"
 
 #include <algorithm>
@@ -23,11 +24,12 @@
 
 int main(int argc, char **argv)
 {
-    std::string dir = "inp";
+    std::string dir = "inp", precision = "fp64";
     unsigned long long seed = 20261202ull;
     for (int a = 1; a < argc; a++) {
         std::string s = argv[a];
         if (s == "--seed" && a + 1 < argc) seed = std::strtoull(argv[++a], nullptr, 10);
+        else if (s == "--precision" && a + 1 < argc) precision = argv[++a];
         else dir = s;
     }
     nuc_config cfg;
@@ -35,6 +37,13 @@ int main(int argc, char **argv)
     std::string err;
     if (nuc3d::read_inp(dir, cfg, rc, err) != NUC_OK) {
         fprintf(stderr, "%s\n", err.c_str());
+        return 1;
+    }
+    if (precision == "fp64") cfg.precision = NUC_FP64;
+    else if (precision == "mixed") cfg.precision = NUC_MIXED;
+    else if (precision == "fp32") cfg.precision = NUC_FP32;
+    else {
+        fprintf(stderr, "--precision takes fp64, mixed or fp32, not %s\n", precision.c_str());
         return 1;
     }
     const long N = (long)cfg.nx * cfg.ny * cfg.nz;

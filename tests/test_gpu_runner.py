@@ -117,3 +117,38 @@ def test_runner_restart_reproduces_the_uninterrupted_run(tmp_path):
         assert (aver / f"Averaged_step_4_id_{r}.bin").read_bytes() == keep_avg[r]
     lines = open(tmp_path / "enstrophy.dat").read().splitlines()
     assert lines[4:] == lines[2:4]   # steps 3 and 4 again, the same text
+
+
+def test_runner_precision_option(tmp_path):
+    """--precision selects the context's precision (the inp files carry none;
+    FP64 is the default): after 4 steps the mixed and FP32 runs' restart files
+    hold states within 1e-5 of the FP64 run's (relative L-infinity per variable,
+    momentum against its largest component), and an unknown name is refused."""
+    if not os.path.exists(RUNNER):
+        pytest.fail("nuc3d_run is not built (make -C nuc3d-cartisian_amd)")
+    states = {}
+    for prec in ("fp64", "mixed", "fp32"):
+        d = tmp_path / prec
+        for sub in ("flowData", "AveragedFlowData", "flowDataTemp"):
+            (d / sub).mkdir(parents=True)
+        _write_inp(d / "inp", 0, 4)
+        print(f"nuc3d_run --precision {prec}", flush=True)
+        r = subprocess.run([RUNNER, str(d / "inp"), "--precision", prec], cwd=d, capture_output=True, text=True,
+                           timeout=120)
+        assert r.returncode == 0, r.stderr
+        cfg, _, _ = G.config_from_inp(str(d / "inp"))
+        states[prec] = G.read_restart(cfg, d / "flowData", 4)[1]
+    N = 24 ** 3
+    a = states["fp64"].reshape(5, N)
+    # the Taylor-Green field starts with w = 0: momentum errors are measured
+    # against the largest momentum component
+    scale = np.abs(a).max(axis=1)
+    scale[1:4] = np.abs(a[1:4]).max()
+    for prec in ("mixed", "fp32"):
+        b = states[prec].reshape(5, N)
+        err = np.abs(a - b).max(axis=1) / scale
+        print(prec, "vs fp64", err, flush=True)
+        assert np.all(err <= 1e-5) and not np.array_equal(a, b), err
+    r = subprocess.run([RUNNER, str(tmp_path / "fp64" / "inp"), "--precision", "fp16"], cwd=tmp_path / "fp64",
+                       capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "--precision" in r.stderr
