@@ -89,7 +89,11 @@ typedef struct nuc_ctx nuc_ctx;
 /* ---- context --------------------------------------------------------------
  * Replaces the per-rank state the reference's block owns (src/block.cpp:143-187):
  * PDEData3d (Q, Q_Euler, Q_work, RHS), EulerData3D/NaiverStokesData3d work
- * fields and the halo buffers.  All device memory is owned by the context. */
+ * fields and the halo buffers.  All device memory is owned by the context.
+ * Grids: each size divisible by its rank count (src/block.cpp:56-58), at most
+ * 2^29 cells; lines above 512 cells need, for the explicit schemes, a divisor
+ * of the length in [64, 255], for the compact ones (CRWENO5, hccs, hccs77)
+ * rank blocks of at most 511 cells (else NUC_EINVAL). */
 int nuc_create(const nuc_config *cfg, nuc_ctx **out);
 int nuc_destroy(nuc_ctx *ctx);
 /* the message of the last failure on ctx; with ctx NULL, the message of the
